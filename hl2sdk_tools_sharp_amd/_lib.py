@@ -1,0 +1,124 @@
+"""ctypes binding of libvradhip.so (the C ABI declared in include/vradhip.h).
+
+The library is the only compute path: if it is missing this module raises,
+it never falls back to anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvradhip.so")
+
+VH_OK = 0
+VH_DECLINED = 1
+VH_EINVAL = 2
+VH_EDEVICE = 3
+VH_ETIMEOUT = 4
+VH_ECANCELED = 5
+VH_ENOTSUP = 6
+VH_ESTATE = 7
+
+STATUS_NAMES = {
+    VH_OK: "VH_OK",
+    VH_DECLINED: "VH_DECLINED",
+    VH_EINVAL: "VH_EINVAL",
+    VH_EDEVICE: "VH_EDEVICE",
+    VH_ETIMEOUT: "VH_ETIMEOUT",
+    VH_ECANCELED: "VH_ECANCELED",
+    VH_ENOTSUP: "VH_ENOTSUP",
+    VH_ESTATE: "VH_ESTATE",
+}
+
+
+class VhOpenOpts(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("self_test", ctypes.c_int32),
+        ("fail_stage", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class VhStats(ctypes.Structure):
+    _fields_ = [
+        ("rays_visibility", ctypes.c_int64),
+        ("rays_closest", ctypes.c_int64),
+        ("calls", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 5),
+    ]
+
+
+class VradHipError(RuntimeError):
+    """A libvradhip call failed; ``status`` is the VH_* code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+_lock = threading.Lock()
+_lib = None
+
+P = ctypes.c_void_p
+I32, I64, F32, F64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double
+
+# name -> (restype, argtypes); the single table of what this module binds.
+SIGNATURES = {
+    "vh_open": (ctypes.c_int, [ctypes.POINTER(VhOpenOpts), ctypes.POINTER(P)]),
+    "vh_close": (None, [P]),
+    "vh_last_error": (ctypes.c_char_p, []),
+    "vh_tracer_identity": (ctypes.c_char_p, [P]),
+    "vh_supports": (ctypes.c_int, [P, F32, I32, I32]),
+    "vh_scene_load": (ctypes.c_int, [P, P, P, P, I32]),
+    "vh_scene_info": (ctypes.c_int, [P, P, P, P]),
+    "vh_trace_visibility": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, I32, P, I64]),
+    "vh_trace_closest": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, P, P, I64]),
+    "vh_trace_point_lights": (ctypes.c_int, [P, P, I64, P, I32, F32, P, I64]),
+    "vh_trace_visibility_dev": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, I32, P, P]),
+    "vh_trace_closest_dev": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, P, P, P]),
+    "vh_trace_point_lights_dev": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
+    "vh_sync": (ctypes.c_int, [P, P]),
+    "vh_get_stats": (ctypes.c_int, [P, ctypes.POINTER(VhStats)]),
+    "vh_timing_enable": (ctypes.c_int, [P, I32]),
+    "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
+    "vh_count_enable": (ctypes.c_int, [P, I32]),
+    "vh_count_read": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+}
+
+VH_KERNEL_POINT_LIGHTS = 0
+VH_KERNEL_VISIBILITY = 1
+VH_KERNEL_CLOSEST = 2
+
+
+def lib():
+    """Loads libvradhip.so; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m hl2sdk_tools_sharp_amd.build` "
+                "(there is no CPU fallback)"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+        return L
+
+
+def last_error() -> str:
+    s = lib().vh_last_error()
+    return s.decode("utf-8", "replace") if s else ""
+
+
+def check(status: int) -> None:
+    if status != VH_OK:
+        raise VradHipError(status, last_error())

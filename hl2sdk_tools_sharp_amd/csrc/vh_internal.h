@@ -1,0 +1,121 @@
+// Internal declarations shared by the libvradhip.so sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vradhip.h"
+
+namespace vh {
+
+// Thread-local last error (vh_last_error).
+void set_error(const std::string &msg);
+void clear_error();
+
+// A device allocation that only grows; reused across calls so a batch does
+// not pay hipMalloc each time.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need, const char *what);
+    void release();
+};
+
+// Scene layout in HBM.
+//
+// BVH node, 64 bytes = 4 x float4, both children's boxes in the parent so one
+// node visit tests two boxes with one 64-byte load:
+//   q0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   q1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   q2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   q3 = (ref0, ref1, -, -) as int bits
+// A child ref >= 0 is an internal node index; ref < 0 is a leaf whose
+// triangles start at ~ref and run to the one carrying kTriLast; kEmptyRef is
+// a child with nothing in it (its box is empty, lo > hi).
+//
+// Triangle, 48 bytes = 3 x float4, the reference's intersection format
+// (Tracing/KdTree.cs:121-167) repacked for aligned loads:
+//   t0 = (nx, ny, nz, d)
+//   t1 = (e0, e1, e2, e3)
+//   t2 = (e4, e5, id bits, meta bits)
+//   meta = cs0 | cs1 << 2 | last-in-leaf << 3 | original index << 4
+constexpr int32_t kEmptyRef = (int32_t)0x80000000;
+constexpr uint32_t kTriLast = 8u;
+constexpr int kMaxDepth = 32;      // stack entries per lane; the builder caps depth here
+constexpr int kMaxOrigIndex = 1 << 27;
+
+struct HostScene {
+    std::vector<float> nodes;  // 16 floats per node
+    std::vector<float> tris;   // 12 floats per triangle
+    int32_t ntri = 0;
+    int32_t nnodes = 0;
+    int32_t depth = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    float pad = 0;
+};
+
+// Builds the intersection-format triangles and the BVH (vh_bvh.cpp).
+int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32_t n,
+                HostScene &out);
+
+}  // namespace vh
+
+struct vh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string identity;
+    // scene
+    vh::DevBuf nodes, tris;
+    int32_t nnodes = 0, ntri = 0, depth = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool have_scene = false;
+    // per-call staging
+    vh::DevBuf rays_buf, out_buf, out2_buf, pts_buf, lights_buf;
+    vh::DevBuf keys_a, keys_b, perm_a, perm_b, sort_tmp;
+    vh_stats stats{};
+    // kernel timing (vh_timing_*): event pairs per kernel, resolved on read
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[VH_KERNEL_COUNT];
+    std::vector<hipEvent_t> event_pool;
+    // work counters (vh_count_*): [node visits, triangle tests]
+    bool counting = false;
+    vh::DevBuf counters;
+};
+
+namespace vh {
+// Brackets one kernel launch with events when timing is on.
+struct KernelTimer {
+    vh_ctx *c;
+    int kernel;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(vh_ctx *ctx, int k, hipStream_t st);
+    void done();
+};
+}  // namespace vh
+
+namespace vh {
+// Kernel launchers (vh_trace.hip).
+int launch_visibility(vh_ctx *c, const float *d_rays, int32_t rw, float shared_reach, int64_t n,
+                      float tmin, int32_t skip_id, int32_t sky_passes, uint64_t *d_bits,
+                      hipStream_t s);
+int launch_closest(vh_ctx *c, const float *d_rays, int32_t rw, float shared_reach, int64_t n,
+                   float tmin, int32_t skip_id, int32_t *d_surface, float *d_fraction,
+                   hipStream_t s);
+int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const float *d_lights,
+                        int32_t nlights, float tmin, uint64_t *d_bits, hipStream_t s);
+}  // namespace vh
+
+#define VH_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            vh::set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+            return VH_EDEVICE;                                                           \
+        }                                                                                \
+    } while (0)

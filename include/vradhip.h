@@ -1,0 +1,169 @@
+/*
+ * vradhip.h -- C ABI of libvradhip.so, the MI355X (gfx950) backend for vrad's
+ * lighting hot path.
+ *
+ * The reference's host is C# (.NET); it reaches this library through
+ * P/Invoke.  Every signature uses plain pointers, sizes and status codes; no
+ * C++ exception and no torch type crosses it.  Each declaration names the
+ * reference interface it replaces (paths under the reference's src/).
+ *
+ * Conventions
+ *   - Every function returning int returns a VH_* status.  On failure the
+ *     message is available from vh_last_error() on the same thread.
+ *   - Host-pointer entry points (no _dev suffix) read the caller's arrays
+ *     during the call only and return when every output is written; no
+ *     pointer is retained (SourceSharp.MapTools.Gpu/VulkanRayTracer.cs:246-271).
+ *   - _dev entry points take device pointers (HBM-resident inputs), enqueue on
+ *     `stream` (a hipStream_t; NULL = the context's own stream) and return
+ *     before the work finishes; vh_sync() waits for it.
+ *   - Calls on one context are serialised by a per-context mutex; results are
+ *     a pure function of each call's inputs.
+ */
+#ifndef VRADHIP_H
+#define VRADHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Errors never throw across the seam; a device that cannot
+ * prove itself declines (SourceSharp.MapTools/Rad/GpuTracing.cs:28,
+ * SourceSharp.MapTools.Gpu/VulkanRayTracer.cs:706-719). */
+#define VH_OK 0
+#define VH_DECLINED 1  /* no usable device / self-test failed: use the CPU tracer */
+#define VH_EINVAL 2    /* bad argument: short output, bad record size (ArgumentException) */
+#define VH_EDEVICE 3   /* a HIP call failed after open */
+#define VH_ETIMEOUT 4
+#define VH_ECANCELED 5
+#define VH_ENOTSUP 6   /* options this kernel cannot honour (NotSupportedException) */
+#define VH_ESTATE 7    /* no scene loaded, or a closed context */
+
+/* Sky bit of a triangle id (SourceSharp.MapTools/Rad/TraceId.cs:31). */
+#define VH_TRACE_ID_SKY 0x01000000
+
+typedef struct vh_ctx vh_ctx;
+
+typedef struct vh_open_opts {
+    int32_t device;     /* HIP device ordinal; -1 = the current device */
+    int32_t self_test;  /* 1 = run the known-answer self-test at open (default) */
+    int32_t fail_stage; /* fault hook for tests: 0 = off; N = fail at open stage N
+                           (1 device query, 2 stream, 3 self-test) without using the GPU */
+    int32_t reserved;
+} vh_open_opts;
+
+/* Opens a context on one GPU.  Replaces IGpuTracerFactory.TryCreateAsync's
+ * device half: VulkanRayTracer.TryCreateAsync -> TryOne (open, construct,
+ * self-test), SourceSharp.MapTools.Gpu/VulkanRayTracer.cs:366,595-704.
+ * Returns VH_DECLINED with the reason in vh_last_error() when no device is
+ * usable; *out is then NULL. */
+int vh_open(const vh_open_opts *opts, vh_ctx **out);
+
+/* Waits for in-flight work, then frees the device buffers.  Replaces
+ * VulkanRayTracer.Dispose (SourceSharp.MapTools.Gpu/VulkanRayTracer.cs:1030-1041).
+ * NULL is ignored; a second close of the same pointer is the caller's bug. */
+void vh_close(vh_ctx *ctx);
+
+/* The last error message on this thread ("" if none). */
+const char *vh_last_error(void);
+
+/* IRayTracer.TracerIdentity: backend, device and kernel revision
+ * (SourceSharp.MapTools/Tracing/IRayTracer.cs:221-229). */
+const char *vh_tracer_identity(const vh_ctx *ctx);
+
+/* IRayTracer.Supports (SourceSharp.MapTools/Tracing/IRayTracer.cs:193-219):
+ * 1 when a visibility batch with these options gets the answer they ask
+ * for.  Skip ids and sky pass-through are honoured in-kernel; only a NaN
+ * minimum distance is refused. */
+int vh_supports(const vh_ctx *ctx, float min_distance, int32_t skip_id, int32_t sky_passes);
+
+/* Loads the shadow casters: n triangles as 9 floats (three vertices), the
+ * caller's id per triangle and FCACHETRI_* flags (may be NULL).  Builds the
+ * intersection-format triangles (KdTreeBuilder.ToIntersectionFormat, Correct
+ * side, SourceSharp.MapTools/Tracing/KdTreeBuilder.cs:1568-1612) and a BVH over
+ * them.  Replaces VulkanDevice.LoadScene (BLAS/TLAS,
+ * SourceSharp.MapTools.Gpu/Interop/VulkanDevice.cs:1380-1608) and
+ * ShadowCasterSet.BuildTracerAsync (SourceSharp.MapTools/Rad/ShadowCasterSet.cs:181). */
+int vh_scene_load(vh_ctx *ctx, const float *v9, const int32_t *ids, const uint8_t *flags,
+                  int32_t n);
+
+/* Scene statistics: triangles, BVH nodes, BVH depth. */
+int vh_scene_info(const vh_ctx *ctx, int32_t *triangles, int32_t *nodes, int32_t *depth);
+
+/* IRayTracer.TraceVisibilityAsync (SourceSharp.MapTools/Tracing/IRayTracer.cs:162-166).
+ * rays: n records of record_words floats (7: origin, direction, reach; 6:
+ * origin, direction with shared_reach for all; SourceSharp.MapTools.Gpu/RayRecord.cs:78-144).
+ * bits: LSB-first, bit i set when ray i is blocked strictly short of its
+ * end; needs bit_words >= (n + 63) / 64.  min_distance: RayTraceOptions.MinDistance;
+ * skip_id: RayTraceOptions.SkipId (-1 = none); sky_passes:
+ * RayTraceOptions.SkyDoesNotBlock. */
+int vh_trace_visibility(vh_ctx *ctx, const float *rays, int32_t record_words, float shared_reach,
+                        int64_t n, float min_distance, int32_t skip_id, int32_t sky_passes,
+                        uint64_t *bits, int64_t bit_words);
+
+/* IRayTracer.TraceClosestAsync (SourceSharp.MapTools/Tracing/IRayTracer.cs:187-191):
+ * per ray the caller's id of the nearest triangle with min_distance <= t <=
+ * reach and its fraction (t, divided by the reach unless the reach is 1;
+ * SourceSharp.MapTools.Gpu/SlabBatcher.cs:1369); a miss writes (-1, 1.0f)
+ * (SourceSharp.MapTools/Tracing/HitId.cs:86).  Equal distances go to the
+ * lower triangle index. */
+int vh_trace_closest(vh_ctx *ctx, const float *rays, int32_t record_words, float shared_reach,
+                     int64_t n, float min_distance, int32_t skip_id, int32_t *surface,
+                     float *fraction, int64_t hit_slots);
+
+/* Shadow rays from sample points to point lights, generated in-kernel in
+ * vrad's correct-mode parameterisation (direction = light - point, reach 1;
+ * SourceSharp.MapTools/Rad/Light/LightRayLog.cs:686-690): bit (p * nlights + l)
+ * is set when the segment from point p to light l is blocked.  The same
+ * answer as vh_trace_visibility over those npoints * nlights rays, without
+ * materialising them (28 B a ray). */
+int vh_trace_point_lights(vh_ctx *ctx, const float *points, int64_t npoints,
+                          const float *lights, int32_t nlights, float min_distance,
+                          uint64_t *bits, int64_t bit_words);
+
+/* Device-pointer forms of the three traces above, enqueued on `stream`. */
+int vh_trace_visibility_dev(vh_ctx *ctx, const float *d_rays, int32_t record_words,
+                            float shared_reach, int64_t n, float min_distance, int32_t skip_id,
+                            int32_t sky_passes, uint64_t *d_bits, void *stream);
+int vh_trace_closest_dev(vh_ctx *ctx, const float *d_rays, int32_t record_words,
+                         float shared_reach, int64_t n, float min_distance, int32_t skip_id,
+                         int32_t *d_surface, float *d_fraction, void *stream);
+int vh_trace_point_lights_dev(vh_ctx *ctx, const float *d_points, int64_t npoints,
+                              const float *d_lights, int32_t nlights, float min_distance,
+                              uint64_t *d_bits, void *stream);
+
+/* Waits for the work enqueued on `stream` (NULL = the context's stream). */
+int vh_sync(vh_ctx *ctx, void *stream);
+
+/* Cumulative counters (IGpuTraceStatistics, SourceSharp.MapTools.Gpu/SlabBatcher.cs:387). */
+typedef struct vh_stats {
+    int64_t rays_visibility;
+    int64_t rays_closest;
+    int64_t calls;
+    int64_t reserved[5];
+} vh_stats;
+int vh_get_stats(const vh_ctx *ctx, vh_stats *out);
+
+/* Kernel timing with HIP events recorded on the launching stream around each
+ * launch of the named kernel (no counterpart in the reference: the bench and
+ * the profiler read it).  Enable, run, then read: vh_timing_read waits for the
+ * recorded events and returns the summed milliseconds and the launch count
+ * since the last read or enable. */
+#define VH_KERNEL_POINT_LIGHTS 0
+#define VH_KERNEL_VISIBILITY 1
+#define VH_KERNEL_CLOSEST 2
+#define VH_KERNEL_COUNT 8
+int vh_timing_enable(vh_ctx *ctx, int32_t on);
+int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
+
+/* Work counters (no counterpart in the reference): while enabled, traversal
+ * launches also count BVH node visits (two child boxes each) and triangle
+ * tests; the roofline's algorithmic FLOPs are computed from them. */
+int vh_count_enable(vh_ctx *ctx, int32_t on);
+int vh_count_read(vh_ctx *ctx, int64_t *node_visits, int64_t *triangle_tests);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
