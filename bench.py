@@ -8,7 +8,7 @@ parameterisation.  Inputs are resident in HBM before the timed region.
 
 One JSON line on stdout.  `value` = shadow rays/s over the timed steps.
 `roofline` is for the traversal kernel (k_point_lights): achieved algorithmic
-FP32 work per launch (counted node visits x 40 + triangle tests x 27 FLOP)
+FP32 work per launch (counted node visits x 44 + triangle tests x 27 FLOP)
 divided by the kernel's mean duration from HIP events on its stream.
 `cpu_baseline` is the oracle's CPU restatement (BVH + the reference's exact
 triangle test, any hit) timed on a bounded sample of the same points.
@@ -89,6 +89,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nboxes", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a device copy of this many MiB (profiler calibration)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -124,8 +126,20 @@ def main(argv=None) -> int:
     nrays = w.rays
     words = (nrays + 63) // 64
     bits = torch.zeros(words, dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    # A stream of our own (default priority): the library launches on it and
+    # the events below are recorded on it, so they bracket exactly its work.
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.synchronize()
     sptr = ctypes.c_void_p(stream.cuda_stream)
+    if a.copy_probe_mib > 0:
+        # a plain copy of known size, the scale for rocprofv3's FETCH_SIZE/WRITE_SIZE
+        src = torch.empty(a.copy_probe_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     P = ctypes.c_void_p
 
     def step():

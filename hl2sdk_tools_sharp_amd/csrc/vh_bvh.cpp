@@ -290,7 +290,7 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
             o[9] = t.e[5];
             std::memcpy(&o[10], &t.id, 4);
             uint32_t meta = (uint32_t)t.cs0 | ((uint32_t)t.cs1 << 2) |
-                            (i == L.count - 1 ? kTriLast : 0u) | ((uint32_t)orig << 4);
+                            (i == L.count - 1 ? kTriLast : 0u) | ((uint32_t)orig << kOrigShift);
             std::memcpy(&o[11], &meta, 4);
             cursor++;
         }

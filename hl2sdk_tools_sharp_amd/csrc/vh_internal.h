@@ -42,9 +42,10 @@ struct DevBuf {
 //   t0 = (nx, ny, nz, d)
 //   t1 = (e0, e1, e2, e3)
 //   t2 = (e4, e5, id bits, meta bits)
-//   meta = cs0 | cs1 << 2 | last-in-leaf << 3 | original index << 4
+//   meta = cs0 | cs1 << 2 | last-in-leaf << 4 | original index << 5
 constexpr int32_t kEmptyRef = (int32_t)0x80000000;
-constexpr uint32_t kTriLast = 8u;
+constexpr uint32_t kTriLast = 16u;
+constexpr int kOrigShift = 5;
 constexpr int kMaxDepth = 32;      // stack entries per lane; the builder caps depth here
 constexpr int kMaxOrigIndex = 1 << 27;
 

@@ -133,7 +133,7 @@ __device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
                 bool hit = tri_test(t0, t1, t2, meta, r, t) && t >= tmin &&
                            !(skip_id != -1 && id == skip_id);
                 if (hit) {
-                    int32_t orig = (int32_t)(meta >> 4);
+                    int32_t orig = (int32_t)(meta >> kOrigShift);
                     if (MODE == kAny) {
                         if (t < r.reach) {
                             t_out = t;
