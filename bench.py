@@ -79,6 +79,69 @@ def cpu_baseline(w, gpu_bits, target_s: float, threads: int):
     }, mismatched_words
 
 
+def bounce_leg(a, log):
+    """The bounce config: 12 x 12 tiled room grid, ~200k patches.  GPU transfer
+    build, direct light from the rooms' point lights, then the bounce to
+    relative residual 1e-4 with FP32 and with FP16 transfer weights."""
+    import torch
+
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, _lib, rooms
+    from hl2sdk_tools_sharp_amd.bounce import HipBounceStage
+    from hl2sdk_tools_sharp_amd.tracer import unpack_bits
+
+    t0 = time.perf_counter()
+    g = rooms.bounce_grid(a.bounce_rooms, a.bounce_rooms)
+    log(f"[bench] bounce grid {a.bounce_rooms}x{a.bounce_rooms}: {g.npatch} patches "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    offer = HipTracerFactory(device=0).try_create(g.casters)
+    if offer.tracer is None:
+        raise RuntimeError(f"HIP tracer declined: {offer.decline_reason}")
+    tr = offer.tracer
+    st = HipBounceStage(tr, g)
+    st.load_patches()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T = st.build_transfers()
+    build_s = time.perf_counter() - t0
+    log(f"[bench] transfer build: {T} transfers ({T / len(g.receivers):.0f}/row) from "
+        f"{st.candidate_count} candidate rays ({st.blocked_count} blocked) in {build_s:.3f} s")
+    pts = rooms.sample_points(g)
+    bits = tr.trace_point_lights(pts, g.lights)
+    e0 = rooms.direct_light(g, unpack_bits(bits, g.npatch * len(g.lights)))
+    st.bounce(e0, max_iter=2, tol=0.0)  # first-use allocations outside the timing
+    out = {"patches": g.npatch, "transfers": T, "transfers_per_row": T / len(g.receivers),
+           "candidate_rays": st.candidate_count, "transfer_build_s": build_s}
+    results = {}
+    for wname in ("f32", "f16"):
+        t0 = time.perf_counter()
+        r = st.bounce(e0, weights=wname, tol=1e-4, max_iter=200)
+        wall = time.perf_counter() - t0
+        results[wname] = r
+        out[f"wall_s_{wname}"] = wall
+        out[f"iterations_{wname}"] = r.iterations
+        log(f"[bench] bounce {wname}: {r.iterations} iterations to residual "
+            f"{r.residuals[-1]:.2e} in {wall * 1e3:.2f} ms")
+    # the gather kernel alone, from HIP events, without the graph
+    L = _lib.lib()
+    _lib.check(L.vh_timing_enable(tr.handle, 1))
+    for wname in ("f32", "f16"):
+        st.bounce(e0, weights=wname, tol=0.0, max_iter=a.bounce_timed_iters, use_graph=False)
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(L.vh_timing_read(tr.handle, _lib.VH_KERNEL_GATHER, ctypes.byref(ms), ctypes.byref(n)))
+        per = ms.value / max(n.value, 1)
+        wbytes = 4 if wname == "f32" else 2
+        algo = T * (4 + wbytes) + g.npatch * 24 + (wname == "f16") * 4 * g.npatch
+        out[f"gather_ms_{wname}"] = per
+        out[f"gather_GBps_{wname}"] = algo / (per / 1e3) / 1e9
+        out[f"gather_bytes_{wname}"] = algo
+        log(f"[bench] gather {wname}: {per:.4f} ms, {algo / (per / 1e3) / 1e9:.0f} GB/s algorithmic")
+    _lib.check(L.vh_timing_enable(tr.handle, 0))
+    a32 = results["f32"].total_light.astype(np.float64)
+    out["f16_vs_f32_rel_l1"] = float(np.abs(results["f16"].total_light - a32).sum() / np.abs(a32).sum())
+    tr.dispose()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +152,9 @@ def main(argv=None) -> int:
     ap.add_argument("--nboxes", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bounce", action="store_true")
+    ap.add_argument("--bounce-rooms", type=int, default=12)
+    ap.add_argument("--bounce-timed-iters", type=int, default=10)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (profiler calibration)")
     a = ap.parse_args(argv)
@@ -208,6 +274,7 @@ def main(argv=None) -> int:
         parity = {"sample_words_differing": mism}
         log(f"[bench] CPU {cpu['value'] / 1e6:.2f} Mrays/s; sample words differing from GPU: {mism}")
     tr.dispose()
+    bounce = None if a.no_bounce else bounce_leg(a, log)
 
     out = {
         "metric": METRIC,
@@ -232,6 +299,8 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "bounce_wall_s": None if bounce is None else bounce["wall_s_f32"],
+        "bounce": bounce,
     }
     print(json.dumps(out), flush=True)
     return 0

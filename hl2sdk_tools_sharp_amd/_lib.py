@@ -51,6 +51,30 @@ class VhStats(ctypes.Structure):
     ]
 
 
+class VhPatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("npatch", ctypes.c_int32), ("nface", ctypes.c_int32), ("ncluster", ctypes.c_int32),
+        ("nreceivers", ctypes.c_int32),
+        ("origin", ctypes.c_void_p), ("normal", ctypes.c_void_p), ("plane_dist", ctypes.c_void_p),
+        ("area", ctypes.c_void_p), ("child1", ctypes.c_void_p), ("child2", ctypes.c_void_p),
+        ("next_parent", ctypes.c_void_p), ("face", ctypes.c_void_p), ("cluster", ctypes.c_void_p),
+        ("winding", ctypes.c_void_p), ("reflectivity", ctypes.c_void_p),
+        ("face_sky", ctypes.c_void_p), ("face_parent", ctypes.c_void_p),
+        ("row_faces_ptr", ctypes.c_void_p), ("row_faces", ctypes.c_void_p),
+        ("receivers", ctypes.c_void_p),
+    ]
+
+
+class VhBounceOpts(ctypes.Structure):
+    _fields_ = [
+        ("weights", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+    ]
+
+
 class VradHipError(RuntimeError):
     """A libvradhip call failed; ``status`` is the VH_* code."""
 
@@ -83,6 +107,11 @@ SIGNATURES = {
     "vh_trace_point_lights_dev": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
     "vh_sync": (ctypes.c_int, [P, P]),
     "vh_get_stats": (ctypes.c_int, [P, ctypes.POINTER(VhStats)]),
+    "vh_patches_load": (ctypes.c_int, [P, ctypes.POINTER(VhPatchDesc)]),
+    "vh_transfers_build": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+    "vh_transfers_read": (ctypes.c_int, [P, P, P, P, I64]),
+    "vh_transfers_load": (ctypes.c_int, [P, P, P, P]),
+    "vh_bounce": (ctypes.c_int, [P, ctypes.POINTER(VhBounceOpts), P, P, P, P, P, ctypes.POINTER(I32)]),
     "vh_timing_enable": (ctypes.c_int, [P, I32]),
     "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
     "vh_count_enable": (ctypes.c_int, [P, I32]),
@@ -92,6 +121,13 @@ SIGNATURES = {
 VH_KERNEL_POINT_LIGHTS = 0
 VH_KERNEL_VISIBILITY = 1
 VH_KERNEL_CLOSEST = 2
+VH_KERNEL_GATHER = 3
+VH_KERNEL_TRANSFERS = 4
+
+VH_WEIGHTS_F32 = 0
+VH_WEIGHTS_F16 = 1
+VH_BOUNCE_PROGRESSIVE = 0
+VH_BOUNCE_FROM_START = 1
 
 
 def lib():

@@ -26,6 +26,7 @@ SOURCES = [
     "vh_context.cpp",
     "vh_bvh.cpp",
     "vh_trace.hip",
+    "vh_bounce.hip",
 ]
 
 FLAGS = [

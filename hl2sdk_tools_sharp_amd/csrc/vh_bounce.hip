@@ -1,0 +1,1061 @@
+// The transfer build and the radiosity bounce on gfx950.
+//
+// Transfer build (VisMatrix.BuildAsync, Rad/Bounce/VisMatrix.cs:219-620).
+// One lane per receiver enumerates its candidate sources in the reference's
+// order (PVS faces, the face-plane test, hierarchical descent), traces each
+// candidate's visibility ray inline through the BVH (vh_traverse.h), and
+// applies MakeTransfer and MakeScales.  A count pass sizes each receiver's
+// staging range; the make pass fills it; a compaction lays the rows out as CSR
+// by patch index and derives FP16 weights with a per-row scale.
+//
+// Bounce (Radiosity.BounceAsync, Rad/Bounce/Radiosity.cs:132-316), per
+// iteration: k_gather (one wave per receiver row: sum shoot[src] * w with FP32
+// accumulation), k_collect (leaves take the gathered light, FP64 partial sums
+// of the residual and the emission), k_pullup (parents from their children,
+// one launch per tree level, deepest first), k_reduce (the FP64 partials in a
+// fixed order).  The iteration is captured once as a hipGraph and replayed.
+#include <hip/hip_fp16.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vh_traverse.h"
+
+#pragma clang fp contract(off)
+
+struct BounceData {
+    int32_t npatch = 0, nface = 0, ncluster = 0, nrecv = 0, nleaves = 0, height = 0;
+    // patch system
+    vh::DevBuf origin_area, normal_pd, children, next_parent, face, cluster, winding, refl, sky;
+    vh::DevBuf face_sky, face_parent, row_faces_ptr, row_faces, receivers, leaves, levels;
+    std::vector<int32_t> level_ptr;  // parents by height: level_ptr[h-1]..level_ptr[h]
+    std::vector<int32_t> h_receivers;
+    std::vector<int32_t> h_child1;
+    // transfers
+    bool have_transfers = false;
+    int64_t ntransfers = 0;
+    vh::DevBuf row_ptr, src, w32, w16, rscale, gather_rows;
+    // bounce state
+    vh::DevBuf e0, emit, shoot, add, total, partials, result;
+    double *h_result = nullptr;  // pinned
+};
+
+namespace vh {
+namespace {
+
+constexpr int kTB = 64;          // transfer build: one wave per block, a lane per receiver
+constexpr int kPatchStack = 32;  // hierarchical-descent stack per lane
+constexpr double kPlaneTestEps = 0.01;     // VisMatrix.PlaneTestEpsilon
+constexpr double kTransferEps = 0.0000001; // VisMatrix.TransferEpsilon
+constexpr int64_t kMaxPatchesPerRow = 4 * 65536;  // VisMatrix.MaxPatches
+constexpr int kGB = 256;         // gather: four rows per block
+constexpr int kCB = 256;         // collect
+constexpr float kF16Top = 16384.0f;  // a row's largest weight maps here in FP16
+
+struct PatchDev {
+    const float4 *origin_area;  // origin xyz, area
+    const float4 *normal_pd;    // plane normal xyz, plane distance
+    const int2 *children;
+    const int32_t *next_parent;
+    const int32_t *face;
+    const int32_t *cluster;
+    const float *winding;  // 12 per patch
+    const uint8_t *face_sky;
+    const int32_t *face_parent;
+    const int32_t *row_faces_ptr;
+    const int32_t *row_faces;
+};
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx) + (ay * by) + (az * bz);
+}
+
+// Vec3.Normalise (MapFormats/Geometry/Vec3.cs:178-190).
+__device__ __forceinline__ float normalise3(float x, float y, float z, float &nx, float &ny,
+                                            float &nz)
+{
+    float len = sqrtf((x * x) + (y * y) + (z * z));
+    if (len == 0.0f) {
+        nx = ny = nz = 0.0f;
+        return 0.0f;
+    }
+    nx = x / len;
+    ny = y / len;
+    nz = z / len;
+    return len;
+}
+
+// FormFactors.DiffToDiff (Rad/Bounce/FormFactors.cs), Correct normalise.
+__device__ __forceinline__ float diff_to_diff(float4 o1, float4 n1, float4 o2, float4 n2)
+{
+    float dx, dy, dz;
+    float length = normalise3(o1.x - o2.x, o1.y - o2.y, o1.z - o2.z, dx, dy, dz);
+    return -dot3(dx, dy, dz, n1.x, n1.y, n1.z) * dot3(dx, dy, dz, n2.x, n2.y, n2.z) /
+           (length * length);
+}
+
+// FormFactors.PolyToDiff, Correct side (a sine above 1 is clamped).  The
+// arcsine is evaluated in double and rounded once to float, the reference's
+// correctly rounded DetMathF.Asin except where that double rounds twice.
+__device__ float poly_to_diff(const float *poly, float area, float4 o, float4 n)
+{
+    float ff = 0.0f;
+    for (int p = 0; p < 4; p++) {
+        int q = p < 3 ? p + 1 : 0;
+        float v1x, v1y, v1z, v2x, v2y, v2z;
+        normalise3(poly[3 * p] - o.x, poly[3 * p + 1] - o.y, poly[3 * p + 2] - o.z, v1x, v1y, v1z);
+        normalise3(poly[3 * q] - o.x, poly[3 * q + 1] - o.y, poly[3 * q + 2] - o.z, v2x, v2y, v2z);
+        float cx = (v1y * v2z) - (v1z * v2y);
+        float cy = (v1z * v2x) - (v1x * v2z);
+        float cz = (v1x * v2y) - (v1y * v2x);
+        float gx, gy, gz;
+        float sin_alpha = normalise3(cx, cy, cz, gx, gy, gz);
+        if (sin_alpha < -1.0f || sin_alpha > 1.0f) sin_alpha = sin_alpha > 1.0f ? 1.0f : -1.0f;
+        float as = (float)asin((double)sin_alpha);
+        gx = gx * as;
+        gy = gy * as;
+        gz = gz * as;
+        ff += dot3(gx, gy, gz, n.x, n.y, n.z);
+    }
+    ff *= 0.5f / area;
+    return ff;
+}
+
+// VisMatrix.MakeTransfer (Correct side).
+__device__ __forceinline__ bool make_transfer(const PatchDev &pd, float4 ro, float4 rn, int32_t s,
+                                              float4 so, float &trans)
+{
+    if (pd.face_sky[pd.face[s]]) return false;
+    float area2 = so.w;
+    if (area2 <= 0) return false;
+    float4 sn = pd.normal_pd[s];
+    float scale = diff_to_diff(so, sn, ro, rn);
+    if (scale <= 0) return false;
+    float dx = ro.x - so.x, dy = ro.y - so.y, dz = ro.z - so.z;
+    float threshold = (float)((M_PI * 0.04) * (double)dot3(dx, dy, dz, dx, dy, dz));
+    if (threshold < area2) {
+        scale = poly_to_diff(pd.winding + 12 * (size_t)s, area2, ro, rn);
+        if ((double)scale <= 0.0) return false;
+    }
+    trans = area2 * scale;
+    return (double)trans > kTransferEps;
+}
+
+// One lane per receiver.  MAKE = false counts the candidates; MAKE = true
+// traces, makes and scales them into the receiver's staging range.
+template <bool MAKE>
+__global__ __launch_bounds__(kTB) void k_transfers(PatchDev pd, const int32_t *__restrict__ recv,
+                                                   int32_t nrecv, const float4 *__restrict__ nodes,
+                                                   const float4 *__restrict__ tris,
+                                                   int64_t *__restrict__ counts,
+                                                   const int64_t *__restrict__ cand_off,
+                                                   int32_t *__restrict__ st_src,
+                                                   float *__restrict__ st_w,
+                                                   int64_t *__restrict__ made,
+                                                   unsigned long long *__restrict__ stats,
+                                                   int *__restrict__ err)
+{
+    __shared__ int trav_mem[kMaxDepth * kTB];
+    __shared__ int pstk_mem[kPatchStack * kTB];
+    int *trav = trav_mem + threadIdx.x;
+    int *pstk = pstk_mem + threadIdx.x;
+    int64_t k = (int64_t)blockIdx.x * kTB + threadIdx.x;
+    uint32_t nblocked = 0;
+    if (k < nrecv) {
+        int32_t r = recv[k];
+        float4 ro = pd.origin_area[r];
+        float4 rn = pd.normal_pd[r];
+        int32_t cl = pd.cluster[r];
+        int32_t myface = pd.face[r];
+        int64_t base = MAKE ? cand_off[k] : 0;
+        int64_t limit = MAKE ? cand_off[k + 1] - base : 0;
+        int64_t cnt = 0, kept = 0;
+        // the transfer ray starts one unit off the receiver (VisMatrix.MakeRays)
+        float p1x = ro.x + rn.x, p1y = ro.y + rn.y, p1z = ro.z + rn.z;
+        for (int32_t q = pd.row_faces_ptr[cl]; q < pd.row_faces_ptr[cl + 1]; q++) {
+            int32_t f = pd.row_faces[q];
+            if (f == myface) continue;
+            int32_t head = pd.face_parent[f];
+            if (head < 0) continue;
+            float4 fn = pd.normal_pd[head];
+            if (!((double)dot3(ro.x, ro.y, ro.z, fn.x, fn.y, fn.z) > (double)fn.w + kPlaneTestEps))
+                continue;
+            for (int32_t p = head; p != -1; p = pd.next_parent[p]) {
+                int sp = 0;
+                pstk[0] = p;
+                sp = 1;
+                while (sp > 0) {
+                    sp--;
+                    int32_t s = pstk[sp * kTB];
+                    float4 so = pd.origin_area[s];
+                    int2 ch = pd.children[s];
+                    if (ch.x != -1) {
+                        float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
+                        if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
+                            if (sp + 2 > kPatchStack) {
+                                atomicOr(err, 1);
+                                continue;
+                            }
+                            pstk[sp * kTB] = ch.y;
+                            pstk[(sp + 1) * kTB] = ch.x;  // child1 first, as the recursion
+                            sp += 2;
+                            continue;
+                        }
+                    }
+                    if (!((double)dot3(so.x, so.y, so.z, rn.x, rn.y, rn.z) >
+                          (double)rn.w + kPlaneTestEps))
+                        continue;
+                    cnt++;
+                    if (!MAKE) continue;
+                    if (cnt > limit) {
+                        atomicOr(err, 2);
+                        continue;
+                    }
+                    // VisMatrix.TransferRay, exact divide
+                    float4 sn = pd.normal_pd[s];
+                    float p2x = so.x + sn.x, p2y = so.y + sn.y, p2z = so.z + sn.z;
+                    float dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
+                    float len = sqrtf((dx * dx) + (dy * dy) + (dz * dz));
+                    float safe = len == 0.0f ? 1.1920929E-07f : len;
+                    Ray ray;
+                    ray.ox = p1x;
+                    ray.oy = p1y;
+                    ray.oz = p1z;
+                    ray.dx = dx / safe;
+                    ray.dy = dy / safe;
+                    ray.dz = dz / safe;
+                    ray.reach = len;
+                    float th;
+                    int32_t hid;
+                    bool blocked = false;
+                    if (isfinite(ray.dx) && isfinite(ray.dy) && isfinite(ray.dz))
+                        blocked = trace_one<kAny, false, kTB>(nodes, tris, trav, ray, 0.0f, -1, th,
+                                                              hid) >= 0;
+                    if (blocked) {
+                        nblocked++;
+                        continue;
+                    }
+                    if (kept >= kMaxPatchesPerRow) continue;
+                    float w;
+                    if (make_transfer(pd, ro, rn, s, so, w)) {
+                        st_src[base + kept] = s;
+                        st_w[base + kept] = w;
+                        kept++;
+                    }
+                }
+            }
+        }
+        if (!MAKE) {
+            counts[k] = cnt;
+        } else {
+            // VisMatrix.MakeScales: the row's float sum, left to right
+            if (kept > 0) {
+                float tot = 0.0f;
+                for (int64_t j = 0; j < kept; j++) tot += st_w[base + j];
+                tot = (double)tot > M_PI ? 1.0f / tot : (float)(1.0f / M_PI);
+                for (int64_t j = 0; j < kept; j++) st_w[base + j] = st_w[base + j] * tot;
+            }
+            made[k] = kept;
+        }
+    }
+    if (MAKE) {
+        unsigned long long x = nblocked;
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (threadIdx.x == 0) atomicAdd(&stats[0], x);
+    }
+}
+
+// One wave per receiver: its staged row to its CSR place, and the FP16 copy
+// with the row's scale (the largest weight maps to kF16Top).
+__global__ __launch_bounds__(kGB) void k_compact(const int32_t *__restrict__ recv, int32_t nrecv,
+                                                 const int64_t *__restrict__ cand_off,
+                                                 const int64_t *__restrict__ row_ptr,
+                                                 const int32_t *__restrict__ st_src,
+                                                 const float *__restrict__ st_w,
+                                                 int32_t *__restrict__ src, float *__restrict__ w32,
+                                                 __half *__restrict__ w16,
+                                                 float *__restrict__ rscale)
+{
+    int lane = threadIdx.x & 63;
+    int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    if (k >= nrecv) return;
+    int32_t r = recv[k];
+    int64_t from = cand_off[k];
+    int64_t to = row_ptr[r];
+    int64_t n = row_ptr[r + 1] - to;
+    float mx = 0.0f;
+    for (int64_t j = lane; j < n; j += 64) {
+        float x = st_w[from + j];
+        src[to + j] = st_src[from + j];
+        w32[to + j] = x;
+        mx = fmaxf(mx, fabsf(x));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float scale = mx > 0.0f ? mx / kF16Top : 1.0f;
+    for (int64_t j = lane; j < n; j += 64) w16[to + j] = __float2half_rn(st_w[from + j] / scale);
+    if (lane == 0) rscale[r] = scale;
+}
+
+// Scales weights given as FP32 CSR into FP16 + row scale (for loaded transfers).
+__global__ __launch_bounds__(kGB) void k_half_rows(const int32_t *__restrict__ rows, int32_t nrows,
+                                                   const int64_t *__restrict__ row_ptr,
+                                                   const float *__restrict__ w32,
+                                                   __half *__restrict__ w16,
+                                                   float *__restrict__ rscale)
+{
+    int lane = threadIdx.x & 63;
+    int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    if (k >= nrows) return;
+    int32_t r = rows[k];
+    int64_t b = row_ptr[r], e = row_ptr[r + 1];
+    float mx = 0.0f;
+    for (int64_t j = b + lane; j < e; j += 64) mx = fmaxf(mx, fabsf(w32[j]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float scale = mx > 0.0f ? mx / kF16Top : 1.0f;
+    for (int64_t j = b + lane; j < e; j += 64) w16[j] = __float2half_rn(w32[j] / scale);
+    if (lane == 0) rscale[r] = scale;
+}
+
+// GatherLight (Radiosity.cs:216-252), flat: one wave per receiver row, FP32
+// accumulation of shoot[src] * weight, the row reduced across the wave.
+template <bool F16>
+__global__ __launch_bounds__(kGB) void k_gather(const int32_t *__restrict__ rows, int32_t nrows,
+                                                const int64_t *__restrict__ row_ptr,
+                                                const int32_t *__restrict__ src,
+                                                const float *__restrict__ w32,
+                                                const __half *__restrict__ w16,
+                                                const float *__restrict__ rscale,
+                                                const float4 *__restrict__ shoot,
+                                                float4 *__restrict__ add)
+{
+    int lane = threadIdx.x & 63;
+    int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    if (k >= nrows) return;
+    int32_t r = rows[k];
+    int64_t b = row_ptr[r], e = row_ptr[r + 1];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (int64_t j = b + lane; j < e; j += 64) {
+        int32_t s = src[j];
+        float wk = F16 ? __half2float(w16[j]) : w32[j];
+        float4 sh = shoot[s];
+        ax = __builtin_fmaf(sh.x, wk, ax);
+        ay = __builtin_fmaf(sh.y, wk, ay);
+        az = __builtin_fmaf(sh.z, wk, az);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off, 64);
+        ay += __shfl_xor(ay, off, 64);
+        az += __shfl_xor(az, off, 64);
+    }
+    if (lane == 0) {
+        float sc = F16 ? rscale[r] : 1.0f;
+        add[r] = make_float4(ax * sc, ay * sc, az * sc, 0.0f);
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += lds[i];
+    return s;
+}
+
+// CollectLight for the leaves (Radiosity.cs:268-316): sky emits nothing; a
+// leaf takes the gathered light.  Block partials (FP64): the residual's
+// numerator and denominator and the summed emission per channel.
+__global__ __launch_bounds__(kCB) void k_collect(const int32_t *__restrict__ leaves, int32_t n,
+                                                 const uint8_t *__restrict__ sky, int mode,
+                                                 const float4 *__restrict__ add,
+                                                 float4 *__restrict__ total,
+                                                 float4 *__restrict__ emit,
+                                                 float4 *__restrict__ shoot,
+                                                 const float4 *__restrict__ e0,
+                                                 const float4 *__restrict__ refl,
+                                                 double *__restrict__ partials)
+{
+    __shared__ double lds[kCB / 64];
+    int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    double num = 0, den = 0, ex = 0, ey = 0, ez = 0;
+    if (i < n) {
+        int32_t p = leaves[i];
+        if (sky[p]) {
+            emit[p] = make_float4(0, 0, 0, 0);
+            shoot[p] = make_float4(0, 0, 0, 0);
+        } else {
+            float4 a = add[p], t = total[p], em;
+            if (mode == 0) {
+                num = fabs((double)a.x) + fabs((double)a.y) + fabs((double)a.z);
+                t = make_float4(t.x + a.x, t.y + a.y, t.z + a.z, 0.0f);
+                em = a;
+            } else {
+                num = fabs((double)a.x - (double)t.x) + fabs((double)a.y - (double)t.y) +
+                      fabs((double)a.z - (double)t.z);
+                t = a;
+                float4 b = e0[p];
+                em = make_float4(b.x + a.x, b.y + a.y, b.z + a.z, 0.0f);
+            }
+            den = fabs((double)t.x) + fabs((double)t.y) + fabs((double)t.z);
+            ex = em.x;
+            ey = em.y;
+            ez = em.z;
+            float4 rf = refl[p];
+            total[p] = t;
+            emit[p] = make_float4(em.x, em.y, em.z, 0.0f);
+            shoot[p] = make_float4(em.x * rf.x, em.y * rf.y, em.z * rf.z, 0.0f);
+        }
+    }
+    double v[5] = {num, den, ex, ey, ez};
+    for (int j = 0; j < 5; j++) {
+        double s = block_sum(v[j], lds);
+        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * 5 + j] = s;
+    }
+}
+
+// Parents of one tree level from their children: v = v[c1]*s1 + v[c2]*s2 with
+// s = area ratio (Radiosity.cs:292-311); WITH_SHOOT also refreshes shoot.
+template <bool WITH_SHOOT>
+__global__ __launch_bounds__(kCB) void k_pullup(const int32_t *__restrict__ list, int32_t n,
+                                                const int2 *__restrict__ children,
+                                                const float4 *__restrict__ origin_area,
+                                                float4 *__restrict__ v,
+                                                float4 *__restrict__ shoot,
+                                                const float4 *__restrict__ refl)
+{
+    int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (i >= n) return;
+    int32_t p = list[i];
+    int2 ch = children[p];
+    float a1 = origin_area[ch.x].w, a2 = origin_area[ch.y].w;
+    float s1 = a1 / (a1 + a2);
+    float s2 = a2 / (a1 + a2);
+    float4 x = v[ch.x], y = v[ch.y];
+    float4 r = make_float4((x.x * s1) + (y.x * s2), (x.y * s1) + (y.y * s2),
+                           (x.z * s1) + (y.z * s2), 0.0f);
+    v[p] = r;
+    if (WITH_SHOOT) {
+        float4 rf = refl[p];
+        shoot[p] = make_float4(r.x * rf.x, r.y * rf.y, r.z * rf.z, 0.0f);
+    }
+}
+
+// The FP64 partials summed in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partials, int32_t nblk,
+                                                double *__restrict__ out)
+{
+    __shared__ double lds[4];
+    for (int j = 0; j < 5; j++) {
+        double s = 0.0;
+        for (int32_t b = threadIdx.x; b < nblk; b += 256) s += partials[(size_t)b * 5 + j];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) out[j] = s;
+    }
+}
+
+// Start of the bounce.  Progressive: emit = direct light as given for every
+// patch (MoveDirectLightToEmission, Radiosity.cs:177-187).  From a start x0:
+// leaves total = x0, emit = direct + x0 (sky 0); parents are pulled up after.
+__global__ __launch_bounds__(kCB) void k_init(int32_t P, int mode, const float4 *__restrict__ e0,
+                                              const float4 *__restrict__ start,
+                                              const int2 *__restrict__ children,
+                                              const uint8_t *__restrict__ sky,
+                                              const float4 *__restrict__ refl,
+                                              float4 *__restrict__ emit,
+                                              float4 *__restrict__ shoot,
+                                              float4 *__restrict__ total,
+                                              float4 *__restrict__ add)
+{
+    int64_t p = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (p >= P) return;
+    float4 z = make_float4(0, 0, 0, 0);
+    float4 d = e0[p];
+    float4 em = d, t = z;
+    if (mode == 1) {
+        float4 x = start ? start[p] : z;
+        t = x;
+        em = make_float4(d.x + x.x, d.y + x.y, d.z + x.z, 0.0f);
+        if (sky[p]) em = z;
+    }
+    float4 rf = refl[p];
+    emit[p] = em;
+    shoot[p] = make_float4(em.x * rf.x, em.y * rf.y, em.z * rf.z, 0.0f);
+    total[p] = t;
+    add[p] = z;
+    (void)children;
+}
+
+int check_loaded(vh_ctx *c, bool need_transfers)
+{
+    if (!c->bounce || c->bounce->npatch == 0) {
+        set_error("no patches loaded: call vh_patches_load first");
+        return VH_ESTATE;
+    }
+    if (need_transfers && !c->bounce->have_transfers) {
+        set_error("no transfers: call vh_transfers_build or vh_transfers_load first");
+        return VH_ESTATE;
+    }
+    return VH_OK;
+}
+
+template <typename T>
+int upload(DevBuf &b, const T *host, size_t count, const char *what, hipStream_t s)
+{
+    int rc = b.ensure(count * sizeof(T) + 16, what);
+    if (rc) return rc;
+    if (count) VH_HIP(hipMemcpyAsync(b.ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return VH_OK;
+}
+
+PatchDev patch_dev(BounceData *B)
+{
+    PatchDev pd;
+    pd.origin_area = (const float4 *)B->origin_area.ptr;
+    pd.normal_pd = (const float4 *)B->normal_pd.ptr;
+    pd.children = (const int2 *)B->children.ptr;
+    pd.next_parent = (const int32_t *)B->next_parent.ptr;
+    pd.face = (const int32_t *)B->face.ptr;
+    pd.cluster = (const int32_t *)B->cluster.ptr;
+    pd.winding = (const float *)B->winding.ptr;
+    pd.face_sky = (const uint8_t *)B->face_sky.ptr;
+    pd.face_parent = (const int32_t *)B->face_parent.ptr;
+    pd.row_faces_ptr = (const int32_t *)B->row_faces_ptr.ptr;
+    pd.row_faces = (const int32_t *)B->row_faces.ptr;
+    return pd;
+}
+
+// Allocates the CSR for `total` transfers and the gather's row list.
+int alloc_transfers(BounceData *B, int64_t total)
+{
+    int rc;
+    size_t T = (size_t)(total > 0 ? total : 1);
+    if ((rc = B->src.ensure(T * 4, "transfer sources")) ||
+        (rc = B->w32.ensure(T * 4, "transfer weights")) ||
+        (rc = B->w16.ensure(T * 2, "FP16 transfer weights")) ||
+        (rc = B->rscale.ensure((size_t)B->npatch * 4, "row scales")))
+        return rc;
+    return VH_OK;
+}
+
+int upload_gather_rows(BounceData *B, const std::vector<int64_t> &row_ptr, hipStream_t s)
+{
+    // rows with transfers, in patch order so the CSR streams sequentially
+    std::vector<int32_t> rows;
+    for (int32_t p = 0; p < B->npatch; p++)
+        if (row_ptr[p + 1] > row_ptr[p]) rows.push_back(p);
+    int rc = upload(B->gather_rows, rows.data(), rows.size(), "gather rows", s);
+    if (rc) return rc;
+    B->nrecv = (int32_t)rows.size();
+    return VH_OK;
+}
+
+}  // namespace
+
+void bounce_release(vh_ctx *c)
+{
+    BounceData *B = c->bounce;
+    if (!B) return;
+    for (DevBuf *b : {&B->origin_area, &B->normal_pd, &B->children, &B->next_parent, &B->face,
+                      &B->cluster, &B->winding, &B->refl, &B->sky, &B->face_sky, &B->face_parent,
+                      &B->row_faces_ptr, &B->row_faces, &B->receivers, &B->leaves, &B->levels,
+                      &B->row_ptr, &B->src, &B->w32, &B->w16, &B->rscale, &B->gather_rows,
+                      &B->e0, &B->emit, &B->shoot, &B->add, &B->total, &B->partials, &B->result})
+        b->release();
+    if (B->h_result) hipHostFree(B->h_result);
+    delete B;
+    c->bounce = nullptr;
+}
+
+}  // namespace vh
+
+using namespace vh;
+
+extern "C" {
+
+int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
+{
+    clear_error();
+    if (!c || !d) {
+        set_error("NULL context or descriptor");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    VH_HIP(hipSetDevice(c->device));
+    int32_t P = d->npatch, F = d->nface, C = d->ncluster, R = d->nreceivers;
+    if (P <= 0 || F <= 0 || C <= 0 || R < 0 || !d->origin || !d->normal || !d->plane_dist ||
+        !d->area || !d->child1 || !d->child2 || !d->next_parent || !d->face || !d->cluster ||
+        !d->winding || !d->reflectivity || !d->face_sky || !d->face_parent || !d->row_faces_ptr ||
+        !d->row_faces || (R > 0 && !d->receivers)) {
+        set_error("incomplete patch descriptor");
+        return VH_EINVAL;
+    }
+    // validate the structure the kernels walk
+    for (int32_t p = 0; p < P; p++) {
+        int32_t a = d->child1[p], b = d->child2[p];
+        if ((a == -1) != (b == -1) || (a != -1 && (a <= p || b <= p || a >= P || b >= P))) {
+            set_error("patch " + std::to_string(p) + ": children must both be -1 or both later patches");
+            return VH_EINVAL;
+        }
+        if (d->face[p] < 0 || d->face[p] >= F || d->cluster[p] < -1 || d->cluster[p] >= C ||
+            d->next_parent[p] < -1 || d->next_parent[p] >= P) {
+            set_error("patch " + std::to_string(p) + " has an out-of-range face, cluster or chain");
+            return VH_EINVAL;
+        }
+    }
+    for (int32_t f = 0; f < F; f++)
+        if (d->face_parent[f] < -1 || d->face_parent[f] >= P) {
+            set_error("face " + std::to_string(f) + " has an out-of-range first patch");
+            return VH_EINVAL;
+        }
+    int32_t nrow = d->row_faces_ptr[C];
+    for (int32_t k = 0; k < C; k++)
+        if (d->row_faces_ptr[k] < 0 || d->row_faces_ptr[k] > d->row_faces_ptr[k + 1]) {
+            set_error("row_faces_ptr is not monotone");
+            return VH_EINVAL;
+        }
+    for (int32_t k = 0; k < nrow; k++)
+        if (d->row_faces[k] < 0 || d->row_faces[k] >= F) {
+            set_error("row_faces holds an out-of-range face");
+            return VH_EINVAL;
+        }
+    for (int32_t k = 0; k < R; k++) {
+        int32_t r = d->receivers[k];
+        if (r < 0 || r >= P || d->child1[r] != -1 || d->cluster[r] < 0) {
+            set_error("receiver " + std::to_string(k) + " is not a leaf patch with a cluster");
+            return VH_EINVAL;
+        }
+    }
+    // heights; the descent stack must hold the deepest chain of children
+    std::vector<int32_t> h(P, 0);
+    int32_t H = 0;
+    for (int32_t p = P - 1; p >= 0; p--)
+        if (d->child1[p] != -1) {
+            h[p] = 1 + std::max(h[d->child1[p]], h[d->child2[p]]);
+            H = std::max(H, h[p]);
+        }
+    if (H + 2 > kPatchStack) {
+        set_error("patch trees deeper than " + std::to_string(kPatchStack - 2) + " levels");
+        return VH_EINVAL;
+    }
+
+    bounce_release(c);
+    BounceData *B = new BounceData();
+    c->bounce = B;
+    B->npatch = P;
+    B->nface = F;
+    B->ncluster = C;
+    B->height = H;
+    B->h_child1.assign(d->child1, d->child1 + P);
+    B->h_receivers.assign(d->receivers, d->receivers + R);
+    std::vector<float> oa((size_t)P * 4), np((size_t)P * 4), rf((size_t)P * 4);
+    std::vector<int32_t> ch((size_t)P * 2);
+    std::vector<uint8_t> sky(P);
+    for (int32_t p = 0; p < P; p++) {
+        for (int k = 0; k < 3; k++) {
+            oa[4 * p + k] = d->origin[3 * p + k];
+            np[4 * p + k] = d->normal[3 * p + k];
+            rf[4 * p + k] = d->reflectivity[3 * p + k];
+        }
+        oa[4 * p + 3] = d->area[p];
+        np[4 * p + 3] = d->plane_dist[p];
+        rf[4 * p + 3] = 0.0f;
+        ch[2 * p] = d->child1[p];
+        ch[2 * p + 1] = d->child2[p];
+        sky[p] = d->face_sky[d->face[p]];
+    }
+    std::vector<int32_t> leaves, levels;
+    for (int32_t p = 0; p < P; p++)
+        if (d->child1[p] == -1) leaves.push_back(p);
+    B->level_ptr.assign(1, 0);
+    for (int32_t lv = 1; lv <= H; lv++) {
+        for (int32_t p = 0; p < P; p++)
+            if (h[p] == lv) levels.push_back(p);
+        B->level_ptr.push_back((int32_t)levels.size());
+    }
+    B->nleaves = (int32_t)leaves.size();
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = upload(B->origin_area, oa.data(), oa.size(), "patch origins", s)) ||
+        (rc = upload(B->normal_pd, np.data(), np.size(), "patch normals", s)) ||
+        (rc = upload(B->refl, rf.data(), rf.size(), "reflectivity", s)) ||
+        (rc = upload(B->children, ch.data(), ch.size(), "children", s)) ||
+        (rc = upload(B->sky, sky.data(), sky.size(), "sky flags", s)) ||
+        (rc = upload(B->next_parent, d->next_parent, (size_t)P, "face chains", s)) ||
+        (rc = upload(B->face, d->face, (size_t)P, "patch faces", s)) ||
+        (rc = upload(B->cluster, d->cluster, (size_t)P, "patch clusters", s)) ||
+        (rc = upload(B->winding, d->winding, (size_t)P * 12, "windings", s)) ||
+        (rc = upload(B->face_sky, d->face_sky, (size_t)F, "face sky flags", s)) ||
+        (rc = upload(B->face_parent, d->face_parent, (size_t)F, "face parents", s)) ||
+        (rc = upload(B->row_faces_ptr, d->row_faces_ptr, (size_t)C + 1, "PVS rows", s)) ||
+        (rc = upload(B->row_faces, d->row_faces, (size_t)nrow, "PVS faces", s)) ||
+        (rc = upload(B->receivers, d->receivers, (size_t)R, "receivers", s)) ||
+        (rc = upload(B->leaves, leaves.data(), leaves.size(), "leaves", s)) ||
+        (rc = upload(B->levels, levels.data(), levels.size(), "levels", s)))
+        return rc;
+    size_t v4 = (size_t)P * 16;
+    int nblk = (B->nleaves + kCB - 1) / kCB;
+    if ((rc = B->e0.ensure(v4, "emission")) || (rc = B->emit.ensure(v4, "emission")) ||
+        (rc = B->shoot.ensure(v4, "shooters")) || (rc = B->add.ensure(v4, "gathered light")) ||
+        (rc = B->total.ensure(v4, "total light")) ||
+        (rc = B->partials.ensure((size_t)(nblk > 0 ? nblk : 1) * 5 * 8, "partial sums")) ||
+        (rc = B->result.ensure(64, "residuals")))
+        return rc;
+    if (!B->h_result) VH_HIP(hipHostMalloc((void **)&B->h_result, 64, hipHostMallocDefault));
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64_t *blocked)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, false);
+    if (rc) return rc;
+    if (!c->have_scene) {
+        set_error("no scene loaded: the transfer build traces against vh_scene_load's casters");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    BounceData *B = c->bounce;
+    hipStream_t s = c->stream;
+    int32_t R = (int32_t)B->h_receivers.size();
+    PatchDev pd = patch_dev(B);
+    DevBuf counts, cand_off, st_src, st_w, made, stats, err;
+    auto cleanup = [&]() {
+        for (DevBuf *b : {&counts, &cand_off, &st_src, &st_w, &made, &stats, &err}) b->release();
+    };
+    if ((rc = counts.ensure((size_t)(R + 1) * 8, "candidate counts")) ||
+        (rc = cand_off.ensure((size_t)(R + 1) * 8, "candidate offsets")) ||
+        (rc = made.ensure((size_t)(R + 1) * 8, "row lengths")) ||
+        (rc = stats.ensure(16, "build statistics")) || (rc = err.ensure(16, "build errors"))) {
+        cleanup();
+        return rc;
+    }
+    VH_HIP(hipMemsetAsync(stats.ptr, 0, 16, s));
+    VH_HIP(hipMemsetAsync(err.ptr, 0, 16, s));
+    uint32_t grid = (uint32_t)((R + kTB - 1) / kTB);
+    KernelTimer timer(c, VH_KERNEL_TRANSFERS, s);
+    if (R > 0)
+        hipLaunchKernelGGL(k_transfers<false>, dim3(grid), dim3(kTB), 0, s, pd,
+                           (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
+                           (const float4 *)c->tris.ptr, (int64_t *)counts.ptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, (int *)err.ptr);
+    VH_HIP(hipGetLastError());
+    std::vector<int64_t> hcount(R + 1, 0), hoff(R + 1, 0);
+    if (R > 0)
+        VH_HIP(hipMemcpyAsync(hcount.data(), counts.ptr, (size_t)R * 8, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    for (int32_t k = 0; k < R; k++) hoff[k + 1] = hoff[k] + hcount[k];
+    int64_t ncand = hoff[R];
+    if ((rc = upload(cand_off, hoff.data(), hoff.size(), "candidate offsets", s)) ||
+        (rc = st_src.ensure((size_t)(ncand > 0 ? ncand : 1) * 4, "staged sources")) ||
+        (rc = st_w.ensure((size_t)(ncand > 0 ? ncand : 1) * 4, "staged weights"))) {
+        cleanup();
+        return rc;
+    }
+    if (R > 0)
+        hipLaunchKernelGGL(k_transfers<true>, dim3(grid), dim3(kTB), 0, s, pd,
+                           (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
+                           (const float4 *)c->tris.ptr, nullptr, (const int64_t *)cand_off.ptr,
+                           (int32_t *)st_src.ptr, (float *)st_w.ptr, (int64_t *)made.ptr,
+                           (unsigned long long *)stats.ptr, (int *)err.ptr);
+    timer.done();
+    VH_HIP(hipGetLastError());
+    std::vector<int64_t> hmade(R + 1, 0);
+    int herr = 0;
+    unsigned long long hstats[2] = {0, 0};
+    if (R > 0)
+        VH_HIP(hipMemcpyAsync(hmade.data(), made.ptr, (size_t)R * 8, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipMemcpyAsync(&herr, err.ptr, 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipMemcpyAsync(hstats, stats.ptr, 16, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    if (herr) {
+        cleanup();
+        set_error(herr & 1 ? "patch descent deeper than the stack"
+                           : "candidate enumeration disagreed between its passes");
+        return VH_EDEVICE;
+    }
+    std::vector<int64_t> rp((size_t)B->npatch + 1, 0), len((size_t)B->npatch, 0);
+    for (int32_t k = 0; k < R; k++) len[B->h_receivers[k]] = hmade[k];
+    for (int32_t p = 0; p < B->npatch; p++) rp[p + 1] = rp[p] + len[p];
+    int64_t total = rp[B->npatch];
+    if ((rc = upload(B->row_ptr, rp.data(), rp.size(), "row offsets", s)) ||
+        (rc = alloc_transfers(B, total))) {
+        cleanup();
+        return rc;
+    }
+    if (R > 0)
+        hipLaunchKernelGGL(k_compact, dim3((uint32_t)((R + 3) / 4)), dim3(kGB), 0, s,
+                           (const int32_t *)B->receivers.ptr, R, (const int64_t *)cand_off.ptr,
+                           (const int64_t *)B->row_ptr.ptr, (const int32_t *)st_src.ptr,
+                           (const float *)st_w.ptr, (int32_t *)B->src.ptr, (float *)B->w32.ptr,
+                           (__half *)B->w16.ptr, (float *)B->rscale.ptr);
+    VH_HIP(hipGetLastError());
+    if ((rc = upload_gather_rows(B, rp, s))) {
+        cleanup();
+        return rc;
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    cleanup();
+    B->ntransfers = total;
+    B->have_transfers = true;
+    if (transfers) *transfers = total;
+    if (candidates) *candidates = ncand;
+    if (blocked) *blocked = (int64_t)hstats[0];
+    return VH_OK;
+}
+
+int vh_transfers_read(vh_ctx *c, int64_t *row_ptr, int32_t *src, float *weight, int64_t capacity)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, true);
+    if (rc) return rc;
+    BounceData *B = c->bounce;
+    if (capacity < B->ntransfers) {
+        set_error(std::to_string(B->ntransfers) + " transfers need as many slots, and " +
+                  std::to_string(capacity) + " were given");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    if (row_ptr)
+        VH_HIP(hipMemcpy(row_ptr, B->row_ptr.ptr, (size_t)(B->npatch + 1) * 8,
+                         hipMemcpyDeviceToHost));
+    if (src && B->ntransfers)
+        VH_HIP(hipMemcpy(src, B->src.ptr, (size_t)B->ntransfers * 4, hipMemcpyDeviceToHost));
+    if (weight && B->ntransfers)
+        VH_HIP(hipMemcpy(weight, B->w32.ptr, (size_t)B->ntransfers * 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_transfers_load(vh_ctx *c, const int64_t *row_ptr, const int32_t *src, const float *weight)
+{
+    clear_error();
+    if (!c || !row_ptr) {
+        set_error("NULL context or row offsets");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, false);
+    if (rc) return rc;
+    BounceData *B = c->bounce;
+    int32_t P = B->npatch;
+    if (row_ptr[0] != 0) {
+        set_error("row_ptr[0] must be 0");
+        return VH_EINVAL;
+    }
+    for (int32_t p = 0; p < P; p++)
+        if (row_ptr[p + 1] < row_ptr[p] || (row_ptr[p + 1] > row_ptr[p] && B->h_child1[p] != -1)) {
+            set_error("row_ptr is not monotone, or a parent patch has transfers");
+            return VH_EINVAL;
+        }
+    int64_t T = row_ptr[P];
+    if (T > 0 && (!src || !weight)) {
+        set_error("NULL sources or weights");
+        return VH_EINVAL;
+    }
+    for (int64_t k = 0; k < T; k++)
+        if (src[k] < 0 || src[k] >= P) {
+            set_error("transfer " + std::to_string(k) + " names patch " + std::to_string(src[k]) +
+                      ", outside 0.." + std::to_string(P - 1));
+            return VH_EINVAL;
+        }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    std::vector<int64_t> rp(row_ptr, row_ptr + P + 1);
+    if ((rc = upload(B->row_ptr, row_ptr, (size_t)P + 1, "row offsets", s)) ||
+        (rc = alloc_transfers(B, T)) || (rc = upload(B->src, src, (size_t)T, "sources", s)) ||
+        (rc = upload(B->w32, weight, (size_t)T, "weights", s)) ||
+        (rc = upload_gather_rows(B, rp, s)))
+        return rc;
+    if (B->nrecv > 0)
+        hipLaunchKernelGGL(k_half_rows, dim3((uint32_t)((B->nrecv + 3) / 4)), dim3(kGB), 0, s,
+                           (const int32_t *)B->gather_rows.ptr, B->nrecv,
+                           (const int64_t *)B->row_ptr.ptr, (const float *)B->w32.ptr,
+                           (__half *)B->w16.ptr, (float *)B->rscale.ptr);
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipStreamSynchronize(s));
+    B->ntransfers = T;
+    B->have_transfers = true;
+    return VH_OK;
+}
+
+int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const float *start,
+              float *total_out, double *residuals, double *energies, int32_t *iters)
+{
+    clear_error();
+    if (!c || !o || !emit0) {
+        set_error("NULL context, options or emission");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, true);
+    if (rc) return rc;
+    if (o->max_iter < 1 || (o->weights != VH_WEIGHTS_F32 && o->weights != VH_WEIGHTS_F16) ||
+        (o->mode != VH_BOUNCE_PROGRESSIVE && o->mode != VH_BOUNCE_FROM_START) || std::isnan(o->tol)) {
+        set_error("bad bounce options");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    BounceData *B = c->bounce;
+    hipStream_t s = c->stream;
+    int32_t P = B->npatch;
+    std::vector<float> buf((size_t)P * 4, 0.0f);
+    for (int32_t p = 0; p < P; p++)
+        for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
+    VH_HIP(hipMemcpyAsync(B->e0.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+    float4 *startd = nullptr;
+    if (o->mode == VH_BOUNCE_FROM_START && start) {
+        for (int32_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) buf[4 * p + k] = start[3 * p + k];
+        // the gathered-light buffer holds the start until k_init copies it
+        VH_HIP(hipMemcpyAsync(B->add.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+        VH_HIP(hipMemcpyAsync(B->total.ptr, B->add.ptr, buf.size() * 4, hipMemcpyDeviceToDevice, s));
+        startd = (float4 *)B->total.ptr;
+    }
+    const float4 *e0 = (const float4 *)B->e0.ptr;
+    float4 *emit = (float4 *)B->emit.ptr, *shoot = (float4 *)B->shoot.ptr;
+    float4 *add = (float4 *)B->add.ptr, *total = (float4 *)B->total.ptr;
+    const float4 *refl = (const float4 *)B->refl.ptr;
+    const int2 *children = (const int2 *)B->children.ptr;
+    const float4 *oa = (const float4 *)B->origin_area.ptr;
+    const uint8_t *sky = (const uint8_t *)B->sky.ptr;
+    // k_init reads the start from `total` and writes it back, so alias is safe per element
+    hipLaunchKernelGGL(k_init, dim3((uint32_t)((P + kCB - 1) / kCB)), dim3(kCB), 0, s, P, o->mode,
+                       e0, (const float4 *)startd, children, sky, refl, emit, shoot, total, add);
+    VH_HIP(hipGetLastError());
+    const int32_t *levels = (const int32_t *)B->levels.ptr;
+    if (o->mode == VH_BOUNCE_FROM_START)
+        for (int32_t lv = 0; lv < B->height; lv++) {
+            int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
+            if (n > 0)
+                hipLaunchKernelGGL(k_pullup<true>, dim3((uint32_t)((n + kCB - 1) / kCB)),
+                                   dim3(kCB), 0, s, levels + a, n, children, oa, emit, shoot, refl);
+        }
+    VH_HIP(hipGetLastError());
+
+    int32_t nblk = (B->nleaves + kCB - 1) / kCB;
+    bool f16 = o->weights == VH_WEIGHTS_F16;
+    auto iteration = [&](bool timed) {
+        if (B->nrecv > 0) {
+            KernelTimer timer(c, VH_KERNEL_GATHER, s);
+            if (!timed) timer.a = timer.b = nullptr;
+            uint32_t g = (uint32_t)((B->nrecv + 3) / 4);
+            if (f16)
+                hipLaunchKernelGGL(k_gather<true>, dim3(g), dim3(kGB), 0, s,
+                                   (const int32_t *)B->gather_rows.ptr, B->nrecv,
+                                   (const int64_t *)B->row_ptr.ptr, (const int32_t *)B->src.ptr,
+                                   (const float *)B->w32.ptr, (const __half *)B->w16.ptr,
+                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add);
+            else
+                hipLaunchKernelGGL(k_gather<false>, dim3(g), dim3(kGB), 0, s,
+                                   (const int32_t *)B->gather_rows.ptr, B->nrecv,
+                                   (const int64_t *)B->row_ptr.ptr, (const int32_t *)B->src.ptr,
+                                   (const float *)B->w32.ptr, (const __half *)B->w16.ptr,
+                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add);
+            if (timed) timer.done();
+        }
+        if (nblk > 0)
+            hipLaunchKernelGGL(k_collect, dim3((uint32_t)nblk), dim3(kCB), 0, s,
+                               (const int32_t *)B->leaves.ptr, B->nleaves, sky, o->mode,
+                               (const float4 *)add, total, emit, shoot, e0, refl,
+                               (double *)B->partials.ptr);
+        for (int32_t lv = 0; lv < B->height; lv++) {
+            int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
+            if (n > 0)
+                hipLaunchKernelGGL(k_pullup<true>, dim3((uint32_t)((n + kCB - 1) / kCB)),
+                                   dim3(kCB), 0, s, levels + a, n, children, oa, emit, shoot, refl);
+        }
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, (const double *)B->partials.ptr,
+                           nblk, (double *)B->result.ptr);
+        hipMemcpyAsync(B->h_result, B->result.ptr, 5 * 8, hipMemcpyDeviceToHost, s);
+    };
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graphed = o->use_graph != 0 && !c->timing;
+    if (graphed) {
+        VH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iteration(false);
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess) {
+            set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+            return VH_EDEVICE;
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(graph);
+            set_error(std::string("graph instantiation failed: ") + hipGetErrorString(e));
+            return VH_EDEVICE;
+        }
+    }
+    int32_t it = 0;
+    hipError_t failure = hipSuccess;
+    for (;;) {
+        if (graphed)
+            failure = hipGraphLaunch(exec, s);
+        else {
+            iteration(true);
+            failure = hipGetLastError();
+        }
+        if (failure == hipSuccess) failure = hipStreamSynchronize(s);
+        if (failure != hipSuccess) break;
+        const double *r = B->h_result;
+        double res = r[1] > 0 ? r[0] / r[1] : 0.0;
+        if (residuals) residuals[it] = res;
+        if (energies) {
+            energies[3 * it] = r[2];
+            energies[3 * it + 1] = r[3];
+            energies[3 * it + 2] = r[4];
+        }
+        it++;
+        bool stop = it >= o->max_iter;
+        if (o->tol > 0)
+            stop = stop || res <= o->tol;
+        else
+            stop = stop || (r[2] < 1.0 && r[3] < 1.0 && r[4] < 1.0);
+        if (stop) break;
+    }
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (failure != hipSuccess) {
+        set_error(std::string("bounce iteration failed: ") + hipGetErrorString(failure));
+        return VH_EDEVICE;
+    }
+    // the parents' totals, pulled up once at the end as CollectLight leaves them
+    for (int32_t lv = 0; lv < B->height; lv++) {
+        int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
+        if (n > 0)
+            hipLaunchKernelGGL(k_pullup<false>, dim3((uint32_t)((n + kCB - 1) / kCB)), dim3(kCB),
+                               0, s, levels + a, n, children, oa, total, nullptr, refl);
+    }
+    VH_HIP(hipGetLastError());
+    if (total_out) {
+        VH_HIP(hipMemcpyAsync(buf.data(), B->total.ptr, buf.size() * 4, hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        for (int32_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) total_out[3 * p + k] = buf[4 * p + k];
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    if (iters) *iters = it;
+    return VH_OK;
+}
+
+}  // extern "C"

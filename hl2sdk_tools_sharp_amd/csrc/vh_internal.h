@@ -86,6 +86,8 @@ struct vh_ctx {
     // work counters (vh_count_*): [node visits, triangle tests]
     bool counting = false;
     vh::DevBuf counters;
+    // patches, transfers and bounce state (vh_bounce.hip)
+    struct BounceData *bounce = nullptr;
 };
 
 namespace vh {
@@ -110,6 +112,8 @@ int launch_closest(vh_ctx *c, const float *d_rays, int32_t rw, float shared_reac
                    hipStream_t s);
 int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const float *d_lights,
                         int32_t nlights, float tmin, uint64_t *d_bits, hipStream_t s);
+// Frees the patch/transfer/bounce buffers (vh_bounce.hip).
+void bounce_release(vh_ctx *c);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

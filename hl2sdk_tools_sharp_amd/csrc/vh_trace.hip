@@ -1,19 +1,9 @@
-// Ray traversal kernels for gfx950: BVH2 any-hit and closest-hit, with the
-// reference KD tracer's exact triangle test.
-//
-// Numerics.  The triangle test is the reference's IntersectTriangle
-// (Tracing/KdRayTracer.cs:1194-1267) operation for operation: this file is
-// built with -ffp-contract=off and `fp contract(off)` below, so no multiply
-// and add fuse, and the divide is IEEE (hipcc's default correctly rounded
-// f32 divide).  Box tests only cull, and use explicit fmaf with a widened
-// interval, so a hit the exact test accepts is never culled.
-//
-// Layout.  One lane traces one ray.  The traversal stack lives in LDS, one
-// column per lane ([depth][lane], so a wave's pushes hit 64 different banks),
-// kMaxDepth entries deep, which the scene builder guarantees is enough.
+// Ray traversal kernels for gfx950: BVH2 any-hit and closest-hit over the
+// caller's rays, and shadow rays generated in-kernel from points and lights.
+// The traversal itself (exact triangle test, LDS stack) is in vh_traverse.h.
 #include <hipcub/hipcub.hpp>
 
-#include "vh_internal.h"
+#include "vh_traverse.h"
 
 #pragma clang fp contract(off)
 
@@ -21,149 +11,6 @@ namespace vh {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr float kParallelEps = 1.0e-10f;  // stock's FourZeros (KdRayTracer.cs:1211)
-constexpr float kZeroSub = 8.6736174e-19f;  // 2^-60, KdRayTracer.cs:193
-constexpr float kSlack = 6.1035156e-5f;     // 2^-14 relative widening of box intervals
-
-enum Mode { kAny = 0, kClosestInclusive = 1, kClosestShort = 2 };
-
-struct Ray {
-    float ox, oy, oz, dx, dy, dz, reach;
-};
-
-__device__ __forceinline__ bool ray_finite(const Ray &r)
-{
-    return isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) &&
-           isfinite(r.dy) && isfinite(r.dz);
-}
-
-__device__ __forceinline__ float inv_saturate(float d)
-{
-    // zeros become +-2^-60 with their sign kept (ReciprocalSaturateExact,
-    // KdRayTracer.cs:1446-1451); the box test tolerates an approximate inverse.
-    if (d == 0.0f) d = copysignf(kZeroSub, d);
-    return __builtin_amdgcn_rcpf(d);
-}
-
-__device__ __forceinline__ float sel3(int c, float x, float y, float z)
-{
-    return c == 0 ? x : (c == 1 ? y : z);
-}
-
-// The exact test.  Returns true with t when the ray hits the triangle.
-__device__ __forceinline__ bool tri_test(const float4 t0, const float4 t1, const float4 t2,
-                                         uint32_t meta, const Ray &r, float &t)
-{
-    float ddotn = r.dx * t0.x;
-    ddotn = (r.dy * t0.y) + ddotn;
-    ddotn = (r.dz * t0.z) + ddotn;
-    float odotn = r.ox * t0.x;
-    odotn = (r.oy * t0.y) + odotn;
-    odotn = (r.oz * t0.z) + odotn;
-    t = (t0.w - odotn) / ddotn;
-    int cs0 = meta & 3, cs1 = (meta >> 2) & 3;
-    float h1 = sel3(cs0, r.ox, r.oy, r.oz) + (t * sel3(cs0, r.dx, r.dy, r.dz));
-    float h2 = sel3(cs1, r.ox, r.oy, r.oz) + (t * sel3(cs1, r.dx, r.dy, r.dz));
-    float b0 = t1.x * h1;
-    b0 = b0 + (t1.y * h2);
-    b0 = b0 + t1.z;
-    float b1 = t1.w * h1;
-    b1 = b1 + (t2.x * h2);
-    b1 = b1 + t2.y;
-    return (ddotn > kParallelEps || ddotn < -kParallelEps) && (t > kParallelEps) &&
-           (b0 >= kParallelEps) && (b1 >= kParallelEps) && ((b1 + b0) <= 1.0f);
-}
-
-// One ray through the BVH.  MODE kAny returns at the first hit strictly short
-// of the reach; the closest modes keep the nearest (ties to the lower
-// original triangle index).  Returns the original triangle index or -1.
-template <int MODE, bool COUNT = false>
-__device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
-                                             const float4 *__restrict__ tris, int *stk,
-                                             const Ray &r, float tmin, int32_t skip_id,
-                                             float &t_out, int32_t &id_out,
-                                             uint32_t *nvisit = nullptr, uint32_t *ntest = nullptr)
-{
-    float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
-    float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
-    float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
-    float best_t = r.reach;
-    int32_t best_orig = -1, best_id = -1;
-    int sp = 0;
-    int32_t ref = 0;
-    for (;;) {
-        if (ref >= 0) {
-            if (COUNT) ++*nvisit;
-            const float4 *nd = nodes + (size_t)ref * 4;
-            float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
-            float thi = MODE == kAny ? r.reach : best_t;
-            float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
-            float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
-            float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
-            float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
-            float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
-            float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
-            float tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
-            float tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
-            float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
-            float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
-            int32_t r0 = __float_as_int(q3.x), r1 = __float_as_int(q3.y);
-            bool h0 = tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0) && r0 != kEmptyRef;
-            bool h1 = tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1) && r1 != kEmptyRef;
-            if (h0 && h1) {
-                bool swap = tn1 < tn0;
-                stk[sp * kBlock] = swap ? r0 : r1;
-                sp++;
-                ref = swap ? r1 : r0;
-                continue;
-            }
-            if (h0 | h1) {
-                ref = h0 ? r0 : r1;
-                continue;
-            }
-        } else {
-            int32_t k = ~ref;
-            for (;;) {
-                if (COUNT) ++*ntest;
-                const float4 *tp = tris + (size_t)k * 3;
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                uint32_t meta = __float_as_uint(t2.w);
-                int32_t id = __float_as_int(t2.z);
-                float t;
-                bool hit = tri_test(t0, t1, t2, meta, r, t) && t >= tmin &&
-                           !(skip_id != -1 && id == skip_id);
-                if (hit) {
-                    int32_t orig = (int32_t)(meta >> kOrigShift);
-                    if (MODE == kAny) {
-                        if (t < r.reach) {
-                            t_out = t;
-                            id_out = id;
-                            return orig;
-                        }
-                    } else {
-                        bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);
-                        bool better = best_orig < 0
-                                          ? within
-                                          : (t < best_t || (t == best_t && orig < best_orig));
-                        if (better) {
-                            best_t = t;
-                            best_orig = orig;
-                            best_id = id;
-                        }
-                    }
-                }
-                if (meta & kTriLast) break;
-                k++;
-            }
-        }
-        if (sp == 0) break;
-        sp--;
-        ref = stk[sp * kBlock];
-    }
-    t_out = best_t;
-    id_out = best_id;
-    return best_orig;
-}
 
 __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays, int rw, float shared_reach,
                                         int64_t i)

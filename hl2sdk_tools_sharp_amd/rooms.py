@@ -1,0 +1,399 @@
+"""Seeded room grids: the patch system vrad's bounce runs on.
+
+The reference gets faces, patches, clusters and a PVS from vbsp and vvis;
+here they are generated, with the reference's subdivision rules and the
+reference's patch fields, so that the transfer build and the bounce see the
+same structure they see on a compiled map.
+"""
+from __future__ import annotations
+
+import sys
+from dataclasses import dataclass
+
+import numpy as np
+
+from .tracer import TRACE_ID_OPAQUE, TRACE_ID_SKY, ShadowCasterSet
+
+
+@dataclass
+class RoomGrid:
+    """A grid of box rooms joined by doorways, with everything the bounce
+    needs: faces, the patch tree of each face, clusters (one per room), a PVS
+    row per cluster and shadow casters.
+
+    Patch fields follow the reference's ``Patch`` (Rad/Light/Patch.cs):
+    origin, plane normal, plane distance, area, ``child1``/``child2`` (-1 for
+    a leaf), the next root patch of the same face (the FaceParents chain),
+    face, cluster and a four-corner winding wound as vrad's are
+    ((c1-c0) x (c2-c0) points away from the face normal, the sense
+    FormFactors.PolyToDiff expects).  Children have higher indices than their
+    parents, as in ``PatchSet``."""
+
+    casters: ShadowCasterSet
+    room_lo: np.ndarray  # (C, 3) interior bounds
+    room_hi: np.ndarray
+    face_lo: np.ndarray  # (F, 3)
+    face_hi: np.ndarray
+    face_normal: np.ndarray  # (F, 3)
+    face_cluster: np.ndarray  # (F,)
+    face_sky: np.ndarray  # (F,) uint8
+    face_refl: np.ndarray  # (F, 3)
+    face_parent: np.ndarray  # (F,) first root patch of the face
+    origin: np.ndarray  # (P, 3) float32
+    normal: np.ndarray
+    plane_dist: np.ndarray
+    area: np.ndarray
+    child1: np.ndarray
+    child2: np.ndarray
+    parent: np.ndarray
+    next_parent: np.ndarray
+    face: np.ndarray
+    cluster: np.ndarray
+    winding: np.ndarray  # (P, 4, 3)
+    row_faces_ptr: np.ndarray  # (C + 1,)
+    row_faces: np.ndarray
+    receivers: np.ndarray  # leaf patches, cluster by cluster
+    lights: np.ndarray  # (L, 3) point lights
+    light_intensity: np.ndarray  # (L,)
+    to_sun: np.ndarray  # (3,) unit, towards the sun
+
+    @property
+    def npatch(self) -> int:
+        return self.origin.shape[0]
+
+    @property
+    def nface(self) -> int:
+        return self.face_lo.shape[0]
+
+    @property
+    def ncluster(self) -> int:
+        return self.room_lo.shape[0]
+
+    @property
+    def leaf(self) -> np.ndarray:
+        return self.child1 < 0
+
+    @property
+    def sky(self) -> np.ndarray:
+        return self.face_sky[self.face]
+
+    @property
+    def refl(self) -> np.ndarray:
+        return self.face_refl[self.face]
+
+    def levels(self) -> np.ndarray:
+        """Height of each patch above its deepest leaf (leaves are 0)."""
+        h = np.zeros(self.npatch, np.int32)
+        for i in range(self.npatch - 1, -1, -1):
+            c1 = self.child1[i]
+            if c1 >= 0:
+                h[i] = 1 + max(h[c1], h[self.child2[i]])
+        return h
+
+    def pull_up(self, values: np.ndarray) -> np.ndarray:
+        """Parents from their children, area weighted, children first
+        (Radiosity.CollectLight's parent rule, Rad/Bounce/Radiosity.cs:268-316)."""
+        v = np.array(values, dtype=np.float32, copy=True)
+        a = self.area
+        for i in range(self.npatch - 1, -1, -1):
+            c1 = self.child1[i]
+            if c1 >= 0:
+                c2 = self.child2[i]
+                s1 = np.float32(a[c1] / (a[c1] + a[c2]))
+                s2 = np.float32(a[c2] / (a[c1] + a[c2]))
+                v[i] = (v[c1] * s1) + (v[c2] * s2)
+        return v
+
+
+def _rect_winding(lo, hi, ax, sign):
+    u, v = (ax + 1) % 3, (ax + 2) % 3
+    c = np.zeros((4, 3), dtype=np.float64)
+    for q, (su, sv) in enumerate([(0, 0), (1, 0), (1, 1), (0, 1)]):
+        c[q, ax] = lo[ax]
+        c[q, u] = hi[u] if su else lo[u]
+        c[q, v] = hi[v] if sv else lo[v]
+    # this order winds along +axis; vrad's windings wind against the normal
+    if sign > 0:
+        c = c[[0, 3, 2, 1]]
+    return c
+
+
+def room_grid(nx: int = 3, ny: int = 3, size=(512.0, 512.0, 256.0), wall: float = 8.0,
+              door: float = 128.0, patch: float = 128.0, minchop: float = 64.0,
+              nlights: int = 8, sky_ceiling: bool = False, seed: int = 1,
+              tile: float = 0.0) -> RoomGrid:
+    """A grid of ``nx`` x ``ny`` rooms of interior ``size``, walls ``wall`` thick,
+    each shared wall pierced by a ``door`` x ``door`` doorway at floor level in
+    its middle.  Faces are the rooms' inner surfaces (a wall with a doorway is
+    split into three rectangles around it) and the doorway jambs, which belong
+    to the room on their lower side.  Patches are subdivided by the
+    reference's rules (PatchSubdivider.SubdividePatch and CreateChildPatch,
+    Rad/Light/PatchSubdivider.cs) with ``patch`` as the chop and ``minchop``
+    as its floor, both in world units.  The PVS row of a room is the room and
+    its doorway neighbours.
+
+    ``tile`` > 0 cuts every face into square tiles of that size first, as
+    vbsp leaves a map whose faces are small: each tile is a face with a
+    single patch (no subdivision, so no parent patches), and every source a
+    receiver sees is then a separate candidate."""
+    rng = np.random.default_rng(seed)
+    W, D, H = (float(x) for x in size)
+    nrooms = nx * ny
+    room_lo = np.zeros((nrooms, 3))
+    room_hi = np.zeros((nrooms, 3))
+    for j in range(ny):
+        for i in range(nx):
+            r = j * nx + i
+            room_lo[r] = (i * (W + wall), j * (D + wall), 0.0)
+            room_hi[r] = room_lo[r] + (W, D, H)
+
+    faces = []  # (lo, hi, axis, sign, cluster, sky)
+
+    def add_face(lo, hi, ax, sign, cluster, sky=False):
+        faces.append((np.array(lo, float), np.array(hi, float), ax, sign, cluster, sky))
+
+    half = door / 2.0
+    for j in range(ny):
+        for i in range(nx):
+            r = j * nx + i
+            lo, hi = room_lo[r], room_hi[r]
+            add_face((lo[0], lo[1], lo[2]), (hi[0], hi[1], lo[2]), 2, +1, r)  # floor
+            add_face((lo[0], lo[1], hi[2]), (hi[0], hi[1], hi[2]), 2, -1, r, sky_ceiling)  # ceiling
+            for ax, door_lo, door_hi in ((0, i > 0, i < nx - 1), (1, j > 0, j < ny - 1)):
+                u = 1 - ax  # the wall's horizontal in-plane axis
+                for side, has_door in ((0, door_lo), (1, door_hi)):
+                    c = hi[ax] if side else lo[ax]
+                    sign = -1 if side else +1
+                    wl = lo.copy()
+                    wh = hi.copy()
+                    wl[ax] = wh[ax] = c
+                    if not has_door:
+                        add_face(wl, wh, ax, sign, r)
+                        continue
+                    mid = 0.5 * (lo[u] + hi[u])
+                    a, b = wl.copy(), wh.copy()
+                    b[u] = mid - half
+                    add_face(a, b, ax, sign, r)
+                    a, b = wl.copy(), wh.copy()
+                    a[u] = mid + half
+                    add_face(a, b, ax, sign, r)
+                    a, b = wl.copy(), wh.copy()
+                    a[u], b[u], a[2] = mid - half, mid + half, door
+                    add_face(a, b, ax, sign, r)
+            for ax, has in ((0, i < nx - 1), (1, j < ny - 1)):
+                if not has:
+                    continue
+                u = 1 - ax
+                mid = 0.5 * (lo[u] + hi[u])
+                x0, x1 = hi[ax], hi[ax] + wall
+                for s_u, sign in ((mid - half, +1), (mid + half, -1)):  # side jambs
+                    a, b = np.zeros(3), np.zeros(3)
+                    a[ax], b[ax] = x0, x1
+                    a[u] = b[u] = s_u
+                    a[2], b[2] = 0.0, door
+                    add_face(a, b, u, sign, r)
+                a, b = np.zeros(3), np.zeros(3)
+                a[ax], b[ax] = x0, x1
+                a[u], b[u] = mid - half, mid + half
+                a[2] = b[2] = door
+                add_face(a, b, 2, -1, r)  # lintel
+                a, b = a.copy(), b.copy()
+                a[2] = b[2] = 0.0
+                add_face(a, b, 2, +1, r)  # threshold
+
+    if tile > 0:
+        cut = []
+        for lo, hi, ax, sign, cl, sky in faces:
+            u, v = (ax + 1) % 3, (ax + 2) % 3
+            nu = max(1, int(np.ceil((hi[u] - lo[u]) / tile - 1e-9)))
+            nv = max(1, int(np.ceil((hi[v] - lo[v]) / tile - 1e-9)))
+            for a in range(nu):
+                for b in range(nv):
+                    tl, th = lo.copy(), hi.copy()
+                    tl[u] = lo[u] + a * tile
+                    th[u] = min(hi[u], lo[u] + (a + 1) * tile)
+                    tl[v] = lo[v] + b * tile
+                    th[v] = min(hi[v], lo[v] + (b + 1) * tile)
+                    cut.append((tl, th, ax, sign, cl, sky))
+        faces = cut
+
+    F = len(faces)
+    face_lo = np.array([f[0] for f in faces], np.float32)
+    face_hi = np.array([f[1] for f in faces], np.float32)
+    face_normal = np.zeros((F, 3), np.float32)
+    for k, f in enumerate(faces):
+        face_normal[k, f[2]] = f[3]
+    face_cluster = np.array([f[4] for f in faces], np.int32)
+    face_sky = np.array([1 if f[5] else 0 for f in faces], np.uint8)
+    face_refl = rng.uniform(0.3, 0.7, size=(F, 3)).astype(np.float32)
+
+    v9 = np.zeros((F, 2, 9), np.float32)
+    ids = np.zeros((F, 2), np.int32)
+    for k, f in enumerate(faces):
+        w = _rect_winding(f[0], f[1], f[2], f[3])
+        v9[k, 0] = np.concatenate([w[0], w[1], w[2]])
+        v9[k, 1] = np.concatenate([w[0], w[2], w[3]])
+        ids[k] = (TRACE_ID_SKY if f[5] else TRACE_ID_OPAQUE) | k
+    casters = ShadowCasterSet(v9.reshape(-1, 9), ids.reshape(-1))
+
+    # Patches: one root per face, then each root's subtree in the order
+    # SubdividePatch appends it (both children, then child1's subtree, then child2's).
+    P_lo, P_hi, P_area, P_face, P_chop = [], [], [], [], []
+    P_child1, P_child2, P_parent = [], [], []
+
+    def new_patch(lo, hi, face_k, chop, parent):
+        ax = faces[face_k][2]
+        u, v = (ax + 1) % 3, (ax + 2) % 3
+        P_lo.append(lo.copy())
+        P_hi.append(hi.copy())
+        P_area.append(np.float32((hi[u] - lo[u]) * (hi[v] - lo[v])))
+        P_face.append(face_k)
+        P_chop.append(chop)
+        P_child1.append(-1)
+        P_child2.append(-1)
+        P_parent.append(parent)
+        return len(P_lo) - 1
+
+    for k, f in enumerate(faces):
+        new_patch(f[0], f[1], k, patch, -1)
+
+    def subdivide(p):
+        f = faces[P_face[p]]
+        if f[5]:  # sky patches are never split
+            return
+        total = P_hi[p] - P_lo[p]
+        chop = P_chop[p]
+        widest_axis, widest, split = -1, -1.0, False
+        for i in range(3):
+            t = total[i]
+            if t > widest:
+                widest_axis, widest = i, t
+            if t >= chop and t >= minchop:
+                split = True
+        if not split and widest_axis != -1:
+            wv = total[widest_axis]
+            if (wv > total[(widest_axis + 1) % 3] * 2 and wv > total[(widest_axis + 2) % 3] * 2
+                    and chop > minchop):
+                split = True
+                chop = max(minchop, chop / 2)
+                P_chop[p] = chop
+        if not split:
+            return
+        ax = widest_axis
+        dist = (P_lo[p][ax] + P_hi[p][ax]) * 0.5
+        lo1, hi1 = P_lo[p].copy(), P_hi[p].copy()
+        lo1[ax] = dist  # child1: the front of the split plane
+        lo2, hi2 = P_lo[p].copy(), P_hi[p].copy()
+        hi2[ax] = dist
+        kids = []
+        for lo, hi in ((lo1, hi1), (lo2, hi2)):
+            cchop = chop
+            t = hi - lo
+            if cchop > minchop and all(t[i] < cchop for i in range(3)):
+                fl, fh = f[0], f[1]
+                for i in range(3):
+                    if (fh[i] == hi[i] or fl[i] == lo[i]) and t[i] > minchop:
+                        cchop = max(minchop, cchop / 2)
+                        break
+            kids.append(new_patch(lo, hi, P_face[p], cchop, p))
+        P_child1[p], P_child2[p] = kids
+        subdivide(kids[0])
+        subdivide(kids[1])
+
+    if tile <= 0:
+        old = sys.getrecursionlimit()
+        sys.setrecursionlimit(max(old, 10000))
+        try:
+            for k in range(F):
+                subdivide(k)
+        finally:
+            sys.setrecursionlimit(old)
+
+    P = len(P_lo)
+    lo_a = np.array(P_lo)
+    hi_a = np.array(P_hi)
+    origin = (0.5 * (lo_a + hi_a)).astype(np.float32)
+    face_arr = np.array(P_face, np.int32)
+    normal = face_normal[face_arr].copy()
+    fplane = np.array([np.float32(face_normal[k] @ faces[k][0]) for k in range(F)], np.float32)
+    plane_dist = fplane[face_arr]
+    winding = np.zeros((P, 4, 3), np.float32)
+    for p in range(P):
+        f = faces[P_face[p]]
+        winding[p] = _rect_winding(P_lo[p], P_hi[p], f[2], f[3])
+    child1 = np.array(P_child1, np.int32)
+    child2 = np.array(P_child2, np.int32)
+    face_parent = np.full(F, -1, np.int32)
+    next_parent = np.full(P, -1, np.int32)
+    for p in range(F):  # root patches chained per face, prepended in index order
+        next_parent[p] = face_parent[P_face[p]]
+        face_parent[P_face[p]] = p
+    cluster = face_cluster[face_arr].copy()
+
+    row_ptr, row = [0], []
+    for c in range(nrooms):
+        i, j = c % nx, c // nx
+        neigh = ((c - 1, i > 0), (c + 1, i < nx - 1), (c - nx, j > 0), (c + nx, j < ny - 1))
+        for cc in sorted({c} | {nb for nb, ok in neigh if ok}):
+            row.extend(np.nonzero(face_cluster == cc)[0].tolist())
+        row_ptr.append(len(row))
+    leaf = child1 < 0
+    receivers = np.concatenate(
+        [np.nonzero(leaf & (cluster == c))[0] for c in range(nrooms)]
+    ).astype(np.int32)
+
+    lroom = rng.integers(0, nrooms, size=nlights)
+    lpos = room_lo[lroom] + 16.0 + rng.uniform(0, 1, size=(nlights, 3)) * (
+        room_hi[lroom] - room_lo[lroom] - 32.0)
+    intensity = rng.uniform(50.0, 300.0, size=nlights).astype(np.float32)
+    phi = rng.uniform(0, 2 * np.pi)
+    z = rng.uniform(0, 1)
+    rxy = np.sqrt(max(0.0, 1 - z * z))
+    to_sun = np.array([rxy * np.cos(phi), rxy * np.sin(phi), z], np.float32)
+
+    return RoomGrid(
+        casters=casters, room_lo=room_lo.astype(np.float32), room_hi=room_hi.astype(np.float32),
+        face_lo=face_lo, face_hi=face_hi, face_normal=face_normal, face_cluster=face_cluster,
+        face_sky=face_sky, face_refl=face_refl, face_parent=face_parent, origin=origin,
+        normal=normal, plane_dist=plane_dist, area=np.array(P_area, np.float32), child1=child1,
+        child2=child2, parent=np.array(P_parent, np.int32), next_parent=next_parent, face=face_arr,
+        cluster=cluster, winding=winding, row_faces_ptr=np.array(row_ptr, np.int32),
+        row_faces=np.array(row, np.int32), receivers=receivers,
+        lights=lpos.astype(np.float32), light_intensity=intensity, to_sun=to_sun,
+    )
+
+
+def bounce_grid(nx: int = 12, ny: int = 12, seed: int = 3) -> RoomGrid:
+    """The bounce config: a 12 x 12 room grid of 64-unit patches, faces cut to
+    the patch size (so every patch is its own source), rooms sized so the map
+    has 200,000 +- 2 % patches, two point lights per room."""
+    return room_grid(nx, ny, size=BOUNCE_ROOM, patch=64.0, minchop=64.0, tile=64.0,
+                     nlights=2 * nx * ny, seed=seed)
+
+
+def sample_points(g: RoomGrid) -> np.ndarray:
+    """Each patch's sample point: its origin one unit along its normal, as
+    FaceLightJob offsets samples (Rad/Light/FaceLightJob.cs:244-251)."""
+    return (g.origin + g.normal).astype(np.float32)
+
+
+def direct_light(g: RoomGrid, blocked: np.ndarray) -> np.ndarray:
+    """Point-light irradiance at every patch from its sample point: for each
+    light not blocked, intensity * max(0, n . l) / max(d, 1)^2 (quadratic
+    falloff, LightFalloff with c = l = 0, q = 1), parents pulled up.
+    ``blocked`` is one bool per (patch, light), patch-major."""
+    pts = sample_points(g).astype(np.float64)
+    L = g.lights.astype(np.float64)
+    blocked = np.asarray(blocked, bool).reshape(g.npatch, len(L))
+    e = np.zeros(g.npatch, np.float64)
+    for l in range(len(L)):
+        d = L[l] - pts
+        dist = np.sqrt((d * d).sum(1))
+        dot = np.maximum(0.0, (d * g.normal).sum(1) / np.maximum(dist, 1e-9))
+        e += np.where(blocked[:, l], 0.0, g.light_intensity[l] * dot / np.maximum(dist, 1.0) ** 2)
+    e = np.where(g.leaf & ~g.sky.astype(bool), e, 0.0)
+    e3 = np.repeat(e[:, None], 3, axis=1).astype(np.float32)
+    return g.pull_up(e3)
+
+
+BOUNCE_ROOM = (1344.0, 1344.0, 384.0)

@@ -145,6 +145,78 @@ typedef struct vh_stats {
 } vh_stats;
 int vh_get_stats(const vh_ctx *ctx, vh_stats *out);
 
+/* ---- The bounce: IVradBounceStage (SourceSharp.MapTools/Rad/VradStages.cs:112-122) ---- */
+
+/* The patch system of one lighting pass (SourceSharp.MapTools/Rad/Light/Patch.cs,
+ * PatchSet.cs), as the transfer build and the bounce read it:
+ *   per patch: origin (3), plane normal (3), plane distance, area, child1 and
+ *     child2 (-1 for a leaf; children have higher indices than parents), the
+ *     next root patch of the same face (-1), face, cluster, a four-corner
+ *     winding (12 floats, wound as vrad's), reflectivity (3);
+ *   per face: sky flag, first root patch (-1 if none);
+ *   per cluster: the faces of its PVS row, in order (VisMatrix.FacesInRow),
+ *     as row_faces_ptr (ncluster + 1 offsets) into row_faces;
+ *   receivers: the leaf patches that gather light, in receiver order. */
+typedef struct vh_patch_desc {
+    int32_t npatch, nface, ncluster, nreceivers;
+    const float *origin, *normal, *plane_dist, *area;
+    const int32_t *child1, *child2, *next_parent, *face, *cluster;
+    const float *winding;
+    const float *reflectivity;
+    const uint8_t *face_sky;
+    const int32_t *face_parent;
+    const int32_t *row_faces_ptr, *row_faces;
+    const int32_t *receivers;
+} vh_patch_desc;
+
+/* Uploads the patch system; replaces any previous one and its transfers. */
+int vh_patches_load(vh_ctx *ctx, const vh_patch_desc *desc);
+
+/* Builds every receiver's transfer list on the GPU against the loaded scene:
+ * VisMatrix.BuildAsync (SourceSharp.MapTools/Rad/Bounce/VisMatrix.cs:219-411) --
+ * candidate enumeration (Enumerator.Run, TestPatchToFace, TestPatchToPatch),
+ * one visibility ray per candidate (VisMatrix.TransferRay, StockExact),
+ * MakeTransfer with FormFactors.DiffToDiff / PolyToDiff, MakeScales.  Rows keep
+ * the reference's build order.  Outputs may be NULL. */
+int vh_transfers_build(vh_ctx *ctx, int64_t *transfers, int64_t *candidates, int64_t *blocked);
+
+/* Copies the transfer lists out as CSR: row_ptr (npatch + 1), src and weight
+ * (row_ptr[npatch] entries; capacity is their length). */
+int vh_transfers_read(vh_ctx *ctx, int64_t *row_ptr, int32_t *src, float *weight,
+                      int64_t capacity);
+
+/* Loads transfer lists built elsewhere (a TransferSet, or the TRF1 cache,
+ * SourceSharp.MapTools/Rad/Bounce/TransferSetCodec.cs:23-80): row_ptr has
+ * npatch + 1 entries. */
+int vh_transfers_load(vh_ctx *ctx, const int64_t *row_ptr, const int32_t *src,
+                      const float *weight);
+
+#define VH_WEIGHTS_F32 0
+#define VH_WEIGHTS_F16 1 /* FP16 weights with a per-row FP32 scale, FP32 accumulation */
+#define VH_BOUNCE_PROGRESSIVE 0 /* Radiosity.BounceAsync: emit = the last bounce's light */
+#define VH_BOUNCE_FROM_START 1  /* start from an estimate of the bounced light */
+
+typedef struct vh_bounce_opts {
+    int32_t weights;   /* VH_WEIGHTS_* */
+    int32_t mode;      /* VH_BOUNCE_* */
+    int32_t max_iter;  /* the bounce count (VradOptions.Bounces, default 100) */
+    int32_t use_graph; /* replay one captured iteration (hipGraph) */
+    double tol;        /* > 0: stop at relative residual <= tol (L1, FP64);
+                          <= 0: the reference's rule, every channel of the
+                          summed emission below 1.0 (Radiosity.cs:163) */
+} vh_bounce_opts;
+
+/* Radiosity.BounceAsync (SourceSharp.MapTools/Rad/Bounce/Radiosity.cs:132-170)
+ * for flat patches: MoveDirectLightToEmission, then per iteration GatherLight
+ * over every receiver's transfers and CollectLight (leaves take the gathered
+ * light, parents the area-weighted light of their children, sky emits 0).
+ * emit0: each patch's direct light (3 per patch); start: the bounced-light
+ * estimate for VH_BOUNCE_FROM_START (3 per patch; NULL = zero).  Writes
+ * total_out (3 per patch: the bounced light, parents pulled up), and per
+ * iteration the residual and the summed leaf emission (3 doubles). */
+int vh_bounce(vh_ctx *ctx, const vh_bounce_opts *opts, const float *emit0, const float *start,
+              float *total_out, double *residuals, double *energies, int32_t *iters);
+
 /* Kernel timing with HIP events recorded on the launching stream around each
  * launch of the named kernel (no counterpart in the reference: the bench and
  * the profiler read it).  Enable, run, then read: vh_timing_read waits for the
@@ -153,6 +225,8 @@ int vh_get_stats(const vh_ctx *ctx, vh_stats *out);
 #define VH_KERNEL_POINT_LIGHTS 0
 #define VH_KERNEL_VISIBILITY 1
 #define VH_KERNEL_CLOSEST 2
+#define VH_KERNEL_GATHER 3
+#define VH_KERNEL_TRANSFERS 4
 #define VH_KERNEL_COUNT 8
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);

@@ -81,6 +81,38 @@ void vc_bvh_point_lights(const vc_bvh *b, const float *points, int64_t npoints,
                          const float *lights, int32_t nlights, float tmin,
                          uint64_t *bits, int32_t threads);
 
+/* ---- transfers and bounce (vradcpu_bounce.c) ---- */
+
+/* The patch system the transfer build walks (Rad/Light/Patch.cs, PatchSet.cs):
+ * per patch its origin, plane normal, plane distance, area, children (-1),
+ * next root patch of the same face (-1), face, cluster and a 4-corner
+ * winding; per face its sky flag and first root patch; per cluster the faces
+ * of its PVS row in order (VisMatrix.FacesInRow); the receivers (leaf
+ * patches) in receiver order. */
+typedef struct vc_patchset {
+    int32_t npatch, nface, ncluster, nreceivers;
+    const float *origin, *normal, *plane_dist, *area;
+    const int32_t *child1, *child2, *next_parent, *face, *cluster;
+    const float *winding; /* 12 per patch */
+    const uint8_t *face_sky;
+    const int32_t *face_parent;
+    const int32_t *row_faces_ptr, *row_faces;
+    const int32_t *receivers;
+} vc_patchset;
+
+float vc_diff_to_diff(const float *o1, const float *n1, const float *o2, const float *n2);
+float vc_poly_to_diff(const float *poly, int32_t npts, float area, const float *o, const float *n);
+void vc_make_scales(float *w, int32_t n);
+int64_t vc_candidates(const vc_patchset *ps, int32_t receiver, int32_t *out, int64_t cap);
+int vc_make_transfer(const vc_patchset *ps, int32_t r, int32_t s, float *trans);
+void vc_transfers_build(const vc_patchset *ps, const vc_bvh *bvh, const int64_t *cand_ptr,
+                        int64_t *made, int32_t *src, float *w, int32_t threads);
+void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const float *w,
+               const float *refl, const float *emit0, const uint8_t *sky, const int32_t *child1,
+               const int32_t *child2, const float *area, const float *start, int32_t mode,
+               int32_t max_iter, double tol, int32_t use_double, double *total_out,
+               double *residuals, double *energies, int32_t *iters);
+
 #ifdef __cplusplus
 }
 #endif
