@@ -829,7 +829,7 @@ int vh_transfers_read(vh_ctx *c, int64_t *row_ptr, int32_t *src, float *weight, 
     int rc = check_loaded(c, true);
     if (rc) return rc;
     BounceData *B = c->bounce;
-    if (capacity < B->ntransfers) {
+    if ((src || weight) && capacity < B->ntransfers) {
         set_error(std::to_string(B->ntransfers) + " transfers need as many slots, and " +
                   std::to_string(capacity) + " were given");
         return VH_EINVAL;

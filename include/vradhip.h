@@ -181,7 +181,8 @@ int vh_patches_load(vh_ctx *ctx, const vh_patch_desc *desc);
 int vh_transfers_build(vh_ctx *ctx, int64_t *transfers, int64_t *candidates, int64_t *blocked);
 
 /* Copies the transfer lists out as CSR: row_ptr (npatch + 1), src and weight
- * (row_ptr[npatch] entries; capacity is their length). */
+ * (row_ptr[npatch] entries; capacity is their length).  With src and weight
+ * NULL only row_ptr is written, so a caller can size the arrays first. */
 int vh_transfers_read(vh_ctx *ctx, int64_t *row_ptr, int32_t *src, float *weight,
                       int64_t capacity);
 
