@@ -112,6 +112,11 @@ SIGNATURES = {
     "vh_transfers_read": (ctypes.c_int, [P, P, P, P, I64]),
     "vh_transfers_load": (ctypes.c_int, [P, P, P, P]),
     "vh_bounce": (ctypes.c_int, [P, ctypes.POINTER(VhBounceOpts), P, P, P, P, P, ctypes.POINTER(I32)]),
+    "vh_gather_load": (ctypes.c_int, [P, P]),
+    "vh_gather_run": (ctypes.c_int, [P, I32, I32]),
+    "vh_gather_read": (ctypes.c_int, [P, P, P, P]),
+    "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
+    "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
     "vh_timing_enable": (ctypes.c_int, [P, I32]),
     "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
     "vh_count_enable": (ctypes.c_int, [P, I32]),
@@ -123,6 +128,7 @@ VH_KERNEL_VISIBILITY = 1
 VH_KERNEL_CLOSEST = 2
 VH_KERNEL_GATHER = 3
 VH_KERNEL_TRANSFERS = 4
+VH_KERNEL_FACELIGHT = 5
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1

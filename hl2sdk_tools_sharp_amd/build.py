@@ -27,6 +27,7 @@ SOURCES = [
     "vh_bvh.cpp",
     "vh_trace.hip",
     "vh_bounce.hip",
+    "vh_gather.hip",
 ]
 
 FLAGS = [

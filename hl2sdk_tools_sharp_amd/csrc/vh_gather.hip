@@ -1,0 +1,778 @@
+// The direct-light gather and the RGBE encode on gfx950.
+//
+// One lane per sample.  Lanes are grouped four by four exactly as the
+// reference's SampleGroup (FaceLightJob.EmitGroup, Rad/Light/FaceLightJob.cs:329-398):
+// a group is four consecutive samples of one face, a short last group repeats
+// its last sample in the padding lanes.  Decisions the reference makes per
+// group (the sun's "all four dots are zero", ResolveGroup's "every fxdot is
+// zero, skip the light" that also gates the direct-sun amount of every lane)
+// are 4-lane shuffles.  Per light, in list order, each lane computes
+// DirectLightGatherer's dot and falloff, traces its own visibility or sky rays
+// inline through the BVH (vh_traverse.h) and adds (dot * mask) * falloff *
+// intensity to its LightingValue per normal (LDR and HDR together: the two
+// passes differ only in intensity).
+//
+// Scheduling: a wave takes a chunk of 16 groups (64 samples).  The persistent
+// form launches a fixed number of waves that pull chunks from an atomic
+// ticket; the grid form launches one wave per chunk; the per-face form is one
+// launch per face, the unbatched baseline.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vh_traverse.h"
+
+#pragma clang fp contract(off)
+
+struct GatherData {
+    int32_t nfaces = 0, nsamples = 0, nlights = 0, pvs_bytes = 0;
+    int32_t sun_samples = 1, sky_count = 0, ao_count = 0;
+    float sun_extent = 0.0f, ao_radius = 0.0f;
+    int64_t ngroups = 0, nchunks = 0;
+    std::vector<int32_t> h_face_ptr, h_normals;
+    std::vector<int64_t> h_face_chunk_ptr;  // chunks of face f: [ptr[f], ptr[f+1])
+    vh::DevBuf face_ptr, face_nc, face_normals, pos, cluster, lights, pvs, jitter, sky, ao_dirs;
+    vh::DevBuf chunk_face, chunk_first;  // per chunk of 16 groups within one face
+    vh::DevBuf out_ldr, out_hdr, out_ao, ticket;
+    bool have_results = false;
+};
+
+namespace vh {
+
+void gather_release(vh_ctx *c)
+{
+    GatherData *G = c->gather;
+    if (!G) return;
+    for (DevBuf *b : {&G->face_ptr, &G->face_nc, &G->face_normals, &G->pos, &G->cluster,
+                      &G->lights, &G->pvs, &G->jitter, &G->sky, &G->ao_dirs, &G->chunk_face,
+                      &G->chunk_first, &G->out_ldr, &G->out_hdr, &G->out_ao, &G->ticket})
+        b->release();
+    delete G;
+    c->gather = nullptr;
+}
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr float kMaxTrace = (float)(1.732050807569 * (2 * 16384));  // LightConstants.MaxTraceLength
+constexpr float kDistEpsilon = 0.03125f;                           // LightConstants.DistEpsilon
+constexpr float kEqualEpsilon = 0.001f;                            // LightConstants.EqualEpsilon
+
+struct GatherArgs {
+    const int32_t *face_ptr;
+    const int32_t *face_nc;
+    const float *face_normals;  // 12 per face
+    const float *pos;
+    const int32_t *cluster;
+    const vh_light *lights;
+    int32_t nlights;
+    const uint8_t *pvs;
+    int32_t pvs_bytes;
+    int32_t nsamples;
+    float sun_extent;
+    int32_t sun_samples;
+    const float *jitter;
+    int32_t sky_count;
+    const float *sky;
+    int32_t ao_count;
+    float ao_radius;
+    const float *ao_dirs;
+    const int32_t *chunk_face;
+    const int32_t *chunk_first;
+    int64_t nchunks;
+    float4 *out_ldr;  // 4 normal slots x nsamples
+    float4 *out_hdr;
+    float *out_ao;
+    const float4 *nodes;
+    const float4 *tris;
+};
+
+__device__ __forceinline__ float maxps(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float minps(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx) + (ay * by) + (az * bz);
+}
+
+// any lane of this lane's 4-lane group
+__device__ __forceinline__ bool group_any(bool v)
+{
+    int x = v ? 1 : 0;
+    x |= __shfl_xor(x, 1, 64);
+    x |= __shfl_xor(x, 2, 64);
+    return x != 0;
+}
+__device__ __forceinline__ bool group_all(bool v) { return !group_any(!v); }
+
+__device__ __forceinline__ bool pvs_check(const GatherArgs &a, int32_t light, int32_t cl)
+{
+    if (cl < 0 || a.pvs_bytes == 0 || cl >= a.pvs_bytes * 8) return true;
+    return (a.pvs[(size_t)light * a.pvs_bytes + (cl >> 3)] >> (cl & 7)) & 1;
+}
+
+// LightRayLog.MakeRay (correct mode): the whole segment, reach 1.
+__device__ __forceinline__ Ray segment(float px, float py, float pz, float ex, float ey, float ez)
+{
+    Ray r;
+    r.ox = px;
+    r.oy = py;
+    r.oz = pz;
+    r.dx = ex - px;
+    r.dy = ey - py;
+    r.dz = ez - pz;
+    r.reach = 1.0f;
+    return r;
+}
+
+__device__ __forceinline__ bool blocked(const GatherArgs &a, int *stk, const Ray &r)
+{
+    if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
+          isfinite(r.oy) && isfinite(r.oz)))
+        return false;
+    float t;
+    int32_t id;
+    return trace_one<kAny, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id) >= 0;
+}
+
+// LightRayLog.Occlusion: the first hit short of the end, unless it is sky.
+__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *stk, const Ray &r)
+{
+    if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
+          isfinite(r.oy) && isfinite(r.oz)))
+        return 0.0f;
+    float t;
+    int32_t id;
+    int32_t h = trace_one<kClosestShort, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id);
+    return (h >= 0 && (id & VH_TRACE_ID_SKY) == 0) ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ float inverse_quadratic(const vh_light &l, float d)
+{
+    float f = d * d;
+    f *= l.quadratic_attn;
+    f += l.linear_attn * d;
+    f += l.constant_attn;
+    return 1.0f / f;
+}
+
+// One wave lights one chunk: 16 groups of 4 samples of one face.  NN is the
+// face's normal count (1, or 4 when bumped), so every per-normal array is in
+// registers.
+template <int NN>
+__device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
+{
+    int lane = threadIdx.x & 63;
+    int sub = lane & 3;
+    int32_t f = a.chunk_face[chunk];
+    int32_t gfirst = a.chunk_first[chunk] + (lane >> 2) * 4;  // this lane's group
+    int32_t fend = a.face_ptr[f + 1];
+    bool group_live = gfirst < fend;
+    int count = group_live ? min(4, fend - gfirst) : 1;
+    bool real = group_live && sub < count;
+    int32_t s = group_live ? gfirst + min(sub, count - 1) : a.face_ptr[f];
+    constexpr int normals = NN;
+    const float *fnp = a.face_normals + (size_t)f * 12;
+    float n0x = fnp[0], n0y = fnp[1], n0z = fnp[2];
+    float bn[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 3; c++) bn[k][c] = fnp[3 + 3 * k + c];
+    // ComputeIlluminationPointAndNormals: one unit off the face
+    float px = a.pos[3 * (size_t)s] + n0x;
+    float py = a.pos[3 * (size_t)s + 1] + n0y;
+    float pz = a.pos[3 * (size_t)s + 2] + n0z;
+    int32_t cl = a.cluster[s];
+
+    float4 acc_l[4], acc_h[4];
+    for (int n = 0; n < 4; n++) {
+        acc_l[n] = make_float4(0, 0, 0, 0);
+        acc_h[n] = make_float4(0, 0, 0, 0);
+    }
+
+    for (int32_t li = 0; li < a.nlights; li++) {
+        const vh_light &L = a.lights[li];
+        bool inmask = real && pvs_check(a, li, cl);
+        if (!group_any(inmask)) continue;
+        float dot[4] = {0, 0, 0, 0};
+        float falloff = 0.0f, sun = 0.0f;
+        if (L.type == VH_EMIT_POINT || L.type == VH_EMIT_SURFACE || L.type == VH_EMIT_SPOTLIGHT) {
+            // GatherStandardLane (DirectLightGatherer.cs:1169-1306)
+            float sx = L.origin[0], sy = L.origin[1], sz = L.origin[2];
+            float dx = sx - px, dy = sy - py, dz = sz - pz;
+            float dist2 = (dx * dx) + (dy * dy) + (dz * dz);
+            float rpc = 1.0f / sqrtf(dist2);
+            dx = dx * rpc;
+            dy = dy * rpc;
+            dz = dz * rpc;
+            float dist = sqrtf(dist2);
+            float d0 = maxps(0.0f, dot3(dx, dy, dz, n0x, n0y, n0z));
+            bool hard = L.end_fade > L.start_fade;
+            bool ok = !(hard && !(dist <= L.end_fade));
+            dist = maxps(dist, 1.0f);
+            float eval = minps(dist, L.cap_dist);
+            float fo = 0.0f;
+            if (ok) {
+                if (L.type == VH_EMIT_POINT) {
+                    fo = inverse_quadratic(L, eval);
+                } else if (L.type == VH_EMIT_SURFACE) {
+                    float dot2 = maxps(0.0f, -dot3(dx, dy, dz, L.normal[0], L.normal[1], L.normal[2]));
+                    if (d0 == 0.0f) ok = false;
+                    fo = dot2 / dist2;
+                    sx = sx + (L.normal[0] * kDistEpsilon);
+                    sy = sy + (L.normal[1] * kDistEpsilon);
+                    sz = sz + (L.normal[2] * kDistEpsilon);
+                } else {
+                    float dot2 = -dot3(dx, dy, dz, L.normal[0], L.normal[1], L.normal[2]);
+                    if (!(dot2 > L.stopdot2)) {
+                        ok = false;
+                    } else {
+                        fo = inverse_quadratic(L, eval) * dot2;
+                        bool fringe = dot2 <= L.stopdot;
+                        float mult = (dot2 - L.stopdot2) / (L.stopdot - L.stopdot2);
+                        mult = minps(mult, 1.0f);
+                        mult = maxps(mult, 0.0f);
+                        if (L.exponent != 0.0f && L.exponent != 1.0f)
+                            mult = (float)pow((double)mult, (double)L.exponent);
+                        mult = fringe ? mult : 1.0f;
+                        fo = mult * fo;
+                    }
+                }
+            }
+            if (ok && hard) {
+                float t = (dist - L.start_fade) / (L.end_fade - L.start_fade);
+                t = minps(t, 1.0f);
+                t = maxps(t, 0.0f);
+                t = 1.0f - t;
+                float mult = (6.0f * t) - 15.0f;
+                mult = (mult * t) + 10.0f;
+                mult = (t * t) * mult;
+                mult = t * mult;
+                fo = mult * fo;
+            }
+            if (ok) {
+                dot[0] = d0;
+                falloff = fo;
+                _Pragma("unroll") for (int n = 1; n < normals; n++)
+                    dot[n] = maxps(0.0f, dot3(bn[n - 1][0], bn[n - 1][1], bn[n - 1][2], dx, dy, dz));
+                if (inmask && d0 != 0.0f && fo != 0.0f) {
+                    if (blocked(a, stk, segment(px, py, pz, sx, sy, sz))) dot[0] = 0.0f * dot[0];
+                }
+            }
+        } else if (L.type == VH_EMIT_SKYLIGHT) {
+            // EmitSkyLight / ResolveSkyLight (DirectLightGatherer.cs:716-845)
+            float dl = maxps(-dot3(n0x, n0y, n0z, L.normal[0], L.normal[1], L.normal[2]), 0.0f);
+            if (!group_all(dl == 0.0f)) {
+                float js = (float)(1.732050807569 * (2 * 16384) * (double)a.sun_extent);
+                float total = 0.0f;
+                for (int d = 0; d < a.sun_samples; d++) {
+                    float ex = L.normal[0] * -kMaxTrace, ey = L.normal[1] * -kMaxTrace,
+                          ez = L.normal[2] * -kMaxTrace;
+                    if (d != 0) {
+                        const float *jt = a.jitter + 3 * (size_t)(d - 1);
+                        ex = ex + (jt[0] * js);
+                        ey = ey + (jt[1] * js);
+                        ez = ez + (jt[2] * js);
+                    }
+                    float occ = 0.0f;
+                    if (real) occ = sky_occlusion(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz));
+                    total += 1.0f - minps(maxps(occ, 0.0f), 1.0f);
+                }
+                float see = total * (1.0f / (float)a.sun_samples);
+                dot[0] = dl * see;
+                falloff = 1.0f;
+                sun = see * 10000.0f;
+                _Pragma("unroll") for (int n = 1; n < normals; n++)
+                    dot[n] = (-dot3(bn[n - 1][0], bn[n - 1][1], bn[n - 1][2], L.normal[0],
+                                    L.normal[1], L.normal[2])) *
+                             see;
+            }
+        } else if (L.type == VH_EMIT_SKYAMBIENT) {
+            // AmbientSky (DirectLightGatherer.cs:847-1012), no 3D skybox
+            float sumdot = 0.0f;
+            float amb[4] = {0, 0, 0, 0}, poss[4] = {0, 0, 0, 0};
+            for (int32_t k = 0; k < a.sky_count; k++) {
+                float ax = a.sky[3 * k], ay = a.sky[3 * k + 1], az = a.sky[3 * k + 2];
+                float dd = -dot3(n0x, n0y, n0z, ax, ay, az);
+                bool valid = dd > kEqualEpsilon;  // the same for every lane of a flat face
+                if (!group_any(valid)) continue;
+                float dts[4];
+                dts[0] = valid ? dd : 0.0f;
+                sumdot = dts[0] + sumdot;
+                poss[0] = (valid ? 1.0f : 0.0f) + poss[0];
+                bool tr = valid;
+                _Pragma("unroll") for (int n = 1; n < normals; n++) {
+                    float d2 = -dot3(bn[n - 1][0], bn[n - 1][1], bn[n - 1][2], ax, ay, az);
+                    bool v2 = d2 > kEqualEpsilon;
+                    dts[n] = v2 ? d2 : 0.0f;
+                    poss[n] = (valid && v2 ? 1.0f : 0.0f) + poss[n];
+                    tr = tr || v2;
+                }
+                float frac = 1.0f;
+                if (real && tr && inmask)
+                    frac = 1.0f - sky_occlusion(a, stk,
+                                                segment(px, py, pz, (ax * -kMaxTrace) + px,
+                                                        (ay * -kMaxTrace) + py,
+                                                        (az * -kMaxTrace) + pz));
+                _Pragma("unroll") for (int n = 0; n < normals; n++) amb[n] = amb[n] + (frac * dts[n]);
+            }
+            falloff = 1.0f;
+            _Pragma("unroll") for (int n = 0; n < normals; n++) {
+                float factor = poss[n] / poss[0];
+                dot[n] = amb[n] / (factor * sumdot);
+            }
+        } else {
+            continue;
+        }
+        // Resolve: a light behind the face lights no bump direction
+        float d0 = maxps(dot[0], 0.0f);
+        dot[0] = d0;
+        _Pragma("unroll") for (int n = 1; n < normals; n++) dot[n] = d0 > 0.0f ? maxps(dot[n], 0.0f) : 0.0f;
+        // ResolveGroup: (dot * mask) * falloff; a group whose every fxdot is 0 skips the light
+        float fx[4];
+        bool nz = false;
+        _Pragma("unroll") for (int n = 0; n < normals; n++) {
+            float masked = inmask ? dot[n] : 0.0f;
+            fx[n] = masked * falloff;
+            nz = nz || fx[n] != 0.0f;
+        }
+        if (!group_any(nz)) continue;
+        if (real) {
+            _Pragma("unroll") for (int n = 0; n < normals; n++) {
+                acc_l[n].x = acc_l[n].x + (fx[n] * L.intensity[0]);
+                acc_l[n].y = acc_l[n].y + (fx[n] * L.intensity[1]);
+                acc_l[n].z = acc_l[n].z + (fx[n] * L.intensity[2]);
+                acc_l[n].w += sun;
+                acc_h[n].x = acc_h[n].x + (fx[n] * L.intensity_hdr[0]);
+                acc_h[n].y = acc_h[n].y + (fx[n] * L.intensity_hdr[1]);
+                acc_h[n].z = acc_h[n].z + (fx[n] * L.intensity_hdr[2]);
+                acc_h[n].w += sun;
+            }
+        }
+    }
+    if (real) {
+        _Pragma("unroll") for (int n = 0; n < normals; n++) {
+            a.out_ldr[(size_t)n * a.nsamples + s] = acc_l[n];
+            a.out_hdr[(size_t)n * a.nsamples + s] = acc_h[n];
+        }
+        if (a.ao_count > 0) {
+            int unblocked = 0;
+            const float *dirs = a.ao_dirs + (size_t)f * a.ao_count * 3;
+            for (int32_t k = 0; k < a.ao_count; k++) {
+                Ray r;
+                r.ox = px;
+                r.oy = py;
+                r.oz = pz;
+                r.dx = dirs[3 * k] * a.ao_radius;
+                r.dy = dirs[3 * k + 1] * a.ao_radius;
+                r.dz = dirs[3 * k + 2] * a.ao_radius;
+                r.reach = 1.0f;
+                if (!blocked(a, stk, r)) unblocked++;
+            }
+            a.out_ao[s] = (float)unblocked / (float)a.ao_count;
+        }
+    }
+}
+
+__device__ __forceinline__ void light_chunk(const GatherArgs &a, int64_t chunk, int *stk)
+{
+    if (a.face_nc[a.chunk_face[chunk]] == 4)
+        light_chunk_n<4>(a, chunk, stk);
+    else
+        light_chunk_n<1>(a, chunk, stk);
+}
+
+template <bool PERSISTENT>
+__global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first_chunk,
+                                                     unsigned long long *__restrict__ ticket)
+{
+    __shared__ int stack_mem[kMaxDepth * kWave];
+    int *stk = stack_mem + threadIdx.x;
+    if (!PERSISTENT) {
+        light_chunk(a, first_chunk + blockIdx.x, stk);
+        return;
+    }
+    for (;;) {
+        unsigned long long c = 0;
+        if (threadIdx.x == 0) c = atomicAdd(ticket, 1ull);
+        c = __shfl(c, 0, 64);
+        if ((int64_t)c >= a.nchunks) break;
+        light_chunk(a, (int64_t)c, stk);
+    }
+}
+
+// StockLightColor.Encode: the largest channel (ties to the later one) sets
+// the exponent, biased exponent - 134 (0 for a zero maximum); each channel is
+// multiplied by 2^-exponent and truncated to a byte as .NET's saturating
+// float -> int conversion does.
+__device__ __forceinline__ int32_t to_int_sat(float x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)x;
+}
+
+__device__ __forceinline__ uint32_t encode_one(float r, float g, float b)
+{
+    float mx = r > g ? (r > b ? r : b) : (g > b ? g : b);
+    int e = 0;
+    if (mx != 0.0f) e = (int)((__float_as_uint(mx) & 0x7F800000u) >> 23) - (7 + 127);
+    float scalar = __uint_as_float((uint32_t)(127 - e) << 23);
+    uint32_t R = (uint8_t)to_int_sat(r * scalar);
+    uint32_t G = (uint8_t)to_int_sat(g * scalar);
+    uint32_t B = (uint8_t)to_int_sat(b * scalar);
+    uint32_t E = (uint8_t)(int8_t)e;
+    return R | (G << 8) | (B << 16) | (E << 24);
+}
+
+__global__ void k_encode(const float *__restrict__ rgb, int64_t n, int32_t stride,
+                         uint32_t *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *v = rgb + i * stride;
+    out[i] = encode_one(v[0], v[1], v[2]);
+}
+
+// The lump of one gather: thread per (face sample, normal).
+__global__ void k_encode_lump(const float4 *__restrict__ lv, int32_t nsamples,
+                              const int32_t *__restrict__ face_of, const int32_t *__restrict__ face_ptr,
+                              const int32_t *__restrict__ face_nc,
+                              const int64_t *__restrict__ face_base, uint32_t *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nsamples * 4) return;
+    int32_t s = (int32_t)(i % nsamples);
+    int32_t b = (int32_t)(i / nsamples);
+    int32_t f = face_of[s];
+    if (b >= face_nc[f]) return;
+    int32_t j = s - face_ptr[f];
+    int32_t ns = face_ptr[f + 1] - face_ptr[f];
+    float4 v = lv[(size_t)b * nsamples + s];
+    out[face_base[f] / 4 + (int64_t)b * ns + j] = encode_one(v.x, v.y, v.z);
+}
+
+template <typename T>
+int upload(DevBuf &b, const T *host, size_t count, const char *what, hipStream_t s)
+{
+    int rc = b.ensure(count * sizeof(T) + 16, what);
+    if (rc) return rc;
+    if (count) VH_HIP(hipMemcpyAsync(b.ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return VH_OK;
+}
+
+GatherArgs gather_args(vh_ctx *c)
+{
+    GatherData *G = c->gather;
+    GatherArgs a;
+    a.face_ptr = (const int32_t *)G->face_ptr.ptr;
+    a.face_nc = (const int32_t *)G->face_nc.ptr;
+    a.face_normals = (const float *)G->face_normals.ptr;
+    a.pos = (const float *)G->pos.ptr;
+    a.cluster = (const int32_t *)G->cluster.ptr;
+    a.lights = (const vh_light *)G->lights.ptr;
+    a.nlights = G->nlights;
+    a.pvs = (const uint8_t *)G->pvs.ptr;
+    a.pvs_bytes = G->pvs_bytes;
+    a.nsamples = G->nsamples;
+    a.sun_extent = G->sun_extent;
+    a.sun_samples = G->sun_samples;
+    a.jitter = (const float *)G->jitter.ptr;
+    a.sky_count = G->sky_count;
+    a.sky = (const float *)G->sky.ptr;
+    a.ao_count = G->ao_count;
+    a.ao_radius = G->ao_radius;
+    a.ao_dirs = (const float *)G->ao_dirs.ptr;
+    a.chunk_face = (const int32_t *)G->chunk_face.ptr;
+    a.chunk_first = (const int32_t *)G->chunk_first.ptr;
+    a.nchunks = G->nchunks;
+    a.out_ldr = (float4 *)G->out_ldr.ptr;
+    a.out_hdr = (float4 *)G->out_hdr.ptr;
+    a.out_ao = (float *)G->out_ao.ptr;
+    a.nodes = (const float4 *)c->nodes.ptr;
+    a.tris = (const float4 *)c->tris.ptr;
+    return a;
+}
+
+}  // namespace
+}  // namespace vh
+
+using namespace vh;
+
+extern "C" {
+
+int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
+{
+    clear_error();
+    if (!c || !d) {
+        set_error("NULL context or descriptor");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int32_t F = d->nfaces, S = d->nsamples, NL = d->nlights;
+    if (F < 0 || S < 0 || NL < 0 || d->pvs_bytes < 0 || (F > 0 && (!d->face_sample_ptr ||
+        !d->face_normal_count || !d->face_normals)) || (S > 0 && (!d->sample_pos ||
+        !d->sample_cluster)) || (NL > 0 && !d->lights) || (d->pvs_bytes > 0 && NL > 0 &&
+        !d->light_pvs)) {
+        set_error("incomplete gather descriptor");
+        return VH_EINVAL;
+    }
+    if (F > 0 && (d->face_sample_ptr[0] != 0 || d->face_sample_ptr[F] != S)) {
+        set_error("face_sample_ptr must run from 0 to nsamples");
+        return VH_EINVAL;
+    }
+    for (int32_t f = 0; f < F; f++) {
+        if (d->face_sample_ptr[f + 1] < d->face_sample_ptr[f]) {
+            set_error("face_sample_ptr is not monotone");
+            return VH_EINVAL;
+        }
+        if (d->face_normal_count[f] != 1 && d->face_normal_count[f] != 4) {
+            set_error("a face has 1 normal, or 4 when bumped");
+            return VH_EINVAL;
+        }
+    }
+    for (int32_t l = 0; l < NL; l++) {
+        const vh_light &L = d->lights[l];
+        if (L.style != 0) {
+            set_error("light " + std::to_string(l) + " has style " + std::to_string(L.style) +
+                      "; only style 0 is supported");
+            return VH_ENOTSUP;
+        }
+        if (L.type != VH_EMIT_POINT && L.type != VH_EMIT_SURFACE && L.type != VH_EMIT_SPOTLIGHT &&
+            L.type != VH_EMIT_SKYLIGHT && L.type != VH_EMIT_SKYAMBIENT) {
+            set_error("light " + std::to_string(l) + " has an unsupported type");
+            return VH_ENOTSUP;
+        }
+    }
+    int32_t ns = d->sun_samples > 0 ? d->sun_samples : 1;
+    if (ns > 1 && !d->sun_jitter) {
+        set_error("sun_samples > 1 needs sun_jitter");
+        return VH_EINVAL;
+    }
+    if ((d->sky_dirs_count > 0 && !d->sky_dirs) || (d->ao_count > 0 && !d->ao_dirs)) {
+        set_error("missing sky or ambient-occlusion directions");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    gather_release(c);
+    GatherData *G = new GatherData();
+    c->gather = G;
+    G->nfaces = F;
+    G->nsamples = S;
+    G->nlights = NL;
+    G->pvs_bytes = NL > 0 ? d->pvs_bytes : 0;
+    G->sun_extent = d->sun_extent;
+    G->sun_samples = ns;
+    G->sky_count = d->sky_dirs_count > 0 ? d->sky_dirs_count : 0;
+    G->ao_count = d->ao_count > 0 ? d->ao_count : 0;
+    G->ao_radius = d->ao_radius;
+    G->h_face_ptr.assign(d->face_sample_ptr, d->face_sample_ptr + F + 1);
+    G->h_normals.assign(d->face_normal_count, d->face_normal_count + F);
+    // chunks: 16 groups (64 samples) of one face each
+    std::vector<int32_t> cf, cs;
+    G->h_face_chunk_ptr.assign(F + 1, 0);
+    for (int32_t f = 0; f < F; f++) {
+        for (int32_t s = d->face_sample_ptr[f]; s < d->face_sample_ptr[f + 1]; s += 64) {
+            cf.push_back(f);
+            cs.push_back(s);
+        }
+        G->h_face_chunk_ptr[f + 1] = (int64_t)cf.size();
+    }
+    G->nchunks = (int64_t)cf.size();
+    hipStream_t st = c->stream;
+    int rc;
+    if ((rc = upload(G->face_ptr, d->face_sample_ptr, (size_t)F + 1, "face samples", st)) ||
+        (rc = upload(G->face_nc, d->face_normal_count, (size_t)F, "face normal counts", st)) ||
+        (rc = upload(G->face_normals, d->face_normals, (size_t)F * 12, "face normals", st)) ||
+        (rc = upload(G->pos, d->sample_pos, (size_t)S * 3, "sample positions", st)) ||
+        (rc = upload(G->cluster, d->sample_cluster, (size_t)S, "sample clusters", st)) ||
+        (rc = upload(G->lights, d->lights, (size_t)NL, "lights", st)) ||
+        (rc = upload(G->pvs, d->light_pvs, (size_t)NL * G->pvs_bytes, "light PVS", st)) ||
+        (rc = upload(G->jitter, d->sun_jitter, (size_t)(ns - 1) * 3, "sun jitter", st)) ||
+        (rc = upload(G->sky, d->sky_dirs, (size_t)G->sky_count * 3, "sky directions", st)) ||
+        (rc = upload(G->ao_dirs, d->ao_dirs, (size_t)F * G->ao_count * 3, "AO directions", st)) ||
+        (rc = upload(G->chunk_face, cf.data(), cf.size(), "chunks", st)) ||
+        (rc = upload(G->chunk_first, cs.data(), cs.size(), "chunks", st)) ||
+        (rc = G->out_ldr.ensure((size_t)S * 4 * 16 + 16, "LDR lighting")) ||
+        (rc = G->out_hdr.ensure((size_t)S * 4 * 16 + 16, "HDR lighting")) ||
+        (rc = G->out_ao.ensure((size_t)S * 4 + 16, "ambient occlusion")) ||
+        (rc = G->ticket.ensure(16, "queue ticket")))
+        return rc;
+    VH_HIP(hipStreamSynchronize(st));
+    return VH_OK;
+}
+
+int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather) {
+        set_error("no gather loaded: call vh_gather_load first");
+        return VH_ESTATE;
+    }
+    if (!c->have_scene) {
+        set_error("no scene loaded: the gather traces against vh_scene_load's casters");
+        return VH_ESTATE;
+    }
+    if (schedule < VH_SCHEDULE_QUEUE || schedule > VH_SCHEDULE_GRID) {
+        set_error("bad schedule");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    GatherData *G = c->gather;
+    hipStream_t st = c->stream;
+    GatherArgs a = gather_args(c);
+    size_t outb = (size_t)G->nsamples * 4 * 16;
+    VH_HIP(hipMemsetAsync(G->out_ldr.ptr, 0, outb, st));
+    VH_HIP(hipMemsetAsync(G->out_hdr.ptr, 0, outb, st));
+    VH_HIP(hipMemsetAsync(G->out_ao.ptr, 0, (size_t)G->nsamples * 4, st));
+    if (G->nchunks > 0) {
+        KernelTimer timer(c, VH_KERNEL_FACELIGHT, st);
+        if (schedule == VH_SCHEDULE_QUEUE) {
+            hipDeviceProp_t prop;
+            VH_HIP(hipGetDeviceProperties(&prop, c->device));
+            int wpc = waves_per_cu > 0 ? waves_per_cu : 20;
+            int64_t waves = (int64_t)prop.multiProcessorCount * wpc;
+            if (waves > G->nchunks) waves = G->nchunks;
+            VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 8, st));
+            hipLaunchKernelGGL(k_facelight<true>, dim3((uint32_t)waves), dim3(kWave), 0, st, a,
+                               (int64_t)0, (unsigned long long *)G->ticket.ptr);
+        } else if (schedule == VH_SCHEDULE_GRID) {
+            hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)G->nchunks), dim3(kWave), 0, st,
+                               a, (int64_t)0, nullptr);
+        } else {
+            for (int32_t f = 0; f < G->nfaces; f++) {
+                int64_t a0 = G->h_face_chunk_ptr[f], n = G->h_face_chunk_ptr[f + 1] - a0;
+                if (n > 0)
+                    hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)n), dim3(kWave), 0, st, a,
+                                       a0, nullptr);
+            }
+        }
+        timer.done();
+        VH_HIP(hipGetLastError());
+    }
+    VH_HIP(hipStreamSynchronize(st));
+    G->have_results = true;
+    return VH_OK;
+}
+
+int vh_gather_read(vh_ctx *c, float *ldr, float *hdr, float *ao)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather || !c->gather->have_results) {
+        set_error("no gather results: call vh_gather_run first");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    GatherData *G = c->gather;
+    size_t outb = (size_t)G->nsamples * 4 * 16;
+    if (ldr) VH_HIP(hipMemcpy(ldr, G->out_ldr.ptr, outb, hipMemcpyDeviceToHost));
+    if (hdr) VH_HIP(hipMemcpy(hdr, G->out_hdr.ptr, outb, hipMemcpyDeviceToHost));
+    if (ao) VH_HIP(hipMemcpy(ao, G->out_ao.ptr, (size_t)G->nsamples * 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_encode_rgbe(vh_ctx *c, const float *rgb, int64_t n, int32_t stride, uint8_t *out)
+{
+    clear_error();
+    if (!c || (n > 0 && (!rgb || !out)) || n < 0 || stride < 3) {
+        set_error("bad encode arguments");
+        return VH_EINVAL;
+    }
+    if (n == 0) return VH_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    VH_HIP(hipSetDevice(c->device));
+    DevBuf in, o;
+    int rc;
+    if ((rc = in.ensure((size_t)n * stride * 4, "colours")) || (rc = o.ensure((size_t)n * 4, "RGBE"))) {
+        in.release();
+        o.release();
+        return rc;
+    }
+    hipStream_t st = c->stream;
+    hipError_t e = hipMemcpyAsync(in.ptr, rgb, (size_t)n * stride * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_encode, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                           (const float *)in.ptr, n, stride, (uint32_t *)o.ptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    in.release();
+    o.release();
+    if (e != hipSuccess) {
+        set_error(std::string("encode failed: ") + hipGetErrorString(e));
+        return VH_EDEVICE;
+    }
+    return VH_OK;
+}
+
+int vh_lightmap_encode(vh_ctx *c, int32_t which, uint8_t *out, int64_t capacity)
+{
+    clear_error();
+    if (!c || !out || (which != 0 && which != 1)) {
+        set_error("bad lump encode arguments");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather || !c->gather->have_results) {
+        set_error("no gather results: call vh_gather_run first");
+        return VH_ESTATE;
+    }
+    GatherData *G = c->gather;
+    std::vector<int64_t> base(G->nfaces + 1, 0);
+    std::vector<int32_t> face_of(G->nsamples > 0 ? G->nsamples : 1);
+    for (int32_t f = 0; f < G->nfaces; f++) {
+        int32_t n = G->h_face_ptr[f + 1] - G->h_face_ptr[f];
+        base[f + 1] = base[f] + 4 * (int64_t)G->h_normals[f] * n;
+        for (int32_t s = G->h_face_ptr[f]; s < G->h_face_ptr[f + 1]; s++) face_of[s] = f;
+    }
+    int64_t bytes = base[G->nfaces];
+    if (capacity < bytes) {
+        set_error(std::to_string(bytes) + " lump bytes need as much room, and " +
+                  std::to_string(capacity) + " were given");
+        return VH_EINVAL;
+    }
+    if (bytes == 0) return VH_OK;
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf fo, fb, o;
+    int rc;
+    if ((rc = upload(fo, face_of.data(), (size_t)G->nsamples, "sample faces", st)) ||
+        (rc = upload(fb, base.data(), base.size(), "face offsets", st)) ||
+        (rc = o.ensure((size_t)bytes, "lump"))) {
+        fo.release();
+        fb.release();
+        o.release();
+        return rc;
+    }
+    int64_t n = (int64_t)G->nsamples * 4;
+    hipLaunchKernelGGL(k_encode_lump, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                       (const float4 *)(which ? G->out_hdr.ptr : G->out_ldr.ptr), G->nsamples,
+                       (const int32_t *)fo.ptr, (const int32_t *)G->face_ptr.ptr,
+                       (const int32_t *)G->face_nc.ptr, (const int64_t *)fb.ptr,
+                       (uint32_t *)o.ptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    fo.release();
+    fb.release();
+    o.release();
+    if (e != hipSuccess) {
+        set_error(std::string("lump encode failed: ") + hipGetErrorString(e));
+        return VH_EDEVICE;
+    }
+    return VH_OK;
+}
+
+}  // extern "C"

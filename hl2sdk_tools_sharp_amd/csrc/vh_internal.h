@@ -88,6 +88,8 @@ struct vh_ctx {
     vh::DevBuf counters;
     // patches, transfers and bounce state (vh_bounce.hip)
     struct BounceData *bounce = nullptr;
+    // faces, samples and lights of the direct-light gather (vh_gather.hip)
+    struct GatherData *gather = nullptr;
 };
 
 namespace vh {
@@ -114,6 +116,8 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
                         int32_t nlights, float tmin, uint64_t *d_bits, hipStream_t s);
 // Frees the patch/transfer/bounce buffers (vh_bounce.hip).
 void bounce_release(vh_ctx *c);
+// Frees the gather's buffers (vh_gather.hip).
+void gather_release(vh_ctx *c);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

@@ -397,3 +397,92 @@ def direct_light(g: RoomGrid, blocked: np.ndarray) -> np.ndarray:
 
 
 BOUNCE_ROOM = (1344.0, 1344.0, 384.0)
+
+
+def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
+    """The grid's faces as lightmaps: one sample at the centre of every
+    ``luxel`` x ``luxel`` cell of each face, the face's cluster for each
+    sample; every ``bumped_every``-th face (0 = none) carries the three bump
+    normals as well."""
+    from .light import FaceSet, bump_basis
+
+    ptr = [0]
+    pos = []
+    cl = []
+    nc = np.ones(g.nface, np.int32)
+    normals = np.zeros((g.nface, 4, 3), np.float32)
+    for f in range(g.nface):
+        lo, hi = g.face_lo[f].astype(np.float64), g.face_hi[f].astype(np.float64)
+        n = g.face_normal[f]
+        ax = int(np.argmax(np.abs(n)))
+        u, v = (ax + 1) % 3, (ax + 2) % 3
+        nu = max(1, int(round((hi[u] - lo[u]) / luxel)))
+        nv = max(1, int(round((hi[v] - lo[v]) / luxel)))
+        a = lo[u] + (np.arange(nu) + 0.5) * (hi[u] - lo[u]) / nu
+        b = lo[v] + (np.arange(nv) + 0.5) * (hi[v] - lo[v]) / nv
+        A, B = np.meshgrid(a, b, indexing="ij")
+        p = np.zeros((nu * nv, 3))
+        p[:, ax] = lo[ax]
+        p[:, u] = A.ravel()
+        p[:, v] = B.ravel()
+        pos.append(p)
+        cl.append(np.full(nu * nv, g.face_cluster[f], np.int32))
+        ptr.append(ptr[-1] + nu * nv)
+        normals[f, 0] = n
+        if bumped_every and f % bumped_every == 0:
+            nc[f] = 4
+            normals[f, 1:] = bump_basis(n)
+    return FaceSet(np.array(ptr, np.int32), nc, normals, np.concatenate(pos).astype(np.float32),
+                   np.concatenate(cl))
+
+
+def room_lights(g: RoomGrid, sun: bool = True, sky_ambient: bool = False, spot_every: int = 0,
+                surface_every: int = 0):
+    """The grid's point lights (quadratic falloff 1/d^2, white, intensity from
+    the generator, HDR = 2x LDR), each visible from its room's PVS row; plus a
+    sun along -to_sun and optionally a sky-ambient light.  Returns (lights,
+    light_pvs rows)."""
+    from .light import EMIT_POINT, EMIT_SKYAMBIENT, EMIT_SKYLIGHT, EMIT_SPOTLIGHT, EMIT_SURFACE, make_lights
+
+    nl = len(g.lights) + (1 if sun else 0) + (1 if sky_ambient else 0)
+    L = make_lights(nl)
+    C = g.ncluster
+    pvs_bytes = (C + 7) // 8
+    pvs = np.zeros((nl, pvs_bytes), np.uint8)
+    for k in range(len(g.lights)):
+        p = g.lights[k]
+        inside = np.all((p >= g.room_lo) & (p <= g.room_hi), axis=1)
+        room = int(np.argmax(inside))
+        L[k]["type"] = EMIT_POINT
+        L[k]["origin"] = p
+        L[k]["intensity"] = g.light_intensity[k]
+        L[k]["intensity_hdr"] = 2.0 * g.light_intensity[k]
+        L[k]["quadratic_attn"] = 1.0
+        if spot_every and k % spot_every == 1:
+            L[k]["type"] = EMIT_SPOTLIGHT
+            L[k]["normal"] = (0.0, 0.0, -1.0)
+            L[k]["stopdot"] = np.float32(np.cos(np.radians(30.0)))
+            L[k]["stopdot2"] = np.float32(np.cos(np.radians(45.0)))
+            L[k]["exponent"] = 2.0
+            L[k]["start_fade"], L[k]["end_fade"] = 100.0, 900.0
+        elif surface_every and k % surface_every == 2:
+            L[k]["type"] = EMIT_SURFACE
+            L[k]["normal"] = (0.0, 0.0, -1.0)
+            L[k]["quadratic_attn"] = 0.0
+        faces = g.row_faces[g.row_faces_ptr[room]:g.row_faces_ptr[room + 1]]
+        for c in np.unique(g.face_cluster[faces]):
+            pvs[k, c >> 3] |= np.uint8(1 << (c & 7))
+    k = len(g.lights)
+    if sun:
+        L[k]["type"] = EMIT_SKYLIGHT
+        L[k]["normal"] = -g.to_sun
+        L[k]["intensity"] = (255.0, 240.0, 200.0)
+        L[k]["intensity_hdr"] = (510.0, 480.0, 400.0)
+        pvs[k] = 0xFF
+        k += 1
+    if sky_ambient:
+        L[k]["type"] = EMIT_SKYAMBIENT
+        L[k]["intensity"] = (40.0, 60.0, 90.0)
+        L[k]["intensity_hdr"] = (80.0, 120.0, 180.0)
+        pvs[k] = 0xFF
+    return L, pvs

@@ -120,3 +120,38 @@ def point_light_rays(points: np.ndarray, lights: np.ndarray) -> np.ndarray:
     out[..., 3:6] = d
     out[..., 6] = 1.0
     return out.reshape(-1, 7)
+
+
+def batching_workload(nfaces: int = 4096, min_luxels: int = 4, max_luxels: int = 16384,
+                      nboxes: int = 2000, seed: int = 4):
+    """The batching config: ``nfaces`` box faces of the visibility config's
+    scene with luxel counts log-uniform over [min_luxels, max_luxels] (sample
+    positions uniform on the face), lit by a sun and the sky.  Returns
+    (casters, FaceSet)."""
+    from .light import FaceSet
+
+    rng = np.random.default_rng(seed)
+    lo, hi = random_boxes(nboxes, np.random.default_rng(2))  # the visibility scene's boxes
+    v9, ids = box_triangles(lo, hi)
+    pick = rng.choice(nboxes * 6, size=nfaces, replace=False)
+    counts = np.exp(rng.uniform(np.log(min_luxels), np.log(max_luxels), size=nfaces)).astype(np.int64)
+    ptr = np.zeros(nfaces + 1, np.int64)
+    np.cumsum(counts, out=ptr[1:])
+    S = int(ptr[-1])
+    pos = np.empty((S, 3), np.float32)
+    normals = np.zeros((nfaces, 4, 3), np.float32)
+    for k, q in enumerate(pick):
+        box, face = q // 6, q % 6
+        ax, side = _FACES[face]
+        u, v = (ax + 1) % 3, (ax + 2) % 3
+        n = int(counts[k])
+        s = rng.uniform(0.0, 1.0, size=(n, 2))
+        p = np.empty((n, 3))
+        p[:, u] = lo[box, u] + s[:, 0] * (hi[box, u] - lo[box, u])
+        p[:, v] = lo[box, v] + s[:, 1] * (hi[box, v] - lo[box, v])
+        p[:, ax] = hi[box, ax] if side else lo[box, ax]
+        pos[ptr[k]:ptr[k + 1]] = p
+        normals[k, 0, ax] = 1.0 if side else -1.0
+    faces = FaceSet(ptr.astype(np.int32), np.ones(nfaces, np.int32), normals, pos,
+                    np.zeros(S, np.int32))
+    return ShadowCasterSet(v9, ids), faces

@@ -218,6 +218,90 @@ typedef struct vh_bounce_opts {
 int vh_bounce(vh_ctx *ctx, const vh_bounce_opts *opts, const float *emit0, const float *start,
               float *total_out, double *residuals, double *energies, int32_t *iters);
 
+/* ---- The direct-light gather: RadWorld.LightFacesAsync (SourceSharp.MapTools/Rad/Light/RadWorld.Facelights.cs:194) ---- */
+
+/* EmitType (SourceSharp.MapFormats/Bsp/Structs/BspStructs.cs:907-925). */
+#define VH_EMIT_SURFACE 0
+#define VH_EMIT_POINT 1
+#define VH_EMIT_SPOTLIGHT 2
+#define VH_EMIT_SKYLIGHT 3
+#define VH_EMIT_SKYAMBIENT 5
+
+/* A DirectLight (SourceSharp.MapTools/Rad/Light/DirectLight.cs) as the gather
+ * reads it; intensity is the LDR pass's, intensity_hdr the HDR pass's (the
+ * two passes differ only there, DirectLightBuilder.cs:218-231).  Only style 0
+ * is supported here. */
+typedef struct vh_light {
+    int32_t type, style;
+    float origin[3], normal[3], intensity[3], intensity_hdr[3];
+    float constant_attn, linear_attn, quadratic_attn, cap_dist;
+    float start_fade, end_fade, stopdot, stopdot2, exponent;
+} vh_light;
+
+/* The faces to light.  Samples of face f are [face_sample_ptr[f],
+ * face_sample_ptr[f+1]); each sample's point is its position plus the face
+ * normal (FaceLightJob.ComputeIlluminationPointAndNormals,
+ * SourceSharp.MapTools/Rad/Light/FaceLightJob.cs:237-327, flat faces);
+ * face_normals holds the face normal and, for a bumped face
+ * (face_normal_count 4), its three bump basis normals.  light_pvs holds one
+ * row of cluster bits per light (pvs_bytes each; 0 = every light reaches
+ * every cluster).  Sun: sun_samples rays per sample (SunAreaLightSamples = 30
+ * in the reference), jittered by sun_jitter (3 x (sun_samples - 1)) scaled by
+ * the angular extent.  Sky ambient: sky_dirs_count directions (162 x
+ * SkySampleScale, from DirectionalSampler).  Ambient occlusion (no reference
+ * counterpart): ao_count directions per face (ao_dirs, 3 x ao_count per face),
+ * any hit within ao_radius. */
+typedef struct vh_gather_desc {
+    int32_t nfaces, nsamples, nlights, pvs_bytes;
+    const int32_t *face_sample_ptr;
+    const int32_t *face_normal_count;
+    const float *face_normals;
+    const float *sample_pos;
+    const int32_t *sample_cluster;
+    const vh_light *lights;
+    const uint8_t *light_pvs;
+    float sun_extent;
+    int32_t sun_samples;
+    const float *sun_jitter;
+    int32_t sky_dirs_count;
+    const float *sky_dirs;
+    int32_t ao_count;
+    float ao_radius;
+    const float *ao_dirs;
+} vh_gather_desc;
+
+/* Uploads the faces, samples, lights and tables (kept on the device). */
+int vh_gather_load(vh_ctx *ctx, const vh_gather_desc *desc);
+
+#define VH_SCHEDULE_QUEUE 0    /* one persistent launch; waves take 64-sample chunks from a queue */
+#define VH_SCHEDULE_PER_FACE 1 /* one launch per face (the unbatched form) */
+#define VH_SCHEDULE_GRID 2     /* one launch, one wave per chunk */
+
+/* Lights every sample against the loaded scene: FaceLightJob.EmitGroup /
+ * ResolveGroup over 4-sample groups (FaceLightJob.cs:329-463) with
+ * DirectLightGatherer's standard, sky and sky-ambient gathers
+ * (DirectLightGatherer.cs:441-1306) and LightingValue.AddLight, visibility
+ * traced inline.  Results stay on the device; vh_gather_read copies them.
+ * waves_per_cu sizes the persistent launch (0 = a default). */
+int vh_gather_run(vh_ctx *ctx, int32_t schedule, int32_t waves_per_cu);
+
+/* Copies the last run's results: ldr/hdr 4 floats (RGB, direct sun amount)
+ * per (normal, sample), normal-major over 4 normal slots; ao one float per
+ * sample.  Any pointer may be NULL. */
+int vh_gather_read(vh_ctx *ctx, float *ldr, float *hdr, float *ao);
+
+/* StockLightColor.Encode (SourceSharp.MapTools/Rad/Ambient/StockLightColor.cs:136-185)
+ * into ColorRGBExp32 (4 bytes: R, G, B, signed exponent) for n colours read
+ * `stride` floats apart (RGB first). */
+int vh_encode_rgbe(vh_ctx *ctx, const float *rgb, int64_t n, int32_t stride, uint8_t *out);
+
+/* The lighting lump of the last gather: for each face f (in order), style 0,
+ * each normal b, each sample j, the RGBE of ldr (which = 0) or hdr (1), at
+ * byte offset face_base(f) + (b * numSamples(f) + j) * 4, face_base the
+ * running sum of 4 * normals * samples (FinalLightFace.cs:270 with one style).
+ * out needs 4 * sum(normals * samples) bytes. */
+int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacity);
+
 /* Kernel timing with HIP events recorded on the launching stream around each
  * launch of the named kernel (no counterpart in the reference: the bench and
  * the profiler read it).  Enable, run, then read: vh_timing_read waits for the
@@ -228,6 +312,7 @@ int vh_bounce(vh_ctx *ctx, const vh_bounce_opts *opts, const float *emit0, const
 #define VH_KERNEL_CLOSEST 2
 #define VH_KERNEL_GATHER 3
 #define VH_KERNEL_TRANSFERS 4
+#define VH_KERNEL_FACELIGHT 5
 #define VH_KERNEL_COUNT 8
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);

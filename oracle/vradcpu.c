@@ -510,6 +510,23 @@ static int32_t bvh_trace_one(const vc_bvh *b, const float o[3], const float d[3]
     return best_orig;
 }
 
+/* Single-ray helpers for the gather oracle. */
+int vc_bvh_blocked(const vc_bvh *b, const float o[3], const float d[3], float reach, float tmin)
+{
+    float t;
+    return bvh_trace_one(b, o, d, reach, tmin, -1, 1, &t, NULL) >= 0;
+}
+
+/* LightRayLog.Occlusion (Rad/Light/LightRayLog.cs:655-659): 1 when the first
+ * hit short of the end is not a sky triangle. */
+int vc_bvh_sky_occluded(const vc_bvh *b, const float o[3], const float d[3], float reach)
+{
+    float t;
+    int32_t id = 0;
+    int32_t k = bvh_trace_one(b, o, d, reach, 0.0f, -1, 2, &t, &id);
+    return k >= 0 && !(id & VC_SKY);
+}
+
 typedef struct {
     const vc_bvh *b;
     const float *rays;

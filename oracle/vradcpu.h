@@ -113,6 +113,48 @@ void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const floa
                int32_t max_iter, double tol, int32_t use_double, double *total_out,
                double *residuals, double *energies, int32_t *iters);
 
+/* ---- direct-light gather, sampler, encode (vradcpu_gather.c) ---- */
+
+/* EmitType (MapFormats/Bsp/Structs/BspStructs.cs:907-925). */
+#define VC_EMIT_SURFACE 0
+#define VC_EMIT_POINT 1
+#define VC_EMIT_SPOTLIGHT 2
+#define VC_EMIT_SKYLIGHT 3
+#define VC_EMIT_SKYAMBIENT 5
+
+/* A DirectLight (Rad/Light/DirectLight.cs) as the gather reads it. */
+typedef struct vc_light {
+    int32_t type, style;
+    float origin[3], normal[3], intensity[3], intensity_hdr[3];
+    float constant_attn, linear_attn, quadratic_attn, cap_dist;
+    float start_fade, end_fade, stopdot, stopdot2, exponent;
+} vc_light;
+
+typedef struct vc_gather_desc {
+    int32_t nfaces, nsamples, nlights, pvs_bytes;
+    const int32_t *face_sample_ptr;   /* nfaces + 1 */
+    const int32_t *face_normal_count; /* 1, or 4 for a bumped face */
+    const float *face_normals;        /* 12 per face: the face normal, then 3 bump normals */
+    const float *sample_pos;          /* 3 per sample (on the face; +normal is the point) */
+    const int32_t *sample_cluster;
+    const vc_light *lights;
+    const uint8_t *light_pvs; /* nlights x pvs_bytes cluster bits */
+    float sun_extent;
+    int32_t sun_samples;
+    const float *sun_jitter; /* 3 x (sun_samples - 1) */
+    int32_t sky_dirs_count;
+    const float *sky_dirs;
+    int32_t ao_count;
+    float ao_radius;
+    const float *ao_dirs; /* 3 x ao_count per face */
+} vc_gather_desc;
+
+void vc_directional_sampler(int32_t n, float *out);
+float vc_halton(int32_t base, int32_t index);
+void vc_encode_rgbe(const float *rgb, int64_t n, int32_t stride, uint8_t *out);
+void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+               float *out_ao, int32_t threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,467 @@
+/*
+ * vradcpu -- CPU oracle for the vrad lighting hot path: the direct-light
+ * gather, the directional sampler and the RGBE encode.
+ *
+ * TEST INFRASTRUCTURE ONLY (see vradcpu.h).  Built with -ffp-contract=off.
+ * Restates (paths under the reference's src/SourceSharp.MapTools/):
+ *   Rad/Light/DirectLightGatherer.cs -- GatherStandardLane (:1169-1306),
+ *       EmitSkyLight/ResolveSkyLight (:716-845), AmbientSky (:847-1012),
+ *       Resolve's bump rule (:441-479), InverseQuadratic (:1579-1587)
+ *   Rad/Light/FaceLightJob.cs  -- EmitGroup / ResolveGroup: 4-sample groups,
+ *       the per-lane PVS mask, (dot * mask) * falloff, AddLight per normal
+ *   Rad/Light/LightingValue.cs -- AddLight
+ *   Rad/Light/DirectionalSampler.cs -- HaltonSequence, DirectionalSampler
+ *   Rad/Ambient/StockLightColor.cs:136-185 -- Encode, CalcExponent
+ * plus the build's own ambient-occlusion term (no reference counterpart).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vradcpu.h"
+
+#define LANES 4
+#define MAX_TRACE_LENGTH ((float)(1.732050807569 * (2 * 16384))) /* LightConstants.cs */
+#define DIST_EPSILON 0.03125f
+#define EQUAL_EPSILON 0.001f
+#define CONSTANT_DOT (0.7f / 2.0f)
+
+int vc_bvh_blocked(const vc_bvh *b, const float o[3], const float d[3], float reach, float tmin);
+int vc_bvh_sky_occluded(const vc_bvh *b, const float o[3], const float d[3], float reach);
+
+static inline float maxps(float a, float b) { return a > b ? a : b; }
+static inline float minps(float a, float b) { return a < b ? a : b; }
+static inline float dot3(const float *a, const float *b)
+{
+    return (a[0] * b[0]) + (a[1] * b[1]) + (a[2] * b[2]);
+}
+
+/* ---- DirectionalSampler (Rad/Light/DirectionalSampler.cs) ---- */
+
+static float halton_next(int base, int *seed)
+{
+    (*seed)++;
+    int tmpseed = *seed;
+    float ret = 0.0f;
+    float fbase = (float)base;
+    float base_inv = (float)(1.0 / fbase);
+    while (tmpseed != 0) {
+        int dig = tmpseed % base;
+        ret += (float)dig * base_inv;
+        base_inv /= fbase;
+        tmpseed /= base;
+    }
+    return ret;
+}
+
+/* The first n directions of DirectionalSampler.  DetMathF's correctly rounded
+ * acos/sin/cos are evaluated in double and rounded once. */
+void vc_directional_sampler(int32_t n, float *out)
+{
+    int s2 = 1, s3 = 1;
+    for (int32_t i = 0; i < n; i++) {
+        float z = halton_next(2, &s2);
+        z = (float)((2 * z) - 1.0);
+        float phi = (float)acos((double)z);
+        float theta = (float)(2.0 * M_PI * halton_next(3, &s3));
+        float sinp = (float)sin((double)phi);
+        out[3 * i] = (float)cos((double)theta) * sinp;
+        out[3 * i + 1] = (float)sin((double)theta) * sinp;
+        out[3 * i + 2] = z;
+    }
+}
+
+float vc_halton(int32_t base, int32_t index)
+{
+    int seed = index;
+    return halton_next(base, &seed);
+}
+
+/* ---- StockLightColor.Encode (Rad/Ambient/StockLightColor.cs:136-185) ---- */
+
+static inline int calc_exponent(float f)
+{
+    if (f == 0.0f) return 0;
+    uint32_t bits;
+    memcpy(&bits, &f, 4);
+    int exp_component = (int)((bits & 0x7F800000u) >> 23);
+    return exp_component - (7 + 127);
+}
+
+/* .NET's float -> int conversion saturates and maps NaN to 0. */
+static inline int32_t to_int_sat(float x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)x;
+}
+
+void vc_encode_rgbe(const float *rgb, int64_t n, int32_t stride, uint8_t *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const float *v = rgb + i * stride;
+        float mx;
+        if (v[0] > v[1])
+            mx = v[0] > v[2] ? v[0] : v[2];
+        else
+            mx = v[1] > v[2] ? v[1] : v[2];
+        int e = calc_exponent(mx);
+        uint32_t fbits = (uint32_t)(127 - e) << 23;
+        float scalar;
+        memcpy(&scalar, &fbits, 4);
+        out[4 * i] = (uint8_t)to_int_sat(v[0] * scalar);
+        out[4 * i + 1] = (uint8_t)to_int_sat(v[1] * scalar);
+        out[4 * i + 2] = (uint8_t)to_int_sat(v[2] * scalar);
+        out[4 * i + 3] = (uint8_t)(int8_t)e;
+    }
+}
+
+/* ---- the gather ---- */
+
+typedef struct {
+    const vc_gather_desc *d;
+    const vc_bvh *bvh;
+    float *out_ldr, *out_hdr, *out_ao;
+    int64_t ngroups;
+    const int32_t *group_face, *group_first;
+    int64_t next;
+    pthread_mutex_t mu;
+} gjob;
+
+static inline int pvs_check(const vc_gather_desc *d, int32_t light, int32_t cluster)
+{
+    if (cluster < 0 || d->pvs_bytes == 0) return 1;
+    if (cluster >= d->pvs_bytes * 8) return 1;
+    return (d->light_pvs[(size_t)light * d->pvs_bytes + (cluster >> 3)] >> (cluster & 7)) & 1;
+}
+
+/* InverseQuadratic: ((d*d)*q + l*d) + c, then its reciprocal. */
+static inline float inverse_quadratic(const vc_light *l, float dd)
+{
+    float f = dd * dd;
+    f *= l->quadratic_attn;
+    f += l->linear_attn * dd;
+    f += l->constant_attn;
+    return 1.0f / f;
+}
+
+/* GatherStandardLane: returns 1 when the lane needs a ray (pos -> src). */
+static int standard_lane(const vc_light *l, const float *pos, const float *nrm, int normals,
+                         int needed, float *dot_out /* [normals] */, float *falloff_out,
+                         float *src_out)
+{
+    float src[3] = {l->origin[0], l->origin[1], l->origin[2]};
+    float delta[3] = {src[0] - pos[0], src[1] - pos[1], src[2] - pos[2]};
+    float dist2 = (delta[0] * delta[0]) + (delta[1] * delta[1]) + (delta[2] * delta[2]);
+    float rpc = 1.0f / sqrtf(dist2);
+    delta[0] = delta[0] * rpc;
+    delta[1] = delta[1] * rpc;
+    delta[2] = delta[2] * rpc;
+    float dist = sqrtf(dist2);
+    float dot = dot3(delta, nrm);
+    dot = maxps(0.0f, dot);
+    int hard = l->end_fade > l->start_fade;
+    if (hard && !(dist <= l->end_fade)) return 0;
+    dist = maxps(dist, 1.0f);
+    float eval = minps(dist, l->cap_dist);
+    float falloff;
+    if (l->type == VC_EMIT_POINT) {
+        falloff = inverse_quadratic(l, eval);
+    } else if (l->type == VC_EMIT_SURFACE) {
+        float dot2 = -dot3(delta, l->normal);
+        dot2 = maxps(0.0f, dot2);
+        if (dot == 0.0f) return 0;
+        falloff = dot2 / dist2;
+        src[0] = src[0] + (l->normal[0] * DIST_EPSILON);
+        src[1] = src[1] + (l->normal[1] * DIST_EPSILON);
+        src[2] = src[2] + (l->normal[2] * DIST_EPSILON);
+    } else if (l->type == VC_EMIT_SPOTLIGHT) {
+        float dot2 = -dot3(delta, l->normal);
+        if (!(dot2 > l->stopdot2)) return 0;
+        falloff = inverse_quadratic(l, eval) * dot2;
+        int in_fringe = dot2 <= l->stopdot;
+        float mult = (dot2 - l->stopdot2) / (l->stopdot - l->stopdot2);
+        mult = minps(mult, 1.0f);
+        mult = maxps(mult, 0.0f);
+        if (l->exponent != 0.0f && l->exponent != 1.0f)
+            mult = (float)pow((double)mult, (double)l->exponent);
+        mult = in_fringe ? mult : 1.0f;
+        falloff = mult * falloff;
+    } else {
+        return 0;
+    }
+    if (hard) {
+        float t = (dist - l->start_fade) / (l->end_fade - l->start_fade);
+        t = minps(t, 1.0f);
+        t = maxps(t, 0.0f);
+        t = 1.0f - t;
+        float mult = (6.0f * t) - 15.0f;
+        mult = (mult * t) + 10.0f;
+        mult = (t * t) * mult;
+        mult = t * mult;
+        falloff = mult * falloff;
+    }
+    int trace = needed && dot != 0.0f && falloff != 0.0f;
+    dot_out[0] = dot;
+    *falloff_out = falloff;
+    for (int n = 1; n < normals; n++) dot_out[n] = maxps(0.0f, dot3(nrm + 3 * n, delta));
+    memcpy(src_out, src, 12);
+    return trace;
+}
+
+static void gather_group(gjob *j, int64_t g)
+{
+    const vc_gather_desc *d = j->d;
+    int32_t f = j->group_face[g];
+    int32_t first = j->group_first[g];
+    int32_t fend = d->face_sample_ptr[f + 1];
+    int count = fend - first < LANES ? fend - first : LANES;
+    int normals = d->face_normal_count[f];
+    const float *fn = d->face_normals + (size_t)f * 12;
+    float p[LANES][3];
+    int32_t cl[LANES];
+    for (int i = 0; i < LANES; i++) {
+        int32_t s = first + (i < count ? i : count - 1);
+        const float *ps = d->sample_pos + 3 * (size_t)s;
+        p[i][0] = ps[0] + fn[0];
+        p[i][1] = ps[1] + fn[1];
+        p[i][2] = ps[2] + fn[2];
+        cl[i] = d->sample_cluster[s];
+    }
+    for (int32_t li = 0; li < d->nlights; li++) {
+        const vc_light *l = &d->lights[li];
+        int mask = 0;
+        for (int s = 0; s < LANES; s++)
+            if (s < count && pvs_check(d, li, cl[s])) mask |= 1 << s;
+        if (mask == 0) continue;
+        float dot[4][LANES];
+        float falloff[LANES], sun[LANES];
+        memset(dot, 0, sizeof dot);
+        memset(falloff, 0, sizeof falloff);
+        memset(sun, 0, sizeof sun);
+        if (l->type == VC_EMIT_POINT || l->type == VC_EMIT_SURFACE ||
+            l->type == VC_EMIT_SPOTLIGHT) {
+            for (int lane = 0; lane < count; lane++) {
+                float dl[4] = {0, 0, 0, 0}, fo = 0, src[3];
+                int trace = standard_lane(l, p[lane], fn, normals, (mask >> lane) & 1, dl, &fo, src);
+                for (int n = 0; n < normals; n++) dot[n][lane] = dl[n];
+                falloff[lane] = fo;
+                if (trace) {
+                    /* LightRayLog.MakeRay: the whole segment, reach 1 */
+                    float dir[3] = {src[0] - p[lane][0], src[1] - p[lane][1], src[2] - p[lane][2]};
+                    float vis = vc_bvh_blocked(j->bvh, p[lane], dir, 1.0f, 0.0f) ? 0.0f : 1.0f;
+                    dot[0][lane] = vis * dot[0][lane];
+                }
+            }
+            for (int lane = count; lane < LANES; lane++) {
+                for (int n = 0; n < normals; n++) dot[n][lane] = dot[n][count - 1];
+                falloff[lane] = falloff[count - 1];
+            }
+        } else if (l->type == VC_EMIT_SKYLIGHT) {
+            float dl[LANES];
+            int all_zero = 1;
+            for (int lane = 0; lane < LANES; lane++) {
+                float dd = -dot3(fn, l->normal);
+                dd = maxps(dd, 0.0f);
+                dl[lane] = dd;
+                all_zero &= dd == 0.0f;
+            }
+            if (!all_zero) {
+                int nsamples = d->sun_samples;
+                float jitter_scale = (float)(1.732050807569 * (2 * 16384) * (double)d->sun_extent);
+                float total[LANES] = {0, 0, 0, 0};
+                for (int s = 0; s < nsamples; s++) {
+                    float delta[3] = {l->normal[0] * -MAX_TRACE_LENGTH, l->normal[1] * -MAX_TRACE_LENGTH,
+                                      l->normal[2] * -MAX_TRACE_LENGTH};
+                    if (s != 0) {
+                        const float *jt = d->sun_jitter + 3 * (size_t)(s - 1);
+                        delta[0] = delta[0] + (jt[0] * jitter_scale);
+                        delta[1] = delta[1] + (jt[1] * jitter_scale);
+                        delta[2] = delta[2] + (jt[2] * jitter_scale);
+                    }
+                    float occ[LANES];
+                    for (int lane = 0; lane < count; lane++) {
+                        float stop[3] = {delta[0] + p[lane][0], delta[1] + p[lane][1], delta[2] + p[lane][2]};
+                        float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
+                        occ[lane] = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
+                    }
+                    for (int lane = count; lane < LANES; lane++) occ[lane] = occ[count - 1];
+                    for (int lane = 0; lane < LANES; lane++) {
+                        float o = minps(maxps(occ[lane], 0.0f), 1.0f);
+                        total[lane] += 1.0f - o;
+                    }
+                }
+                float inv = 1.0f / (float)nsamples;
+                for (int lane = 0; lane < LANES; lane++) {
+                    float see = total[lane] * inv;
+                    dot[0][lane] = dl[lane] * see;
+                    falloff[lane] = 1.0f;
+                    sun[lane] = see * 10000.0f;
+                    for (int n = 1; n < normals; n++) dot[n][lane] = (-dot3(fn + 3 * n, l->normal)) * see;
+                }
+            }
+        } else if (l->type == VC_EMIT_SKYAMBIENT) {
+            float sumdot[LANES] = {0, 0, 0, 0};
+            float ambient[4][LANES], possible[4][LANES], dots[4][LANES];
+            memset(ambient, 0, sizeof ambient);
+            memset(possible, 0, sizeof possible);
+            for (int32_t a = 0; a < d->sky_dirs_count; a++) {
+                const float *an = d->sky_dirs + 3 * (size_t)a;
+                int valid[LANES], any = 0, tr[LANES];
+                for (int lane = 0; lane < LANES; lane++) {
+                    float dd = -dot3(fn, an);
+                    valid[lane] = dd > EQUAL_EPSILON;
+                    any |= valid[lane];
+                    dots[0][lane] = valid[lane] ? dd : 0.0f;
+                }
+                if (!any) continue;
+                for (int lane = 0; lane < LANES; lane++) {
+                    sumdot[lane] = dots[0][lane] + sumdot[lane];
+                    possible[0][lane] = (valid[lane] ? 1.0f : 0.0f) + possible[0][lane];
+                    tr[lane] = valid[lane];
+                    for (int n = 1; n < normals; n++) {
+                        float dd = -dot3(fn + 3 * n, an);
+                        int v2 = dd > EQUAL_EPSILON;
+                        dots[n][lane] = v2 ? dd : 0.0f;
+                        possible[n][lane] = (valid[lane] && v2 ? 1.0f : 0.0f) + possible[n][lane];
+                        tr[lane] |= v2;
+                    }
+                }
+                float frac[LANES];
+                for (int lane = 0; lane < LANES; lane++) {
+                    if (lane >= count) {
+                        frac[lane] = frac[count - 1];
+                        continue;
+                    }
+                    int need = tr[lane] && ((mask >> lane) & 1);
+                    if (need) {
+                        float stop[3] = {(an[0] * -MAX_TRACE_LENGTH) + p[lane][0],
+                                         (an[1] * -MAX_TRACE_LENGTH) + p[lane][1],
+                                         (an[2] * -MAX_TRACE_LENGTH) + p[lane][2]};
+                        float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
+                        frac[lane] = 1.0f - (vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f);
+                    } else {
+                        frac[lane] = 1.0f;
+                    }
+                }
+                for (int n = 0; n < normals; n++)
+                    for (int lane = 0; lane < LANES; lane++)
+                        ambient[n][lane] = ambient[n][lane] + (frac[lane] * dots[n][lane]);
+            }
+            for (int lane = 0; lane < LANES; lane++) {
+                falloff[lane] = 1.0f;
+                for (int n = 0; n < normals; n++) {
+                    float factor = possible[n][lane] / possible[0][lane];
+                    dot[n][lane] = ambient[n][lane] / (factor * sumdot[lane]);
+                }
+            }
+        } else {
+            continue;
+        }
+        /* Resolve: a light behind the face lights no bump direction */
+        for (int lane = 0; lane < LANES; lane++) {
+            float d0 = maxps(dot[0][lane], 0.0f);
+            dot[0][lane] = d0;
+            int nz = d0 > 0.0f;
+            for (int n = 1; n < normals; n++) {
+                float dn = maxps(dot[n][lane], 0.0f);
+                dot[n][lane] = nz ? dn : 0.0f;
+            }
+        }
+        /* ResolveGroup: (dot * mask) * falloff, skip when all zero */
+        float fx[4][LANES];
+        int skip = 1;
+        for (int n = 0; n < normals; n++)
+            for (int lane = 0; lane < LANES; lane++) {
+                float masked = ((mask >> lane) & 1) ? dot[n][lane] : 0.0f;
+                fx[n][lane] = masked * falloff[lane];
+                skip &= fx[n][lane] == 0.0f;
+            }
+        if (skip) continue;
+        for (int n = 0; n < normals; n++)
+            for (int i = 0; i < count; i++) {
+                size_t o = ((size_t)n * d->nsamples + first + i) * 4;
+                float amt = fx[n][i];
+                for (int pass = 0; pass < 2; pass++) {
+                    float *out = pass ? j->out_hdr : j->out_ldr;
+                    const float *col = pass ? l->intensity_hdr : l->intensity;
+                    if (!out) continue;
+                    out[o] = out[o] + (amt * col[0]);
+                    out[o + 1] = out[o + 1] + (amt * col[1]);
+                    out[o + 2] = out[o + 2] + (amt * col[2]);
+                    out[o + 3] += sun[i];
+                }
+            }
+    }
+    /* ambient occlusion (the build's own term): cosine-weighted directions in
+     * the face's frame, any hit within ao_radius; ao = unblocked fraction. */
+    if (j->out_ao && d->ao_count > 0) {
+        for (int i = 0; i < count; i++) {
+            int unblocked = 0;
+            for (int32_t a = 0; a < d->ao_count; a++) {
+                const float *dir = d->ao_dirs + ((size_t)f * d->ao_count + a) * 3;
+                float seg[3] = {dir[0] * d->ao_radius, dir[1] * d->ao_radius, dir[2] * d->ao_radius};
+                if (!vc_bvh_blocked(j->bvh, p[i], seg, 1.0f, 0.0f)) unblocked++;
+            }
+            j->out_ao[first + i] = (float)unblocked / (float)d->ao_count;
+        }
+    }
+}
+
+static void *gather_worker(void *arg)
+{
+    gjob *j = (gjob *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        int64_t g = j->next;
+        j->next += 16;
+        pthread_mutex_unlock(&j->mu);
+        if (g >= j->ngroups) break;
+        for (int64_t k = g; k < g + 16 && k < j->ngroups; k++) gather_group(j, k);
+    }
+    return NULL;
+}
+
+/* The whole gather: every face's samples in groups of four.  out_ldr/out_hdr
+ * hold 4 floats (RGB, direct sun amount) per (normal, sample), normal-major,
+ * zeroed by the caller; out_ao one float per sample (may be NULL). */
+void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+               float *out_ao, int32_t threads)
+{
+    int64_t ng = 0;
+    for (int32_t f = 0; f < d->nfaces; f++) {
+        int32_t n = d->face_sample_ptr[f + 1] - d->face_sample_ptr[f];
+        ng += (n + LANES - 1) / LANES;
+    }
+    int32_t *gf = (int32_t *)malloc((size_t)(ng > 0 ? ng : 1) * 4);
+    int32_t *gs = (int32_t *)malloc((size_t)(ng > 0 ? ng : 1) * 4);
+    int64_t k = 0;
+    for (int32_t f = 0; f < d->nfaces; f++)
+        for (int32_t s = d->face_sample_ptr[f]; s < d->face_sample_ptr[f + 1]; s += LANES) {
+            gf[k] = f;
+            gs[k] = s;
+            k++;
+        }
+    gjob j;
+    memset(&j, 0, sizeof j);
+    j.d = d;
+    j.bvh = bvh;
+    j.out_ldr = out_ldr;
+    j.out_hdr = out_hdr;
+    j.out_ao = out_ao;
+    j.ngroups = ng;
+    j.group_face = gf;
+    j.group_first = gs;
+    pthread_mutex_init(&j.mu, NULL);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (int i = 1; i < threads; i++) pthread_create(&th[i], NULL, gather_worker, &j);
+    gather_worker(&j);
+    for (int i = 1; i < threads; i++) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&j.mu);
+    free(gf);
+    free(gs);
+}

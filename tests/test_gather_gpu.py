@@ -1,0 +1,145 @@
+"""The GPU direct-light gather, lightmap encode and AO against the oracle.
+Needs an MI355X.
+
+The bar: every sample's LDR and HDR lighting (per normal slot: RGB and the
+direct-sun amount) and AO equal to the oracle's, except where a spotlight's
+pow() rounds a last bit differently on the device (relative 1e-6); the RGBE
+bytes equal wherever the float inputs are equal; every schedule (persistent
+queue, grid, one launch per face) gives the same bits."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle.gather as og
+from hl2sdk_tools_sharp_amd import HipTracerFactory, light, rooms, scenes
+from hl2sdk_tools_sharp_amd.light import HipFaceLighter
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+
+
+def open_lighter(casters):
+    offer = HipTracerFactory().try_create(casters)
+    assert offer.tracer is not None, offer.decline_reason
+    return offer.tracer, HipFaceLighter(offer.tracer)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    """A 2x2 room map under a sky ceiling: point, spot and surface lights
+    behind PVS rows, a sun with a 30-sample area, sky ambient, every fifth
+    face bumped, 32-ray AO."""
+    g = rooms.room_grid(2, 2, sky_ceiling=True, seed=7)
+    faces = rooms.room_faces(g, luxel=32.0, bumped_every=5)
+    lights, pvs = rooms.room_lights(g, sun=True, sky_ambient=True, spot_every=3, surface_every=3)
+    kw = dict(light_pvs=pvs, sun_extent=0.02, sun_samples=30, sky_dirs=light.sky_directions(),
+              ao_count=32, ao_radius=40.0)
+    d = light.pack_desc(faces, lights, pvs, kw["sun_extent"], kw["sun_samples"], kw["sky_dirs"],
+                        kw["ao_count"], kw["ao_radius"])
+    want = og.gather(g.casters, d, faces.nsamples)
+    print(f"[c1] {faces.nfaces} faces, {faces.nsamples} samples, {len(lights)} lights", flush=True)
+    return g, faces, lights, kw, want
+
+
+def check(got, want):
+    ldr, hdr, ao = want
+    for name, a, b in (("ldr", got.ldr, ldr), ("hdr", got.hdr, hdr)):
+        same = np.mean(a.view(np.uint32) == b.view(np.uint32))
+        assert same > 0.999, (name, same)
+        np.testing.assert_allclose(a, b, rtol=RTOL, atol=0, err_msg=name)
+    assert np.array_equal(got.ao, ao)
+
+
+@pytest.mark.parametrize("schedule", ["queue", "grid", "per_face"])
+def test_gather_matches_oracle(c1, schedule):
+    g, faces, lights, kw, want = c1
+    print(f"[gather] schedule={schedule}", flush=True)
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        fl.load(faces, lights, **kw)
+        got = fl.light_faces(schedule=schedule)
+    check(got, want)
+    assert np.any(got.ldr[0, :, 3] > 0) and np.any(got.ldr[0, :, 3] == 0)  # sun both seen and shaded
+    assert np.any(got.ldr[1:, :, 0] > 0)  # bump slots lit
+    assert 0 < got.ao.min() < got.ao.max() <= 1
+
+
+def test_schedules_bitwise_equal_and_rerun_is_idempotent(c1):
+    g, faces, lights, kw, _ = c1
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        fl.load(faces, lights, **kw)
+        outs = [fl.light_faces(schedule=s, waves_per_cu=w)
+                for s, w in (("queue", 0), ("queue", 1), ("queue", 7), ("grid", 0), ("per_face", 0), ("queue", 0))]
+    for o in outs[1:]:
+        assert np.array_equal(o.ldr.view(np.uint32), outs[0].ldr.view(np.uint32))
+        assert np.array_equal(o.hdr.view(np.uint32), outs[0].hdr.view(np.uint32))
+        assert np.array_equal(o.ao, outs[0].ao)
+
+
+def test_lightmap_lump_matches_oracle_encode(c1):
+    g, faces, lights, kw, want = c1
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        fl.load(faces, lights, **kw)
+        got = fl.light_faces()
+        lump = fl.lightmap(hdr=False)
+        lump_hdr = fl.lightmap(hdr=True)
+    # the lump's order: face, normal slot, sample (BSP lighting lump layout)
+    for which, lm, src in ((0, lump, got.ldr), (1, lump_hdr, got.hdr)):
+        parts = []
+        for f in range(faces.nfaces):
+            a, b = faces.face_sample_ptr[f], faces.face_sample_ptr[f + 1]
+            for k in range(faces.face_normal_count[f]):
+                parts.append(src[k, a:b, :3])
+        exp = og.encode_rgbe(np.concatenate(parts)).reshape(-1)
+        assert np.array_equal(lm, exp), which
+
+
+def test_encode_rgbe_matches_oracle():
+    rng = np.random.default_rng(11)
+    v = (rng.uniform(0, 1, (100_003, 3)) * 10.0 ** rng.uniform(-8, 8, (100_003, 1))).astype(np.float32)
+    v[:50] = [[0, 0, 0], [1, 1, 1], [3, 3, 1], [-1, 2, 0], [np.nan, 1, 1], [np.inf, 0, 0],
+              [1e38, 1e38, 1e38], [1e-40, 0, 0], [255, 255, 255], [256, 0, 0]] * 5
+    g = rooms.room_grid(1, 1)
+    tr, _ = open_lighter(g.casters)
+    with tr:
+        got = light.encode_rgbe(tr, v)
+        wide = np.concatenate([v, np.ones((len(v), 1), np.float32)], axis=1)
+        got4 = light.encode_rgbe(tr, wide)
+    exp = og.encode_rgbe(v)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got4, exp)
+
+
+def test_gather_batching_scene_subset():
+    """Faces with ragged luxel counts (4 to 2,048) of the batching config's scene, sun
+    plus sky ambient, queue against per-face launches and the oracle."""
+    casters, faces = scenes.batching_workload(nfaces=96, min_luxels=4, max_luxels=2048)
+    L = light.make_lights(2)
+    L[0]["type"], L[0]["normal"], L[0]["intensity"] = light.EMIT_SKYLIGHT, (0.3, 0.2, -0.93), (1, 1, 1)
+    L[1]["type"], L[1]["intensity"] = light.EMIT_SKYAMBIENT, (0.2, 0.2, 0.3)
+    kw = dict(sun_extent=0.0, sun_samples=1, sky_dirs=light.sky_directions())
+    d = light.pack_desc(faces, L, None, 0.0, 1, kw["sky_dirs"])
+    want = og.gather(casters, d, faces.nsamples)
+    tr, fl = open_lighter(casters)
+    with tr:
+        fl.load(faces, L, **kw)
+        for s in ("queue", "per_face"):
+            check(fl.light_faces(schedule=s), want)
+
+
+def test_gather_errors():
+    g = rooms.room_grid(1, 1)
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        with pytest.raises(Exception):
+            fl.run()  # nothing loaded
+        faces = rooms.room_faces(g, luxel=64.0)
+        lights, pvs = rooms.room_lights(g)
+        bad = lights.copy()
+        bad[0]["type"] = 9
+        with pytest.raises(Exception):
+            fl.load(faces, bad, light_pvs=pvs)
