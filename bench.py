@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import re
 import sys
@@ -142,6 +143,92 @@ def bounce_leg(a, log):
     return out
 
 
+SUN_EXTENT_HALF_DEGREE = math.sin(math.radians(0.5))
+
+
+def batching_leg(a, log):
+    """The batching config: 4,096 faces of the visibility scene with luxel
+    counts log-uniform over 4-16,384; per luxel 162 sky-ambient rays, 64
+    jittered sun-disk rays (extent 0.5 degrees) and 32 AO rays (radius 40).
+    One persistent-queue launch (waves per CU from the autotuner) against one
+    launch per face; rays counted by the kernel.  The CPU baseline is the
+    oracle on a seeded face subsample (10 % by default), whose GPU result it
+    also checks bit for bit."""
+    import oracle.gather as og
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, autotune, light, scenes
+
+    t0 = time.perf_counter()
+    casters, faces = scenes.batching_workload(nfaces=a.batch_faces, seed=4)
+    rng = np.random.default_rng(4)
+    z = rng.uniform(0.0, 1.0)
+    phi = rng.uniform(0.0, 2 * math.pi)
+    to_sun = np.array([math.sqrt(1 - z * z) * math.cos(phi), math.sqrt(1 - z * z) * math.sin(phi), z])
+    L = light.make_lights(2)
+    L[0]["type"], L[0]["normal"] = light.EMIT_SKYLIGHT, -to_sun
+    L[0]["intensity"], L[0]["intensity_hdr"] = (1.0, 0.9, 0.8), (2.0, 1.8, 1.6)
+    L[1]["type"], L[1]["intensity"], L[1]["intensity_hdr"] = light.EMIT_SKYAMBIENT, (0.2, 0.25, 0.3), (0.4, 0.5, 0.6)
+    kw = dict(sun_extent=SUN_EXTENT_HALF_DEGREE, sun_samples=64, sky_dirs=light.sky_directions(),
+              ao_count=32, ao_radius=40.0)
+    nominal = faces.nsamples * (162 + 64 + 32)
+    log(f"[bench] batching: {faces.nfaces} faces, {faces.nsamples} luxels, {nominal} nominal rays "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    offer = HipTracerFactory(device=0).try_create(casters)
+    if offer.tracer is None:
+        raise RuntimeError(f"HIP tracer declined: {offer.decline_reason}")
+    tr = offer.tracer
+    fl = light.HipFaceLighter(tr)
+    fl.load(faces, L, **kw)
+    fl.run("queue")  # first-use allocations outside the timing
+    tune = autotune.tune_gather(fl, use_cache=False, log=log)
+    wpc = tune.waves_per_cu
+    q_ms, rays = [], 0
+    for _ in range(a.batch_steps):
+        ms, _n, r = fl.timed_run("queue", wpc)
+        q_ms.append(ms)
+        rays = r
+    q = float(np.median(q_ms))
+    ref = fl.read()
+    pf_ms, _n, pf_rays = fl.timed_run("per_face")
+    same = bool(np.array_equal(fl.read().ldr.view(np.uint32), ref.ldr.view(np.uint32)))
+    log(f"[bench] batching queue (waves/CU {wpc}): {q:.1f} ms, {rays / (q / 1e3) / 1e9:.2f} Grays/s; "
+        f"per-face launches {pf_ms:.1f} ms; bitwise equal: {same}")
+    del ref
+    out = {
+        "faces": faces.nfaces, "luxels": faces.nsamples, "rays_nominal": nominal, "rays_traced": rays,
+        "queue_ms": q, "queue_rays_per_s": rays / (q / 1e3), "waves_per_cu": wpc,
+        "tune_ms": {str(k): round(v, 3) for k, v in tune.ms.items()},
+        "per_face_ms": pf_ms, "per_face_rays_per_s": pf_rays / (pf_ms / 1e3),
+        "queue_speedup": pf_ms / q, "queue_equals_per_face": same,
+    }
+    # a seeded face subsample: the CPU baseline and a parity check at the config's scale
+    nsub = max(1, int(round(faces.nfaces * a.batch_cpu_fraction)))
+    sub = np.sort(np.random.default_rng(5).choice(faces.nfaces, nsub, replace=False))
+    fs = faces.subset(sub)
+    fl.load(fs, L, **kw)
+    _ms, _n, sub_rays = fl.timed_run("queue", wpc)
+    got = fl.read()
+    tr.dispose()
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        d = light.pack_desc(fs, L, None, kw["sun_extent"], kw["sun_samples"], kw["sky_dirs"],
+                            kw["ao_count"], kw["ao_radius"])
+        t0 = time.perf_counter()
+        ldr, hdr, ao = og.gather(casters, d, fs.nsamples, threads=threads)
+        dt = time.perf_counter() - t0
+        diff = int(np.count_nonzero(ldr.view(np.uint32) != got.ldr.view(np.uint32))
+                   + np.count_nonzero(hdr.view(np.uint32) != got.hdr.view(np.uint32))
+                   + np.count_nonzero(ao != got.ao))
+        out["cpu_baseline"] = {
+            "value": sub_rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sub)} of {faces.nfaces} faces ({fs.nsamples} luxels, {sub_rays} rays, "
+            f"{dt:.1f} s); CPU restatement of the reference's gather (sun, sky ambient) plus AO, pthreads",
+        }
+        out["subset_values_differing"] = diff
+        log(f"[bench] batching CPU {sub_rays / dt / 1e6:.2f} Mrays/s on {threads} threads; "
+            f"subset values differing from GPU: {diff}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +242,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-bounce", action="store_true")
     ap.add_argument("--bounce-rooms", type=int, default=12)
     ap.add_argument("--bounce-timed-iters", type=int, default=10)
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--batch-faces", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-cpu-fraction", type=float, default=0.1)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (profiler calibration)")
     a = ap.parse_args(argv)
@@ -275,6 +366,7 @@ def main(argv=None) -> int:
         log(f"[bench] CPU {cpu['value'] / 1e6:.2f} Mrays/s; sample words differing from GPU: {mism}")
     tr.dispose()
     bounce = None if a.no_bounce else bounce_leg(a, log)
+    batching = None if a.no_batching else batching_leg(a, log)
 
     out = {
         "metric": METRIC,
@@ -301,6 +393,7 @@ def main(argv=None) -> int:
         "parity": parity,
         "bounce_wall_s": None if bounce is None else bounce["wall_s_f32"],
         "bounce": bounce,
+        "batching": batching,
     }
     print(json.dumps(out), flush=True)
     return 0

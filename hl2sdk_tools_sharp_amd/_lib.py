@@ -47,7 +47,8 @@ class VhStats(ctypes.Structure):
         ("rays_visibility", ctypes.c_int64),
         ("rays_closest", ctypes.c_int64),
         ("calls", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 5),
+        ("rays_gather", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 4),
     ]
 
 

@@ -291,6 +291,7 @@ int vh_open(const vh_open_opts *opts, vh_ctx **out)
         destroy(c);
         return VH_DECLINED;
     }
+    c->cu_count = prop.multiProcessorCount;
     c->identity = std::string("hip-bvh2-exact-1:") + prop.gcnArchName + ":" +
                   std::to_string(prop.multiProcessorCount) + "cu";
     if (o.fail_stage == 3) {

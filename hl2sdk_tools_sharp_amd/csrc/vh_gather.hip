@@ -87,6 +87,7 @@ struct GatherArgs {
     float *out_ao;
     const float4 *nodes;
     const float4 *tris;
+    unsigned long long *rays;  // rays traced, summed per wave
 };
 
 __device__ __forceinline__ float maxps(float a, float b) { return a > b ? a : b; }
@@ -126,22 +127,24 @@ __device__ __forceinline__ Ray segment(float px, float py, float pz, float ex, f
     return r;
 }
 
-__device__ __forceinline__ bool blocked(const GatherArgs &a, int *stk, const Ray &r)
+__device__ __forceinline__ bool blocked(const GatherArgs &a, int *stk, const Ray &r, int &nr)
 {
     if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
           isfinite(r.oy) && isfinite(r.oz)))
         return false;
+    nr++;
     float t;
     int32_t id;
     return trace_one<kAny, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id) >= 0;
 }
 
 // LightRayLog.Occlusion: the first hit short of the end, unless it is sky.
-__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *stk, const Ray &r)
+__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *stk, const Ray &r, int &nr)
 {
     if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
           isfinite(r.oy) && isfinite(r.oz)))
         return 0.0f;
+    nr++;
     float t;
     int32_t id;
     int32_t h = trace_one<kClosestShort, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id);
@@ -184,6 +187,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
     float pz = a.pos[3 * (size_t)s + 2] + n0z;
     int32_t cl = a.cluster[s];
 
+    int nr = 0;
     float4 acc_l[4], acc_h[4];
     for (int n = 0; n < 4; n++) {
         acc_l[n] = make_float4(0, 0, 0, 0);
@@ -256,7 +260,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                 _Pragma("unroll") for (int n = 1; n < normals; n++)
                     dot[n] = maxps(0.0f, dot3(bn[n - 1][0], bn[n - 1][1], bn[n - 1][2], dx, dy, dz));
                 if (inmask && d0 != 0.0f && fo != 0.0f) {
-                    if (blocked(a, stk, segment(px, py, pz, sx, sy, sz))) dot[0] = 0.0f * dot[0];
+                    if (blocked(a, stk, segment(px, py, pz, sx, sy, sz), nr)) dot[0] = 0.0f * dot[0];
                 }
             }
         } else if (L.type == VH_EMIT_SKYLIGHT) {
@@ -275,7 +279,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                         ez = ez + (jt[2] * js);
                     }
                     float occ = 0.0f;
-                    if (real) occ = sky_occlusion(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz));
+                    if (real) occ = sky_occlusion(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz), nr);
                     total += 1.0f - minps(maxps(occ, 0.0f), 1.0f);
                 }
                 float see = total * (1.0f / (float)a.sun_samples);
@@ -313,7 +317,8 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                     frac = 1.0f - sky_occlusion(a, stk,
                                                 segment(px, py, pz, (ax * -kMaxTrace) + px,
                                                         (ay * -kMaxTrace) + py,
-                                                        (az * -kMaxTrace) + pz));
+                                                        (az * -kMaxTrace) + pz),
+                                                nr);
                 _Pragma("unroll") for (int n = 0; n < normals; n++) amb[n] = amb[n] + (frac * dts[n]);
             }
             falloff = 1.0f;
@@ -367,11 +372,14 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                 r.dy = dirs[3 * k + 1] * a.ao_radius;
                 r.dz = dirs[3 * k + 2] * a.ao_radius;
                 r.reach = 1.0f;
-                if (!blocked(a, stk, r)) unblocked++;
+                if (!blocked(a, stk, r, nr)) unblocked++;
             }
             a.out_ao[s] = (float)unblocked / (float)a.ao_count;
         }
     }
+    // every lane of the wave is here: one atomic per chunk
+    for (int o = 32; o > 0; o >>= 1) nr += __shfl_xor(nr, o, 64);
+    if (lane == 0) atomicAdd(a.rays, (unsigned long long)nr);
 }
 
 __device__ __forceinline__ void light_chunk(const GatherArgs &a, int64_t chunk, int *stk)
@@ -492,6 +500,7 @@ GatherArgs gather_args(vh_ctx *c)
     a.out_ao = (float *)G->out_ao.ptr;
     a.nodes = (const float4 *)c->nodes.ptr;
     a.tris = (const float4 *)c->tris.ptr;
+    a.rays = (unsigned long long *)G->ticket.ptr + 1;
     return a;
 }
 
@@ -631,15 +640,13 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
     VH_HIP(hipMemsetAsync(G->out_ldr.ptr, 0, outb, st));
     VH_HIP(hipMemsetAsync(G->out_hdr.ptr, 0, outb, st));
     VH_HIP(hipMemsetAsync(G->out_ao.ptr, 0, (size_t)G->nsamples * 4, st));
+    VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 16, st));  // queue ticket, ray count
     if (G->nchunks > 0) {
         KernelTimer timer(c, VH_KERNEL_FACELIGHT, st);
         if (schedule == VH_SCHEDULE_QUEUE) {
-            hipDeviceProp_t prop;
-            VH_HIP(hipGetDeviceProperties(&prop, c->device));
             int wpc = waves_per_cu > 0 ? waves_per_cu : 20;
-            int64_t waves = (int64_t)prop.multiProcessorCount * wpc;
+            int64_t waves = (int64_t)c->cu_count * wpc;
             if (waves > G->nchunks) waves = G->nchunks;
-            VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 8, st));
             hipLaunchKernelGGL(k_facelight<true>, dim3((uint32_t)waves), dim3(kWave), 0, st, a,
                                (int64_t)0, (unsigned long long *)G->ticket.ptr);
         } else if (schedule == VH_SCHEDULE_GRID) {
@@ -656,7 +663,11 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
         timer.done();
         VH_HIP(hipGetLastError());
     }
+    unsigned long long rays = 0;
+    VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
     VH_HIP(hipStreamSynchronize(st));
+    c->stats.rays_gather += (int64_t)rays;
+    c->stats.calls++;
     G->have_results = true;
     return VH_OK;
 }

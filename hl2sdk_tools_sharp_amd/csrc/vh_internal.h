@@ -67,6 +67,7 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
 
 struct vh_ctx {
     int device = 0;
+    int cu_count = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string identity;

@@ -155,6 +155,17 @@ class FaceSet:
     def nsamples(self) -> int:
         return self.sample_pos.shape[0]
 
+    def subset(self, faces: np.ndarray) -> "FaceSet":
+        """The listed faces (in the given order) with their samples."""
+        faces = np.asarray(faces, np.int64)
+        lo = self.face_sample_ptr[faces].astype(np.int64)
+        hi = self.face_sample_ptr[faces + 1].astype(np.int64)
+        idx = np.concatenate([np.arange(x, y) for x, y in zip(lo, hi)]) if len(faces) else np.zeros(0, np.int64)
+        ptr = np.zeros(len(faces) + 1, np.int32)
+        np.cumsum(hi - lo, out=ptr[1:])
+        return FaceSet(ptr, self.face_normal_count[faces].copy(), self.face_normals[faces].copy(),
+                       self.sample_pos[idx].copy(), self.sample_cluster[idx].copy())
+
 
 class VhGatherDesc(ctypes.Structure):
     _fields_ = [
@@ -233,6 +244,21 @@ class HipFaceLighter:
             self.tracer.handle, ldr.ctypes.data_as(ctypes.c_void_p), hdr.ctypes.data_as(ctypes.c_void_p),
             ao.ctypes.data_as(ctypes.c_void_p)))
         return GatherResult(ldr, hdr, ao)
+
+    def timed_run(self, schedule: str = "queue", waves_per_cu: int = 0) -> tuple:
+        """One run with the gather kernel timed by HIP events on the library's
+        stream: (kernel ms summed over its launches, launches, rays traced)."""
+        L = _lib.lib()
+        h = self.tracer.handle
+        before = self.tracer.stats()["rays_gather"]
+        _lib.check(L.vh_timing_enable(h, 1))
+        try:
+            self.run(schedule, waves_per_cu)
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_FACELIGHT, ctypes.byref(ms), ctypes.byref(n)))
+        finally:
+            _lib.check(L.vh_timing_enable(h, 0))
+        return ms.value, n.value, self.tracer.stats()["rays_gather"] - before
 
     def light_faces(self, **kw) -> GatherResult:
         self.run(**kw)

@@ -169,7 +169,8 @@ class HipRayTracer:
     def stats(self):
         s = _lib.VhStats()
         _lib.check(_lib.lib().vh_get_stats(self._h, ctypes.byref(s)))
-        return {"rays_visibility": s.rays_visibility, "rays_closest": s.rays_closest, "calls": s.calls}
+        return {"rays_visibility": s.rays_visibility, "rays_closest": s.rays_closest, "calls": s.calls,
+                "rays_gather": s.rays_gather}
 
     # -- the two batch operations ----------------------------------------
     def trace_visibility(self, rays, hit_bits: np.ndarray, options: RayTraceOptions = RayTraceOptions()) -> None:

@@ -141,7 +141,8 @@ typedef struct vh_stats {
     int64_t rays_visibility;
     int64_t rays_closest;
     int64_t calls;
-    int64_t reserved[5];
+    int64_t rays_gather; /* rays traced by vh_gather_run: light, sun, sky and AO */
+    int64_t reserved[4];
 } vh_stats;
 int vh_get_stats(const vh_ctx *ctx, vh_stats *out);
 

@@ -143,3 +143,21 @@ def test_gather_errors():
         bad[0]["type"] = 9
         with pytest.raises(Exception):
             fl.load(faces, bad, light_pvs=pvs)
+
+
+def test_ray_count_matches_closed_form():
+    """AO traces ao_count rays per sample; the sun traces sun_samples rays per
+    sample of every face that faces it (a flat face: one dot for all samples)."""
+    g = rooms.room_grid(2, 1, seed=3)
+    faces = rooms.room_faces(g, luxel=64.0)
+    L = light.make_lights(1)
+    sun_dir = np.array([0.2, 0.3, -0.93], np.float32)
+    L[0]["type"], L[0]["normal"], L[0]["intensity"] = light.EMIT_SKYLIGHT, sun_dir, (1, 1, 1)
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        fl.load(faces, L, sun_extent=0.01, sun_samples=16, ao_count=8, ao_radius=40.0)
+        _ms, n, rays = fl.timed_run("queue")
+    per_face = np.diff(faces.face_sample_ptr)
+    facing = (-(faces.face_normals[:, 0] @ sun_dir)) > 0
+    assert n == 1
+    assert rays == int(per_face.sum()) * 8 + int(per_face[facing].sum()) * 16
