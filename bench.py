@@ -229,6 +229,60 @@ def batching_leg(a, log):
     return out
 
 
+def plain_vrad_leg(a, log):
+    """The plain-vrad config: a 3 x 3 grid of 512 x 512 x 256 rooms (8-unit
+    walls, 128 x 128 doorways) under a sky ceiling, 8 point lights
+    (intensity 50-300, 1/d^2) and one sun, luxel 16, patch 128, FP32, bounce
+    to relative residual 1e-4.  The GPU pass (scene, gather, patch lights,
+    transfer build, bounce, lump) is timed stage by stage, after one untimed
+    pass; the CPU baseline is the oracle's restatement of the same stages on
+    the host's cores, and the bounced light of the two is compared."""
+    import oracle.bounce as ob
+    import oracle.gather as og
+    from hl2sdk_tools_sharp_amd import light, rooms
+    from hl2sdk_tools_sharp_amd.vrad import light_pass
+
+    g = rooms.room_grid(3, 3, size=(512, 512, 256), wall=8, door=128, patch=128, minchop=64,
+                        nlights=8, sky_ceiling=True, seed=1)
+    faces = rooms.room_faces(g, luxel=16.0)
+    lights, pvs = rooms.room_lights(g, sun=True)
+    kw = dict(sun_extent=0.0, sun_samples=1, tol=1e-4)
+    log(f"[bench] plain vrad: {g.npatch} patches, {faces.nsamples} luxels, {len(lights)} lights")
+    light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
+    runs = [light_pass(g, faces, lights, pvs, read_direct=True, **kw) for _ in range(3)]
+    best = min(runs, key=lambda r: r.wall_s)
+    out = {"patches": g.npatch, "luxels": faces.nsamples, "lights": len(lights),
+           "transfers": best.transfers, "iterations": best.bounce.iterations,
+           "wall_s": best.wall_s, "stage_s": {k: round(v, 5) for k, v in best.stage_s.items()}}
+    log(f"[bench] plain vrad GPU pass {best.wall_s * 1e3:.1f} ms: "
+        + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in best.stage_s.items()))
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        d = light.pack_desc(faces, lights, pvs, 0.0, 1, None)
+        ldr, _hdr, _ao = og.gather(g.casters, d, faces.nsamples, threads=threads)
+        direct = ob.patch_lights(g, faces, ldr[0], faces.sample_area)
+        rp, src, w, _nc = ob.build_transfers(g, threads=threads)
+        tot, _res, _en, it = ob.bounce(g, rp, src, w, direct, tol=1e-4, max_iter=100)
+        parts = [ldr[0, faces.face_sample_ptr[f]:faces.face_sample_ptr[f + 1], :3] for f in range(faces.nfaces)]
+        lump = og.encode_rgbe(np.concatenate(parts)).reshape(-1)
+        dt = time.perf_counter() - t0
+        b = best.bounce.total_light.astype(np.float64)
+        out["cpu_baseline"] = {
+            "value": dt, "unit": "s", "cores": threads, "kind": "port",
+            "sample": "the whole pass (gather, patch lights, transfer build, bounce, RGBE lump); "
+            "CPU restatement of the reference's stages, pthreads where the stage is parallel",
+        }
+        out["speedup_vs_cpu"] = dt / best.wall_s
+        out["lump_equal"] = bool(np.array_equal(lump, best.lump))
+        out["direct_equal"] = bool(np.array_equal(direct.view(np.uint32), best.direct.view(np.uint32)))
+        out["bounce_rel_l1_vs_cpu"] = float(np.abs(b - tot).sum() / max(np.abs(tot).sum(), 1e-30))
+        out["cpu_iterations"] = it
+        log(f"[bench] plain vrad CPU {dt:.2f} s on {threads} threads; lump equal {out['lump_equal']}, "
+            f"direct equal {out['direct_equal']}, bounce rel L1 {out['bounce_rel_l1_vs_cpu']:.2e}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +297,7 @@ def main(argv=None) -> int:
     ap.add_argument("--bounce-rooms", type=int, default=12)
     ap.add_argument("--bounce-timed-iters", type=int, default=10)
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--batch-faces", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-cpu-fraction", type=float, default=0.1)
@@ -367,6 +422,7 @@ def main(argv=None) -> int:
     tr.dispose()
     bounce = None if a.no_bounce else bounce_leg(a, log)
     batching = None if a.no_batching else batching_leg(a, log)
+    plain = None if a.no_plain else plain_vrad_leg(a, log)
 
     out = {
         "metric": METRIC,
@@ -394,6 +450,7 @@ def main(argv=None) -> int:
         "bounce_wall_s": None if bounce is None else bounce["wall_s_f32"],
         "bounce": bounce,
         "batching": batching,
+        "plain_vrad": plain,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -118,6 +118,7 @@ SIGNATURES = {
     "vh_gather_read": (ctypes.c_int, [P, P, P, P]),
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
+    "vh_patch_lights": (ctypes.c_int, [P, I32, P, P]),
     "vh_timing_enable": (ctypes.c_int, [P, I32]),
     "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
     "vh_count_enable": (ctypes.c_int, [P, I32]),

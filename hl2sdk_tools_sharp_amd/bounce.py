@@ -121,13 +121,29 @@ class HipBounceStage:
         HipRayTracer._raise(_lib.lib().vh_transfers_read(self._h, _p(rp), _p(src), _p(w), max(T, 1)))
         return rp, src[:T], w[:T]
 
+    def patch_lights(self, lighter, hdr: bool = False, read: bool = True):
+        """PatchLighting.BuildPatchLights: the last gather of ``lighter`` (a
+        HipFaceLighter over this patch system's faces, with sample areas)
+        handed to the patches.  The result stays on the device for a
+        ``bounce(None)``; returns it as (P, 3) when ``read``."""
+        if not self._loaded:
+            self.load_patches()
+        f = lighter.faces
+        if f is None or f.sample_area is None:
+            raise ValueError("the lighter's faces carry no sample areas")
+        area = np.ascontiguousarray(f.sample_area, np.float32)
+        out = np.zeros((self.patches.npatch, 3), np.float32) if read else None
+        HipRayTracer._raise(_lib.lib().vh_patch_lights(self._h, 1 if hdr else 0, _p(area), _p(out)))
+        return out
+
     def bounce(self, emit0, start=None, weights: str = "f32", max_iter: int = 100, tol: float = 0.0,
                use_graph: bool = True) -> BounceResult:
         """Radiosity.BounceAsync.  ``tol`` > 0 iterates to that relative residual;
         ``tol`` <= 0 keeps the reference's stop rule.  ``start`` (an estimate of
-        the bounced light) switches to the from-start form."""
+        the bounced light) switches to the from-start form.  ``emit0`` None
+        starts from the device-resident result of ``patch_lights``."""
         P = self.patches.npatch
-        e0 = np.ascontiguousarray(emit0, np.float32).reshape(P, 3)
+        e0 = None if emit0 is None else np.ascontiguousarray(emit0, np.float32).reshape(P, 3)
         st = None if start is None else np.ascontiguousarray(start, np.float32).reshape(P, 3)
         opts = _lib.VhBounceOpts(
             _lib.VH_WEIGHTS_F16 if weights == "f16" else _lib.VH_WEIGHTS_F32,

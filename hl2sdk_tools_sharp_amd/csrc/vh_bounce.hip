@@ -41,6 +41,9 @@ struct BounceData {
     vh::DevBuf row_ptr, src, w32, w16, rscale, gather_rows;
     // bounce state
     vh::DevBuf e0, emit, shoot, add, total, partials, result;
+    // the faces' direct light handed to the patches (vh_patch_lights)
+    vh::DevBuf direct, sample_area;
+    bool have_direct = false;
     double *h_result = nullptr;  // pinned
 };
 
@@ -546,6 +549,63 @@ int alloc_transfers(BounceData *B, int64_t total)
     return VH_OK;
 }
 
+// AddSampleToPatch and the leaf step of BuildPatchLights
+// (Rad/Light/PatchLighting.cs:65-130, 194-249): a lane per leaf patch walks
+// its face's samples in sample order.  A sample whose average light is under 1
+// is dropped; one whose position grown by sqrt(area)/2 (a float root, a double
+// halving narrowed) overlaps the patch winding's bounds adds its area and
+// area * light.  The leaf's direct light is SampleLight * (float)(1/SampleArea).
+// Sky patches take nothing.  Parents follow by k_pullup, level by level.
+__global__ __launch_bounds__(kCB) void k_patch_lights(const int32_t *__restrict__ leaves, int32_t nleaves,
+                                                      const int32_t *__restrict__ face,
+                                                      const uint8_t *__restrict__ sky,
+                                                      const float *__restrict__ winding,
+                                                      const int32_t *__restrict__ face_ptr,
+                                                      const float *__restrict__ pos,
+                                                      const float *__restrict__ sarea,
+                                                      const float4 *__restrict__ light,
+                                                      float4 *__restrict__ direct)
+{
+    int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (i >= nleaves) return;
+    int32_t p = leaves[i];
+    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!sky[p]) {
+        const float *w = winding + (size_t)p * 12;
+        float mn[3], mx[3];
+        for (int k = 0; k < 3; k++) mn[k] = mx[k] = w[k];
+        for (int q = 1; q < 4; q++)
+            for (int k = 0; k < 3; k++) {
+                float v = w[3 * q + k];
+                mn[k] = v < mn[k] ? v : mn[k];
+                mx[k] = v > mx[k] ? v : mx[k];
+            }
+        int32_t f = face[p];
+        float sa = 0.0f, lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        for (int32_t s = face_ptr[f]; s < face_ptr[f + 1]; s++) {
+            float4 L = light[s];
+            if (((L.x + L.y) + L.z) / 3.0f < 1.0f) continue;
+            float a = sarea[s];
+            float r = (float)((double)sqrtf(a) / 2.0);
+            bool inside = true;
+            for (int k = 0; k < 3; k++) {
+                float x = pos[3 * (size_t)s + k];
+                if (mn[k] > x + r || mx[k] < x - r) inside = false;
+            }
+            if (!inside) continue;
+            sa += a;
+            lx = lx + (a * L.x);
+            ly = ly + (a * L.y);
+            lz = lz + (a * L.z);
+        }
+        if (sa != 0.0f) {
+            float sc = (float)(1.0 / (double)sa);
+            out = make_float4(lx * sc, ly * sc, lz * sc, 0.0f);
+        }
+    }
+    direct[p] = out;
+}
+
 int upload_gather_rows(BounceData *B, const std::vector<int64_t> &row_ptr, hipStream_t s)
 {
     // rows with transfers, in patch order so the CSR streams sequentially
@@ -568,7 +628,8 @@ void bounce_release(vh_ctx *c)
                       &B->cluster, &B->winding, &B->refl, &B->sky, &B->face_sky, &B->face_parent,
                       &B->row_faces_ptr, &B->row_faces, &B->receivers, &B->leaves, &B->levels,
                       &B->row_ptr, &B->src, &B->w32, &B->w16, &B->rscale, &B->gather_rows,
-                      &B->e0, &B->emit, &B->shoot, &B->add, &B->total, &B->partials, &B->result})
+                      &B->e0, &B->emit, &B->shoot, &B->add, &B->total, &B->partials, &B->result,
+                      &B->direct, &B->sample_area})
         b->release();
     if (B->h_result) hipHostFree(B->h_result);
     delete B;
@@ -901,13 +962,17 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
               float *total_out, double *residuals, double *energies, int32_t *iters)
 {
     clear_error();
-    if (!c || !o || !emit0) {
-        set_error("NULL context, options or emission");
+    if (!c || !o) {
+        set_error("NULL context or options");
         return VH_EINVAL;
     }
     std::lock_guard<std::mutex> lk(c->mu);
     int rc = check_loaded(c, true);
     if (rc) return rc;
+    if (!emit0 && !c->bounce->have_direct) {
+        set_error("NULL emission and no vh_patch_lights result to start from");
+        return VH_ESTATE;
+    }
     if (o->max_iter < 1 || (o->weights != VH_WEIGHTS_F32 && o->weights != VH_WEIGHTS_F16) ||
         (o->mode != VH_BOUNCE_PROGRESSIVE && o->mode != VH_BOUNCE_FROM_START) || std::isnan(o->tol)) {
         set_error("bad bounce options");
@@ -918,9 +983,13 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     hipStream_t s = c->stream;
     int32_t P = B->npatch;
     std::vector<float> buf((size_t)P * 4, 0.0f);
-    for (int32_t p = 0; p < P; p++)
-        for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
-    VH_HIP(hipMemcpyAsync(B->e0.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+    if (emit0) {
+        for (int32_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
+        VH_HIP(hipMemcpyAsync(B->e0.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+    } else {
+        VH_HIP(hipMemcpyAsync(B->e0.ptr, B->direct.ptr, buf.size() * 4, hipMemcpyDeviceToDevice, s));
+    }
     float4 *startd = nullptr;
     if (o->mode == VH_BOUNCE_FROM_START && start) {
         for (int32_t p = 0; p < P; p++)
@@ -1055,6 +1124,65 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     }
     VH_HIP(hipStreamSynchronize(s));
     if (iters) *iters = it;
+    return VH_OK;
+}
+
+int vh_patch_lights(vh_ctx *c, int32_t which, const float *sample_area, float *direct)
+{
+    clear_error();
+    if (!c || (which != 0 && which != 1)) {
+        set_error("NULL context or bad lighting pass");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, false);
+    if (rc) return rc;
+    GatherView gv;
+    if (!gather_view(c, &gv) || !gv.have_results) {
+        set_error("no gather results: call vh_gather_load and vh_gather_run first");
+        return VH_ESTATE;
+    }
+    BounceData *B = c->bounce;
+    if (gv.nfaces != B->nface) {
+        set_error("the gather has " + std::to_string(gv.nfaces) + " faces and the patch system " +
+                  std::to_string(B->nface) + "; they must be the same faces");
+        return VH_EINVAL;
+    }
+    if (gv.nsamples > 0 && !sample_area) {
+        set_error("NULL sample areas");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    int32_t P = B->npatch;
+    if ((rc = upload(B->sample_area, sample_area, (size_t)gv.nsamples, "sample areas", s)) ||
+        (rc = B->direct.ensure((size_t)P * 16 + 16, "patch direct light")))
+        return rc;
+    const float4 *oa = (const float4 *)B->origin_area.ptr;
+    const int2 *children = (const int2 *)B->children.ptr;
+    const int32_t *levels = (const int32_t *)B->levels.ptr;
+    float4 *d = (float4 *)B->direct.ptr;
+    if (B->nleaves > 0)
+        hipLaunchKernelGGL(k_patch_lights, dim3((uint32_t)((B->nleaves + kCB - 1) / kCB)), dim3(kCB), 0,
+                           s, (const int32_t *)B->leaves.ptr, B->nleaves, (const int32_t *)B->face.ptr,
+                           (const uint8_t *)B->sky.ptr, (const float *)B->winding.ptr, gv.face_ptr,
+                           gv.pos, (const float *)B->sample_area.ptr, which ? gv.hdr : gv.ldr, d);
+    for (int32_t lv = 0; lv < B->height; lv++) {
+        int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
+        if (n > 0)
+            hipLaunchKernelGGL(k_pullup<false>, dim3((uint32_t)((n + kCB - 1) / kCB)), dim3(kCB), 0, s,
+                               levels + a, n, children, oa, d, nullptr, (const float4 *)B->refl.ptr);
+    }
+    VH_HIP(hipGetLastError());
+    if (direct) {
+        std::vector<float> buf((size_t)P * 4);
+        VH_HIP(hipMemcpyAsync(buf.data(), d, buf.size() * 4, hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        for (int32_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) direct[3 * p + k] = buf[4 * p + k];
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    B->have_direct = true;
     return VH_OK;
 }
 

@@ -53,6 +53,20 @@ void gather_release(vh_ctx *c)
     c->gather = nullptr;
 }
 
+bool gather_view(vh_ctx *c, GatherView *v)
+{
+    GatherData *G = c->gather;
+    if (!G) return false;
+    v->nfaces = G->nfaces;
+    v->nsamples = G->nsamples;
+    v->face_ptr = (const int32_t *)G->face_ptr.ptr;
+    v->pos = (const float *)G->pos.ptr;
+    v->ldr = (const float4 *)G->out_ldr.ptr;
+    v->hdr = (const float4 *)G->out_hdr.ptr;
+    v->have_results = G->have_results;
+    return true;
+}
+
 namespace {
 
 constexpr int kWave = 64;

@@ -119,6 +119,16 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
 void bounce_release(vh_ctx *c);
 // Frees the gather's buffers (vh_gather.hip).
 void gather_release(vh_ctx *c);
+// The loaded gather's faces and results, for the patch-light hand-off.
+struct GatherView {
+    int32_t nfaces = 0, nsamples = 0;
+    const int32_t *face_ptr = nullptr;  // device, nfaces + 1
+    const float *pos = nullptr;         // device, 3 per sample
+    const float4 *ldr = nullptr;        // device, slot 0 = the flat normal
+    const float4 *hdr = nullptr;
+    bool have_results = false;
+};
+bool gather_view(vh_ctx *c, GatherView *v);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

@@ -146,6 +146,7 @@ class FaceSet:
     face_normals: np.ndarray  # (F, 4, 3)
     sample_pos: np.ndarray  # (S, 3)
     sample_cluster: np.ndarray  # (S,)
+    sample_area: Optional[np.ndarray] = None  # (S,) luxel areas, for the patch-light hand-off
 
     @property
     def nfaces(self) -> int:
@@ -164,7 +165,8 @@ class FaceSet:
         ptr = np.zeros(len(faces) + 1, np.int32)
         np.cumsum(hi - lo, out=ptr[1:])
         return FaceSet(ptr, self.face_normal_count[faces].copy(), self.face_normals[faces].copy(),
-                       self.sample_pos[idx].copy(), self.sample_cluster[idx].copy())
+                       self.sample_pos[idx].copy(), self.sample_cluster[idx].copy(),
+                       None if self.sample_area is None else self.sample_area[idx].copy())
 
 
 class VhGatherDesc(ctypes.Structure):

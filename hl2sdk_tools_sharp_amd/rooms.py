@@ -150,7 +150,10 @@ def room_grid(nx: int = 3, ny: int = 3, size=(512.0, 512.0, 256.0), wall: float 
     faces = []  # (lo, hi, axis, sign, cluster, sky)
 
     def add_face(lo, hi, ax, sign, cluster, sky=False):
-        faces.append((np.array(lo, float), np.array(hi, float), ax, sign, cluster, sky))
+        lo, hi = np.array(lo, float), np.array(hi, float)
+        ext = np.delete(hi - lo, ax)
+        if np.all(ext > 0):  # a doorway as tall as the room leaves no lintel face
+            faces.append((lo, hi, ax, sign, cluster, sky))
 
     half = door / 2.0
     for j in range(ny):
@@ -401,23 +404,30 @@ BOUNCE_ROOM = (1344.0, 1344.0, 384.0)
 
 def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
     """The grid's faces as lightmaps: one sample at the centre of every
-    ``luxel`` x ``luxel`` cell of each face, the face's cluster for each
-    sample; every ``bumped_every``-th face (0 = none) carries the three bump
-    normals as well."""
+    ``luxel`` x ``luxel`` cell of each face (its area the cell's), the face's
+    cluster for each sample; every ``bumped_every``-th face (0 = none) carries
+    the three bump normals as well.  Sky faces carry no lightmap (vrad lights
+    no sky surface) and so have no samples."""
     from .light import FaceSet, bump_basis
 
     ptr = [0]
     pos = []
     cl = []
+    ar = []
     nc = np.ones(g.nface, np.int32)
     normals = np.zeros((g.nface, 4, 3), np.float32)
     for f in range(g.nface):
         lo, hi = g.face_lo[f].astype(np.float64), g.face_hi[f].astype(np.float64)
         n = g.face_normal[f]
+        normals[f, 0] = n
+        if g.face_sky[f]:
+            ptr.append(ptr[-1])
+            continue
         ax = int(np.argmax(np.abs(n)))
         u, v = (ax + 1) % 3, (ax + 2) % 3
         nu = max(1, int(round((hi[u] - lo[u]) / luxel)))
         nv = max(1, int(round((hi[v] - lo[v]) / luxel)))
+        ar.append(np.full(nu * nv, (hi[u] - lo[u]) / nu * (hi[v] - lo[v]) / nv))
         a = lo[u] + (np.arange(nu) + 0.5) * (hi[u] - lo[u]) / nu
         b = lo[v] + (np.arange(nv) + 0.5) * (hi[v] - lo[v]) / nv
         A, B = np.meshgrid(a, b, indexing="ij")
@@ -428,12 +438,11 @@ def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
         pos.append(p)
         cl.append(np.full(nu * nv, g.face_cluster[f], np.int32))
         ptr.append(ptr[-1] + nu * nv)
-        normals[f, 0] = n
         if bumped_every and f % bumped_every == 0:
             nc[f] = 4
             normals[f, 1:] = bump_basis(n)
     return FaceSet(np.array(ptr, np.int32), nc, normals, np.concatenate(pos).astype(np.float32),
-                   np.concatenate(cl))
+                   np.concatenate(cl), np.concatenate(ar).astype(np.float32))
 
 
 def room_lights(g: RoomGrid, sun: bool = True, sky_ambient: bool = False, spot_every: int = 0,

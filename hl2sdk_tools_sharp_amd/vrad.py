@@ -1,0 +1,86 @@
+"""The lighting pass end to end on one GPU, in vrad's stage order.
+
+The reference runs a pass as FaceLight (the per-luxel direct gather,
+Rad/RadWorld.Facelights.cs), BuildPatchLights (the luxels' light handed to
+the radiosity patches, Rad/Light/PatchLighting.cs), the transfer build
+(Rad/Bounce/VisMatrix.cs), the bounce (Rad/Bounce/Radiosity.cs, behind
+IVradBounceStage, Rad/VradStages.cs:112-122) and the lump write
+(Rad/Final/FinalLightFace.cs with StockLightColor.Encode).  Here every stage
+runs in libvradhip.so against one scene upload; the patch lights stay on the
+device and feed the bounce without a host round trip.  The lump holds the
+direct light (the luxel radial filter that folds bounced light back into
+luxels is not part of this pass).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+from .bounce import BounceResult, HipBounceStage
+from .light import FaceSet, HipFaceLighter
+from .tracer import HipTracerFactory
+
+
+@dataclass
+class PassResult:
+    """One lighting pass: per-stage wall seconds, the patches' direct light,
+    the bounce, the lump bytes."""
+
+    stage_s: Dict[str, float] = field(default_factory=dict)
+    direct: Optional[np.ndarray] = None
+    bounce: Optional[BounceResult] = None
+    lump: Optional[np.ndarray] = None
+    transfers: int = 0
+
+    @property
+    def wall_s(self) -> float:
+        return float(sum(self.stage_s.values()))
+
+
+def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bool = False,
+               sun_extent: float = 0.0, sun_samples: int = 30, sky_dirs=None, ao_count: int = 0,
+               ao_radius: float = 40.0, weights: str = "f32", tol: float = 1e-4, max_iter: int = 100,
+               device: int = 0, read_direct: bool = False) -> PassResult:
+    """Runs one pass (LDR, or HDR with ``hdr``) over room grid ``g`` whose
+    faces, in the patch system's order, are ``faces``.  Raises if the HIP
+    tracer declines (there is no CPU path)."""
+    out = PassResult()
+    clock = time.perf_counter
+
+    t = clock()
+    offer = HipTracerFactory(device=device).try_create(g.casters)
+    if offer.tracer is None:
+        raise RuntimeError(f"HIP tracer declined: {offer.decline_reason}")
+    tr = offer.tracer
+    out.stage_s["scene"] = clock() - t
+    try:
+        t = clock()
+        fl = HipFaceLighter(tr)
+        fl.load(faces, lights, light_pvs=light_pvs, sun_extent=sun_extent, sun_samples=sun_samples,
+                sky_dirs=sky_dirs, ao_count=ao_count, ao_radius=ao_radius)
+        fl.run("queue")
+        out.stage_s["facelight"] = clock() - t
+
+        t = clock()
+        st = HipBounceStage(tr, g)
+        st.load_patches()
+        out.direct = st.patch_lights(fl, hdr=hdr, read=read_direct)
+        out.stage_s["patch_lights"] = clock() - t
+
+        t = clock()
+        out.transfers = st.build_transfers()
+        out.stage_s["transfers"] = clock() - t
+
+        t = clock()
+        out.bounce = st.bounce(None, weights=weights, tol=tol, max_iter=max_iter)
+        out.stage_s["bounce"] = clock() - t
+
+        t = clock()
+        out.lump = fl.lightmap(hdr=hdr)
+        out.stage_s["lump"] = clock() - t
+    finally:
+        tr.dispose()
+    return out

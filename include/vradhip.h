@@ -212,7 +212,8 @@ typedef struct vh_bounce_opts {
  * for flat patches: MoveDirectLightToEmission, then per iteration GatherLight
  * over every receiver's transfers and CollectLight (leaves take the gathered
  * light, parents the area-weighted light of their children, sky emits 0).
- * emit0: each patch's direct light (3 per patch); start: the bounced-light
+ * emit0: each patch's direct light (3 per patch; NULL = the device-resident
+ * result of the last vh_patch_lights); start: the bounced-light
  * estimate for VH_BOUNCE_FROM_START (3 per patch; NULL = zero).  Writes
  * total_out (3 per patch: the bounced light, parents pulled up), and per
  * iteration the residual and the summed leaf emission (3 doubles). */
@@ -302,6 +303,20 @@ int vh_encode_rgbe(vh_ctx *ctx, const float *rgb, int64_t n, int32_t stride, uin
  * running sum of 4 * normals * samples (FinalLightFace.cs:270 with one style).
  * out needs 4 * sum(normals * samples) bytes. */
 int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacity);
+
+/* PatchLighting.BuildPatchLights with AddSampleToPatch
+ * (SourceSharp.MapTools/Rad/Light/PatchLighting.cs:65-249): the last gather's
+ * flat style-0 light (which = 0 LDR, 1 HDR) handed to the loaded patches.
+ * The gather's faces must be the patch system's faces, in the same order;
+ * sample_area gives each gather sample's area (nsamples floats).  Each
+ * non-sky leaf patch sums, in sample order, area and area * light over the
+ * samples of its face whose average light is at least 1 and whose position
+ * grown by sqrt(area)/2 overlaps the patch winding's bounds, and takes
+ * light sum * (float)(1 / area sum); parents take their children's
+ * area-weighted blend.  The result stays on the device as the next
+ * vh_bounce's emission when that call passes emit0 = NULL; direct (3 per
+ * patch, may be NULL) receives a copy. */
+int vh_patch_lights(vh_ctx *ctx, int32_t which, const float *sample_area, float *direct);
 
 /* Kernel timing with HIP events recorded on the launching stream around each
  * launch of the named kernel (no counterpart in the reference: the bench and

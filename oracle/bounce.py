@@ -163,3 +163,43 @@ def bounce(g, row_ptr, src, w, emit0, start=None, mode=0, max_iter=100, tol=0.0,
                 ctypes.byref(it))
     n = it.value
     return out, res[:n], en[:n], n
+
+
+def patch_lights(g, faces, light, sample_area):
+    """PatchLighting.BuildPatchLights with AddSampleToPatch
+    (Rad/Light/PatchLighting.cs:65-249), numpy, in the reference's float order.
+
+    ``light`` is each sample's flat style-0 lighting (S, >=3); ``faces`` has the
+    patch system's face order.  Every leaf patch that is not sky takes, in
+    sample order, each sample of its face whose average light is at least 1 and
+    whose position grown by sqrt(area)/2 (a float sqrt, a double halving
+    narrowed) overlaps the patch winding's bounds: SampleArea += area,
+    SampleLight += area * light.  A leaf's direct light is SampleLight *
+    (float)(1.0 / SampleArea) (0 with no sample); a parent's is the
+    area-weighted blend of its children's, children first (:133-148).
+    Returns (P, 3) float32."""
+    f32 = np.float32
+    light = np.asarray(light, f32)[:, :3]
+    area = np.asarray(sample_area, f32)
+    pos = np.asarray(faces.sample_pos, f32)
+    mins = g.winding.astype(f32).min(axis=1)
+    maxs = g.winding.astype(f32).max(axis=1)
+    avg_ok = ~(((light[:, 0] + light[:, 1]) + light[:, 2]) / f32(3) < f32(1))
+    radius = (np.sqrt(area).astype(np.float64) / 2.0).astype(f32)
+    direct = np.zeros((g.npatch, 3), f32)
+    sky = g.face_sky[g.face].astype(bool)
+    for p in np.nonzero((g.child1 < 0) & ~sky)[0]:
+        f = g.face[p]
+        a0, a1 = faces.face_sample_ptr[f], faces.face_sample_ptr[f + 1]
+        if a1 == a0:
+            continue
+        ps, rs = pos[a0:a1], radius[a0:a1, None]
+        inside = ~np.any((mins[p] > ps + rs) | (maxs[p] < ps - rs), axis=1)
+        m = inside & avg_ok[a0:a1]
+        if not m.any():
+            continue
+        sa = np.cumsum(np.where(m, area[a0:a1], f32(0)), dtype=f32)[-1]
+        contrib = np.where(m[:, None], area[a0:a1, None] * light[a0:a1], f32(0)).astype(f32)
+        sl = np.cumsum(contrib, axis=0, dtype=f32)[-1]
+        direct[p] = sl * f32(1.0 / np.float64(sa))
+    return g.pull_up(direct)
