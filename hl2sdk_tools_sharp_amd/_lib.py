@@ -92,6 +92,19 @@ P = ctypes.c_void_p
 I32, I64, F32, F64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double
 
 # name -> (restype, argtypes); the single table of what this module binds.
+class VhMlpOpts(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_float), ("floor", ctypes.c_float), ("gain", ctypes.c_float),
+                ("scale", ctypes.c_float)]
+
+
+class VhMlpTrainOpts(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("seed", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+    ]
+
+
 SIGNATURES = {
     "vh_open": (ctypes.c_int, [ctypes.POINTER(VhOpenOpts), ctypes.POINTER(P)]),
     "vh_close": (None, [P]),
@@ -119,6 +132,15 @@ SIGNATURES = {
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
     "vh_patch_lights": (ctypes.c_int, [P, I32, P, P]),
+    "vh_mlp_init": (ctypes.c_int, [P, P, P, ctypes.POINTER(VhMlpOpts)]),
+    "vh_mlp_params": (ctypes.c_int, [P, P]),
+    "vh_mlp_forward": (ctypes.c_int, [P, P, I64, P]),
+    "vh_mlp_grad": (ctypes.c_int, [P, P, P, I64, P, ctypes.POINTER(F32)]),
+    "vh_mlp_adam": (ctypes.c_int, [P, P, F32, F32, F32, F32]),
+    "vh_mlp_features": (ctypes.c_int, [P, P, P]),
+    "vh_mlp_dataset": (ctypes.c_int, [P, I32, ctypes.POINTER(I64)]),
+    "vh_mlp_train": (ctypes.c_int, [P, ctypes.POINTER(VhMlpTrainOpts), P]),
+    "vh_mlp_predict": (ctypes.c_int, [P, P, P]),
     "vh_timing_enable": (ctypes.c_int, [P, I32]),
     "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
     "vh_count_enable": (ctypes.c_int, [P, I32]),
@@ -131,11 +153,17 @@ VH_KERNEL_CLOSEST = 2
 VH_KERNEL_GATHER = 3
 VH_KERNEL_TRANSFERS = 4
 VH_KERNEL_FACELIGHT = 5
+VH_KERNEL_MLP = 6
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1
 VH_BOUNCE_PROGRESSIVE = 0
 VH_BOUNCE_FROM_START = 1
+VH_BOUNCE_FROM_PREDICTION = 2
+VH_MLP_PARAMS = 36739
+VH_MLP_INPUTS = 25
+VH_MLP_DATA_CLEAR = 0
+VH_MLP_DATA_ADD = 1
 
 
 def lib():

@@ -137,17 +137,20 @@ class HipBounceStage:
         return out
 
     def bounce(self, emit0, start=None, weights: str = "f32", max_iter: int = 100, tol: float = 0.0,
-               use_graph: bool = True) -> BounceResult:
+               use_graph: bool = True, from_prediction: bool = False) -> BounceResult:
         """Radiosity.BounceAsync.  ``tol`` > 0 iterates to that relative residual;
         ``tol`` <= 0 keeps the reference's stop rule.  ``start`` (an estimate of
-        the bounced light) switches to the from-start form.  ``emit0`` None
-        starts from the device-resident result of ``patch_lights``."""
+        the bounced light) switches to the from-start form; ``from_prediction``
+        starts from the warm-start network's last prediction instead.
+        ``emit0`` None starts from the device-resident result of
+        ``patch_lights``."""
         P = self.patches.npatch
         e0 = None if emit0 is None else np.ascontiguousarray(emit0, np.float32).reshape(P, 3)
         st = None if start is None else np.ascontiguousarray(start, np.float32).reshape(P, 3)
         opts = _lib.VhBounceOpts(
             _lib.VH_WEIGHTS_F16 if weights == "f16" else _lib.VH_WEIGHTS_F32,
-            _lib.VH_BOUNCE_FROM_START if st is not None else _lib.VH_BOUNCE_PROGRESSIVE,
+            _lib.VH_BOUNCE_FROM_PREDICTION if from_prediction
+            else _lib.VH_BOUNCE_FROM_START if st is not None else _lib.VH_BOUNCE_PROGRESSIVE,
             int(max_iter), 1 if use_graph else 0, float(tol),
         )
         total = np.zeros((P, 3), np.float32)

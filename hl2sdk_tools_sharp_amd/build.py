@@ -28,6 +28,7 @@ SOURCES = [
     "vh_trace.hip",
     "vh_bounce.hip",
     "vh_gather.hip",
+    "vh_mlp.hip",
 ]
 
 FLAGS = [

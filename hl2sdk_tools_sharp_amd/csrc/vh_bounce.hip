@@ -44,6 +44,10 @@ struct BounceData {
     // the faces' direct light handed to the patches (vh_patch_lights)
     vh::DevBuf direct, sample_area;
     bool have_direct = false;
+    // the last bounce's emission and totals are valid; a predicted start
+    bool have_e0 = false, have_total = false, have_pred = false;
+    vh::DevBuf pred;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of the patch origins
     double *h_result = nullptr;  // pinned
 };
 
@@ -629,11 +633,55 @@ void bounce_release(vh_ctx *c)
                       &B->row_faces_ptr, &B->row_faces, &B->receivers, &B->leaves, &B->levels,
                       &B->row_ptr, &B->src, &B->w32, &B->w16, &B->rscale, &B->gather_rows,
                       &B->e0, &B->emit, &B->shoot, &B->add, &B->total, &B->partials, &B->result,
-                      &B->direct, &B->sample_area})
+                      &B->direct, &B->sample_area, &B->pred})
         b->release();
     if (B->h_result) hipHostFree(B->h_result);
     delete B;
     c->bounce = nullptr;
+}
+
+bool bounce_view(vh_ctx *c, BounceView *v)
+{
+    BounceData *B = c->bounce;
+    if (!B || B->npatch == 0) return false;
+    v->npatch = B->npatch;
+    v->nleaves = B->nleaves;
+    v->origin_area = (const float4 *)B->origin_area.ptr;
+    v->normal_pd = (const float4 *)B->normal_pd.ptr;
+    v->e0 = B->have_e0 ? (const float4 *)B->e0.ptr : nullptr;
+    v->direct = B->have_direct ? (const float4 *)B->direct.ptr : nullptr;
+    v->total = B->have_total ? (const float4 *)B->total.ptr : nullptr;
+    v->leaves = (const int32_t *)B->leaves.ptr;
+    v->sky = (const uint8_t *)B->sky.ptr;
+    v->cluster = (const int32_t *)B->cluster.ptr;
+    for (int k = 0; k < 3; k++) {
+        v->lo[k] = B->lo[k];
+        v->hi[k] = B->hi[k];
+    }
+    return true;
+}
+
+float4 *bounce_pred_buffer(vh_ctx *c)
+{
+    BounceData *B = c->bounce;
+    if (!B || B->pred.ensure((size_t)B->npatch * 16 + 16, "predicted start")) return nullptr;
+    B->have_pred = false;
+    return (float4 *)B->pred.ptr;
+}
+
+void bounce_pred_ready(vh_ctx *c) { c->bounce->have_pred = true; }
+
+void bounce_pull_up(vh_ctx *c, float4 *v, hipStream_t s)
+{
+    BounceData *B = c->bounce;
+    const int32_t *levels = (const int32_t *)B->levels.ptr;
+    for (int32_t lv = 0; lv < B->height; lv++) {
+        int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
+        if (n > 0)
+            hipLaunchKernelGGL(k_pullup<false>, dim3((uint32_t)((n + kCB - 1) / kCB)), dim3(kCB), 0, s,
+                               levels + a, n, (const int2 *)B->children.ptr,
+                               (const float4 *)B->origin_area.ptr, v, nullptr, (const float4 *)B->refl.ptr);
+    }
 }
 
 }  // namespace vh
@@ -736,6 +784,16 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
     std::vector<int32_t> leaves, levels;
     for (int32_t p = 0; p < P; p++)
         if (d->child1[p] == -1) leaves.push_back(p);
+    for (int k = 0; k < 3; k++) {
+        B->lo[k] = P > 0 ? d->origin[k] : 0.0f;
+        B->hi[k] = B->lo[k];
+    }
+    for (int32_t p = 0; p < P; p++)
+        for (int k = 0; k < 3; k++) {
+            float v = d->origin[3 * p + k];
+            B->lo[k] = v < B->lo[k] ? v : B->lo[k];
+            B->hi[k] = v > B->hi[k] ? v : B->hi[k];
+        }
     B->level_ptr.assign(1, 0);
     for (int32_t lv = 1; lv <= H; lv++) {
         for (int32_t p = 0; p < P; p++)
@@ -974,10 +1032,17 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
         return VH_ESTATE;
     }
     if (o->max_iter < 1 || (o->weights != VH_WEIGHTS_F32 && o->weights != VH_WEIGHTS_F16) ||
-        (o->mode != VH_BOUNCE_PROGRESSIVE && o->mode != VH_BOUNCE_FROM_START) || std::isnan(o->tol)) {
+        (o->mode != VH_BOUNCE_PROGRESSIVE && o->mode != VH_BOUNCE_FROM_START &&
+         o->mode != VH_BOUNCE_FROM_PREDICTION) || std::isnan(o->tol)) {
         set_error("bad bounce options");
         return VH_EINVAL;
     }
+    if (o->mode == VH_BOUNCE_FROM_PREDICTION && !c->bounce->have_pred) {
+        set_error("no predicted start: call vh_mlp_predict first");
+        return VH_ESTATE;
+    }
+    // the kernels know two forms: progressive (0) and from a start (1)
+    const int kmode = o->mode == VH_BOUNCE_PROGRESSIVE ? 0 : 1;
     VH_HIP(hipSetDevice(c->device));
     BounceData *B = c->bounce;
     hipStream_t s = c->stream;
@@ -990,7 +1055,13 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     } else {
         VH_HIP(hipMemcpyAsync(B->e0.ptr, B->direct.ptr, buf.size() * 4, hipMemcpyDeviceToDevice, s));
     }
+    B->have_e0 = true;
+    B->have_total = false;
     float4 *startd = nullptr;
+    if (o->mode == VH_BOUNCE_FROM_PREDICTION) {
+        VH_HIP(hipMemcpyAsync(B->total.ptr, B->pred.ptr, buf.size() * 4, hipMemcpyDeviceToDevice, s));
+        startd = (float4 *)B->total.ptr;
+    }
     if (o->mode == VH_BOUNCE_FROM_START && start) {
         for (int32_t p = 0; p < P; p++)
             for (int k = 0; k < 3; k++) buf[4 * p + k] = start[3 * p + k];
@@ -1007,11 +1078,11 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     const float4 *oa = (const float4 *)B->origin_area.ptr;
     const uint8_t *sky = (const uint8_t *)B->sky.ptr;
     // k_init reads the start from `total` and writes it back, so alias is safe per element
-    hipLaunchKernelGGL(k_init, dim3((uint32_t)((P + kCB - 1) / kCB)), dim3(kCB), 0, s, P, o->mode,
+    hipLaunchKernelGGL(k_init, dim3((uint32_t)((P + kCB - 1) / kCB)), dim3(kCB), 0, s, P, kmode,
                        e0, (const float4 *)startd, children, sky, refl, emit, shoot, total, add);
     VH_HIP(hipGetLastError());
     const int32_t *levels = (const int32_t *)B->levels.ptr;
-    if (o->mode == VH_BOUNCE_FROM_START)
+    if (kmode == 1)
         for (int32_t lv = 0; lv < B->height; lv++) {
             int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
             if (n > 0)
@@ -1043,7 +1114,7 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
         }
         if (nblk > 0)
             hipLaunchKernelGGL(k_collect, dim3((uint32_t)nblk), dim3(kCB), 0, s,
-                               (const int32_t *)B->leaves.ptr, B->nleaves, sky, o->mode,
+                               (const int32_t *)B->leaves.ptr, B->nleaves, sky, kmode,
                                (const float4 *)add, total, emit, shoot, e0, refl,
                                (double *)B->partials.ptr);
         for (int32_t lv = 0; lv < B->height; lv++) {
@@ -1124,6 +1195,7 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     }
     VH_HIP(hipStreamSynchronize(s));
     if (iters) *iters = it;
+    B->have_total = true;
     return VH_OK;
 }
 

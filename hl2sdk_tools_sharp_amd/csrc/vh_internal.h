@@ -91,6 +91,8 @@ struct vh_ctx {
     struct BounceData *bounce = nullptr;
     // faces, samples and lights of the direct-light gather (vh_gather.hip)
     struct GatherData *gather = nullptr;
+    // the warm-start network: weights, optimiser state, data set (vh_mlp.hip)
+    struct MlpData *mlp = nullptr;
 };
 
 namespace vh {
@@ -129,6 +131,28 @@ struct GatherView {
     bool have_results = false;
 };
 bool gather_view(vh_ctx *c, GatherView *v);
+// The loaded patch system and bounce state, for the warm-start network (vh_bounce.hip).
+struct BounceView {
+    int32_t npatch = 0, nleaves = 0;
+    const float4 *origin_area = nullptr;  // origin xyz, area
+    const float4 *normal_pd = nullptr;    // normal xyz, plane distance
+    const float4 *e0 = nullptr;           // the last vh_bounce's direct light, or null
+    const float4 *direct = nullptr;       // the last vh_patch_lights result, or null
+    const float4 *total = nullptr;        // the last vh_bounce's bounced light, or null
+    const int32_t *leaves = nullptr;      // nleaves leaf patches, ascending
+    const uint8_t *sky = nullptr;         // per patch
+    const int32_t *cluster = nullptr;     // per patch
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of the patch origins
+};
+bool bounce_view(vh_ctx *c, BounceView *v);
+// The device buffer (npatch float4) a predicted start is written into; the
+// prediction counts as present only after bounce_pred_ready.
+float4 *bounce_pred_buffer(vh_ctx *c);
+void bounce_pred_ready(vh_ctx *c);
+// Parents from their children, area weighted, deepest level first.
+void bounce_pull_up(vh_ctx *c, float4 *v, hipStream_t s);
+// Frees the warm-start network's buffers (vh_mlp.hip).
+void mlp_release(vh_ctx *c);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

@@ -1,0 +1,1148 @@
+// The warm-start network on gfx950: a per-patch MLP (25 -> 128 -> 128 -> 128 -> 3,
+// GELU) that predicts a patch's converged bounced light from its position,
+// normal, direct light and Fourier features of its position.  It has no
+// counterpart in the reference; the bounce (vh_bounce.hip) iterates from its
+// prediction to the same residual, so it changes only where the iteration
+// starts.
+//
+// The prediction is max(gain * direct + scale * y, 0): the network's output y
+// is the correction to a start proportional to the patch's own direct light
+// (the gain is the bounced-to-direct ratio of the training maps), in units of
+// `scale`.  The correction is learned in linear light so the squared error
+// weighs patches as the bounce's L1 residual does; the data set keeps each
+// row's direct and bounced light, and the targets (bounced - gain * direct) /
+// scale are formed when a batch is drawn, so gain and scale can change
+// without rebuilding it.
+//
+// Precision: FP32 master weights and Adam state; BF16 copies of the weights,
+// BF16 activations and hidden-layer gradients as MFMA operands; FP32
+// accumulation (v_mfma_f32_16x16x32_bf16), FP32 biases, pre-activations and
+// output.
+//
+// Every GEMM is one kernel (k_gemm): a wave owns one 16 x 16 tile of C and
+// walks K in steps of 32, one MFMA per step.  The 64 lanes hold the operands as
+// the instruction wants them: lane l carries row (l & 15) of A and column
+// (l & 15) of B at the eight K positions 8 * (l >> 4) .. + 7; its four results
+// are rows 4 * (l >> 4) .. + 3 of column (l & 15).  Operands whose K runs along
+// memory load as one 16-byte vector; transposed operands are gathered.  The
+// epilogue is fused per use: bias + GELU + BF16 store (forward), bias (output
+// layer), GELU' x the incoming gradient + BF16 store (backward data), and the
+// weight gradient with the bias gradient as the column sums of the B operand
+// (backward weights).
+//
+// Layout: every layer is padded to MFMA shapes (inputs 25 -> 32, outputs
+// 3 -> 32); padded weights start at zero, see zero gradients and stay zero.
+// A training step (batch gather by an affine permutation of the data set,
+// forward, loss, backward, Adam) is captured once into a hipGraph and replayed;
+// its counters live in device memory so every replay advances by itself.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "vh_internal.h"
+
+#pragma clang fp contract(off)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace vh {
+namespace {
+
+constexpr int kLayers = 4;
+constexpr int kIn = 25, kHidden = 128, kOut = 3, kFourier = 8;
+constexpr int kInPad = 32, kOutPad = 32;
+constexpr int kLight = 6;  // data-set light per row: direct rgb, bounced rgb
+constexpr int kInDim[kLayers] = {kIn, kHidden, kHidden, kHidden};
+constexpr int kOutDim[kLayers] = {kHidden, kHidden, kHidden, kOut};
+constexpr int kInP[kLayers] = {kInPad, kHidden, kHidden, kHidden};
+constexpr int kOutP[kLayers] = {kHidden, kHidden, kHidden, kOutPad};
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+
+struct Layout {
+    int64_t w[kLayers], b[kLayers], wt[kLayers];  // offsets into the padded arrays
+    int64_t nparam;                               // padded FP32 parameters
+    int64_t nweight;                              // padded weights (BF16 copies)
+};
+
+Layout layout()
+{
+    Layout L{};
+    int64_t o = 0, ow = 0;
+    for (int l = 0; l < kLayers; l++) {
+        L.w[l] = o;
+        o += (int64_t)kInP[l] * kOutP[l];
+        L.b[l] = o;
+        o += kOutP[l];
+        L.wt[l] = ow;  // BF16 copies: W [in][out] at w-offset order, W^T at wt
+        ow += (int64_t)kInP[l] * kOutP[l];
+    }
+    L.nparam = o;
+    L.nweight = ow;
+    return L;
+}
+
+}  // namespace
+}  // namespace vh
+
+struct MlpData {
+    vh::Layout L;
+    bool ready = false;
+    float length = 1024.0f;
+    float floor = 1e-7f;  // the direct light enters as log(max(x, 0) + floor)
+    float gain = 1.0f;    // prediction = max(gain * direct + scale * y, 0)
+    float scale = 1.0f;
+    float freq[24];       // the 8 x 3 Fourier frequency matrix, row-major
+    vh::DevBuf master, m, v, grad, wb, wtb;
+    // data set: BF16 features (rows x 32); FP32 direct then bounced light (rows x 6)
+    vh::DevBuf X, T;
+    int64_t rows = 0, cap = 0;
+    // activations of one pass over `act_rows` rows
+    vh::DevBuf h[4], z[3], y, dy, dz[3], xf, tf, loss_part, losses, state, loss1;
+    int64_t act_rows = 0;
+    int64_t step = 0;  // Adam steps taken
+};
+
+namespace vh {
+namespace {
+
+struct GemmArgs {
+    const __bf16 *a;
+    int64_t lda;
+    const __bf16 *b;
+    int64_t ldb;
+    int32_t M, N, K;
+    const float *bias;   // EPI_GELU, EPI_OUT
+    float *zout;         // EPI_GELU (may be null): pre-activation
+    int64_t ldz;
+    __bf16 *hout;        // EPI_GELU, EPI_DGELU
+    int64_t ldh;
+    float *cout;         // EPI_OUT, EPI_STORE
+    int64_t ldc;
+    const float *zin;    // EPI_DGELU: the pre-activation of this layer's input
+    int64_t ldzi;
+    float *colsum;       // EPI_STORE (may be null): column sums of B
+};
+
+enum { EPI_GELU = 0, EPI_OUT = 1, EPI_DGELU = 2, EPI_STORE = 3 };
+
+__device__ __forceinline__ float gelu(float z)
+{
+    float u = kGeluC * (z + ((0.044715f * z) * z) * z);
+    return (0.5f * z) * (1.0f + tanhf(u));
+}
+
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    float u = kGeluC * (z + ((0.044715f * z) * z) * z);
+    float t = tanhf(u);
+    float du = kGeluC * (1.0f + ((3.0f * 0.044715f) * z) * z);
+    return (0.5f * (1.0f + t)) + ((0.5f * z) * (1.0f - t * t)) * du;
+}
+
+// C (M x N) = A (M x K) . B (K x N); M, N multiples of 16, K of 32.
+// A(m, k) = a[m * lda + k] (AT: a[k * lda + m]); B(k, n) = b[n * ldb + k]
+// (BT: b[k * ldb + n]).
+template <bool AT, bool BT, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_n = g.N >> 4;
+    if (tile >= (int64_t)(g.M >> 4) * tiles_n) return;  // whole wave
+    const int64_t m0 = (tile / tiles_n) * 16;
+    const int n0 = (int)(tile % tiles_n) * 16;
+    const int r = lane & 15, kg = (lane >> 4) * 8;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float cs = 0.0f;
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        bf16x8 af, bfr;
+        const int64_t k = k0 + kg;
+        if (!AT) {
+            af = *(const bf16x8 *)(g.a + (m0 + r) * g.lda + k);
+        } else {
+            for (int j = 0; j < 8; j++) af[j] = g.a[(k + j) * g.lda + m0 + r];
+        }
+        if (!BT) {
+            bfr = *(const bf16x8 *)(g.b + (int64_t)(n0 + r) * g.ldb + k);
+        } else {
+            for (int j = 0; j < 8; j++) bfr[j] = g.b[(k + j) * g.ldb + n0 + r];
+        }
+        if (EPI == EPI_STORE && g.colsum) {
+            for (int j = 0; j < 8; j++) cs += (float)bfr[j];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc, 0, 0, 0);
+    }
+    const int col = n0 + r;
+    const int64_t row0 = m0 + (lane >> 4) * 4;
+    for (int i = 0; i < 4; i++) {
+        const int64_t row = row0 + i;
+        float v = acc[i];
+        if (EPI == EPI_GELU) {
+            float zz = v + g.bias[col];
+            if (g.zout) g.zout[row * g.ldz + col] = zz;
+            g.hout[row * g.ldh + col] = (__bf16)gelu(zz);
+        } else if (EPI == EPI_OUT) {
+            g.cout[row * g.ldc + col] = v + g.bias[col];
+        } else if (EPI == EPI_DGELU) {
+            g.hout[row * g.ldh + col] = (__bf16)(v * gelu_grad(g.zin[row * g.ldzi + col]));
+        } else {
+            g.cout[row * g.ldc + col] = v;
+        }
+    }
+    if (EPI == EPI_STORE && g.colsum) {
+        cs += __shfl_xor(cs, 16, 64);
+        cs += __shfl_xor(cs, 32, 64);
+        if (m0 == 0 && lane < 16) g.colsum[col] = cs;
+    }
+}
+
+// Output-layer gradient and loss: d = y - t over the three outputs,
+// dy = 2 d / (3 n) as BF16 (padded columns 0), per-block sums of d^2.
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y, const float *__restrict__ t,
+                                              int64_t n, __bf16 *__restrict__ dy, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (row < n) {
+        const float scale = 2.0f / (3.0f * (float)n);
+        for (int c = 0; c < kOutPad; c++) {
+            float d = 0.0f;
+            if (c < kOut) {
+                d = y[row * kOutPad + c] - t[row * kOut + c];
+                acc += (double)d * (double)d;
+            }
+            dy[row * kOutPad + c] = (__bf16)(d * scale);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// The partial sums in block order: loss = sum / (3 n), at out[*index].
+__global__ void k_loss_reduce(const double *__restrict__ part, int32_t nblk, int64_t n,
+                              float *__restrict__ out, const int64_t *__restrict__ state)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int32_t b = threadIdx.x; b < nblk; b += 256) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[state ? state[4] : 0] = (float)(red[0] / (3.0 * (double)n));
+}
+
+struct AdamArgs {
+    float *w, *m, *v;
+    const float *g;
+    int64_t n;
+    float lr, beta1, beta2, eps;
+    const int64_t *state;  // state[0]: steps taken before this one
+    __bf16 *wb, *wtb;
+    int64_t woff[kLayers], boff[kLayers], wtoff[kLayers];
+    int32_t inp[kLayers], outp[kLayers];
+};
+
+// Adam (Kingma and Ba, 2015, Algorithm 1) per FP32 parameter, then the BF16
+// copies of the weights in both layouts.
+__global__ __launch_bounds__(256) void k_adam(AdamArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const double t = (double)(a.state[0] + 1);
+    const float bc1 = (float)(1.0 - pow((double)a.beta1, t));
+    const float bc2 = (float)(1.0 - pow((double)a.beta2, t));
+    const float gi = a.g[i];
+    const float mi = a.beta1 * a.m[i] + (1.0f - a.beta1) * gi;
+    const float vi = a.beta2 * a.v[i] + (1.0f - a.beta2) * (gi * gi);
+    a.m[i] = mi;
+    a.v[i] = vi;
+    const float w = a.w[i] - a.lr * (mi / bc1) / (sqrtf(vi / bc2) + a.eps);
+    a.w[i] = w;
+    for (int l = 0; l < kLayers; l++) {
+        const int64_t off = i - a.woff[l];
+        if (off >= 0 && off < (int64_t)a.inp[l] * a.outp[l]) {
+            const int64_t k = off / a.outp[l], o = off % a.outp[l];
+            a.wb[a.wtoff[l] + off] = (__bf16)w;
+            a.wtb[a.wtoff[l] + o * a.inp[l] + k] = (__bf16)w;
+        }
+    }
+}
+
+// The BF16 weight copies from the FP32 master (after a load).
+__global__ __launch_bounds__(256) void k_weights(AdamArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const float w = a.w[i];
+    for (int l = 0; l < kLayers; l++) {
+        const int64_t off = i - a.woff[l];
+        if (off >= 0 && off < (int64_t)a.inp[l] * a.outp[l]) {
+            const int64_t k = off / a.outp[l], o = off % a.outp[l];
+            a.wb[a.wtoff[l] + off] = (__bf16)w;
+            a.wtb[a.wtoff[l] + o * a.inp[l] + k] = (__bf16)w;
+        }
+    }
+}
+
+// state: [0] Adam steps, [1] step within the epoch, [2] a, [3] c, [4] loss slot.
+__global__ void k_advance(int64_t *state)
+{
+    state[0] += 1;
+    state[1] += 1;
+    state[4] += 1;
+}
+
+// One batch of `nb` rows: data row (a * (step * nb + i) + c) mod n, a bijection
+// of [0, n) for a coprime to n.
+__global__ __launch_bounds__(256) void k_batch(const __bf16 *__restrict__ X, const float *__restrict__ T,
+                                               int64_t n, const int64_t *__restrict__ state, int32_t nb,
+                                               float gain, float inv_scale, __bf16 *__restrict__ xb,
+                                               float *__restrict__ tb)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    const uint64_t a = (uint64_t)state[2], c = (uint64_t)state[3];
+    const uint64_t pos = (uint64_t)state[1] * (uint64_t)nb + (uint64_t)i;
+    // a, c < n < 2^31 and pos < n: the product fits in 64 bits
+    const uint64_t j = (a * pos + c) % (uint64_t)n;
+    const uint4 *src = (const uint4 *)(X + j * kInPad);
+    uint4 *dst = (uint4 *)(xb + i * kInPad);
+    for (int q = 0; q < kInPad * 2 / 16; q++) dst[q] = src[q];
+    // the target: (bounced - gain * direct) / scale
+    for (int k = 0; k < kOut; k++)
+        tb[i * kOut + k] = (T[j * kLight + kOut + k] - gain * T[j * kLight + k]) * inv_scale;
+}
+
+struct FeatArgs {
+    const int32_t *rows;  // patches of each row (null: row = patch)
+    int64_t n;
+    const float4 *origin_area, *normal_pd, *e0, *total;
+    float lo[3], ext[3];
+    float freq[24];
+    float length, floor;
+    __bf16 *x;       // n x 32
+    float *xf;       // n x 25 (may be null)
+    float *t;        // n x 6: direct then bounced light (may be null)
+};
+
+// The 25 inputs of a patch: 2 (p - lo) / ext - 1, n, log(max(e0, 0) + floor),
+// and sin, cos of 2 pi (F p) / length for the 8 rows of F; for the data set,
+// the patch's direct and bounced light.
+__global__ __launch_bounds__(256) void k_features(FeatArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int32_t p = a.rows ? a.rows[i] : (int32_t)i;
+    const float4 o = a.origin_area[p], nn = a.normal_pd[p], e = a.e0[p];
+    float f[kInPad];
+    const float pp[3] = {o.x, o.y, o.z};
+    for (int k = 0; k < 3; k++) f[k] = 2.0f * (pp[k] - a.lo[k]) / a.ext[k] - 1.0f;
+    f[3] = nn.x;
+    f[4] = nn.y;
+    f[5] = nn.z;
+    f[6] = logf(fmaxf(e.x, 0.0f) + a.floor);
+    f[7] = logf(fmaxf(e.y, 0.0f) + a.floor);
+    f[8] = logf(fmaxf(e.z, 0.0f) + a.floor);
+    for (int q = 0; q < kFourier; q++) {
+        const float d = (a.freq[3 * q] * o.x + a.freq[3 * q + 1] * o.y) + a.freq[3 * q + 2] * o.z;
+        const float ang = 6.283185307179586f * (d / a.length);
+        f[9 + q] = sinf(ang);
+        f[9 + kFourier + q] = cosf(ang);
+    }
+    for (int k = kIn; k < kInPad; k++) f[k] = 0.0f;
+    for (int k = 0; k < kInPad; k++) a.x[i * kInPad + k] = (__bf16)f[k];
+    if (a.xf)
+        for (int k = 0; k < kIn; k++) a.xf[i * kIn + k] = f[k];
+    if (a.t) {
+        const float4 tt = a.total[p];
+        float *r = a.t + i * kLight;
+        r[0] = e.x;
+        r[1] = e.y;
+        r[2] = e.z;
+        r[3] = tt.x;
+        r[4] = tt.y;
+        r[5] = tt.z;
+    }
+}
+
+// Host rows (n x 25 FP32) to BF16 operands (n x 32), padded rows zero.
+__global__ __launch_bounds__(256) void k_pack(const float *__restrict__ xf, int64_t n, int64_t npad,
+                                              __bf16 *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= npad) return;
+    for (int k = 0; k < kInPad; k++) x[i * kInPad + k] = (__bf16)((i < n && k < kIn) ? xf[i * kIn + k] : 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// The predicted bounced light of one cluster's rows (a block per cluster):
+// q = max(gain * e0 + scale * y, 0), then per channel rescaled so that the
+// cluster's sum is the prior's, gain * sum(max(e0, 0)): the network places
+// the light within the cluster and the prior sets how much there is.  A
+// cluster whose q sums to zero takes the prior itself.  Sums run in a fixed
+// order.
+__global__ __launch_bounds__(256) void k_cluster_start(const float *__restrict__ y,
+                                                       const int32_t *__restrict__ rows,
+                                                       const int32_t *__restrict__ groups,
+                                                       const float4 *__restrict__ e0, float gain, float scale,
+                                                       float4 *__restrict__ start)
+{
+    __shared__ double red[6][256];
+    const int32_t a = groups[blockIdx.x], b = groups[blockIdx.x + 1];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int32_t i = a + (int32_t)threadIdx.x; i < b; i += 256) {
+        const float *r = y + (int64_t)i * kOutPad;
+        const float4 e = e0[rows[i]];
+        const float ev[3] = {fmaxf(e.x, 0.0f), fmaxf(e.y, 0.0f), fmaxf(e.z, 0.0f)};
+        for (int k = 0; k < 3; k++) {
+            acc[k] += (double)fmaxf(gain * ev[k] + scale * r[k], 0.0f);
+            acc[3 + k] += (double)(gain * ev[k]);
+        }
+    }
+    for (int k = 0; k < 6; k++) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    float f[3];
+    bool use_prior[3];
+    for (int k = 0; k < 3; k++) {
+        use_prior[k] = !(red[k][0] > 0.0);
+        f[k] = use_prior[k] ? 0.0f : (float)(red[3 + k][0] / red[k][0]);
+    }
+    for (int32_t i = a + (int32_t)threadIdx.x; i < b; i += 256) {
+        const float *r = y + (int64_t)i * kOutPad;
+        const int32_t p = rows[i];
+        const float4 e = e0[p];
+        const float ev[3] = {fmaxf(e.x, 0.0f), fmaxf(e.y, 0.0f), fmaxf(e.z, 0.0f)};
+        float o[3];
+        for (int k = 0; k < 3; k++) {
+            const float q = fmaxf(gain * ev[k] + scale * r[k], 0.0f);
+            o[k] = use_prior[k] ? gain * ev[k] : q * f[k];
+        }
+        start[p] = make_float4(o[0], o[1], o[2], 0.0f);
+    }
+}
+
+inline uint32_t blocks(int64_t n, int per = 256) { return (uint32_t)((n + per - 1) / per); }
+
+// A device buffer that lives for one call.
+struct LocalBuf : DevBuf {
+    LocalBuf() = default;
+    LocalBuf(const LocalBuf &) = delete;
+    LocalBuf &operator=(const LocalBuf &) = delete;
+    ~LocalBuf() { release(); }
+};
+
+template <bool AT, bool BT, int EPI>
+void gemm(const GemmArgs &g, hipStream_t s)
+{
+    int64_t tiles = (int64_t)(g.M / 16) * (g.N / 16);
+    hipLaunchKernelGGL((k_gemm<AT, BT, EPI>), dim3(blocks(tiles, 4)), dim3(256), 0, s, g);
+}
+
+AdamArgs adam_args(MlpData *D)
+{
+    AdamArgs a{};
+    a.w = (float *)D->master.ptr;
+    a.m = (float *)D->m.ptr;
+    a.v = (float *)D->v.ptr;
+    a.g = (const float *)D->grad.ptr;
+    a.n = D->L.nparam;
+    a.wb = (__bf16 *)D->wb.ptr;
+    a.wtb = (__bf16 *)D->wtb.ptr;
+    for (int l = 0; l < kLayers; l++) {
+        a.woff[l] = D->L.w[l];
+        a.boff[l] = D->L.b[l];
+        a.wtoff[l] = D->L.wt[l];
+        a.inp[l] = kInP[l];
+        a.outp[l] = kOutP[l];
+    }
+    return a;
+}
+
+int ensure_acts(MlpData *D, int64_t rows)
+{
+    if (rows <= D->act_rows) return VH_OK;
+    int rc;
+    if ((rc = D->h[0].ensure((size_t)rows * kInPad * 2, "network inputs"))) return rc;
+    for (int l = 1; l < 4; l++)
+        if ((rc = D->h[l].ensure((size_t)rows * kHidden * 2, "activations"))) return rc;
+    for (int l = 0; l < 3; l++) {
+        if ((rc = D->z[l].ensure((size_t)rows * kHidden * 4, "pre-activations"))) return rc;
+        if ((rc = D->dz[l].ensure((size_t)rows * kHidden * 2, "activation gradients"))) return rc;
+    }
+    if ((rc = D->y.ensure((size_t)rows * kOutPad * 4, "outputs")) ||
+        (rc = D->dy.ensure((size_t)rows * kOutPad * 2, "output gradients")) ||
+        (rc = D->tf.ensure((size_t)rows * kOut * 4, "targets")) ||
+        (rc = D->loss_part.ensure((size_t)blocks(rows) * 8 + 8, "loss partials")))
+        return rc;
+    D->act_rows = rows;
+    return VH_OK;
+}
+
+// Forward over `rows` rows of h[0]; keeps the pre-activations when `keep`.
+void forward(MlpData *D, int64_t rows, bool keep, hipStream_t s)
+{
+    const __bf16 *wtb = (const __bf16 *)D->wtb.ptr;
+    const float *mp = (const float *)D->master.ptr;
+    for (int l = 0; l < kLayers; l++) {
+        GemmArgs g{};
+        g.a = (const __bf16 *)D->h[l].ptr;
+        g.lda = kInP[l];
+        g.b = wtb + D->L.wt[l];
+        g.ldb = kInP[l];
+        g.M = (int32_t)rows;
+        g.N = kOutP[l];
+        g.K = kInP[l];
+        g.bias = mp + D->L.b[l];
+        if (l < kLayers - 1) {
+            g.zout = keep ? (float *)D->z[l].ptr : nullptr;
+            g.ldz = kHidden;
+            g.hout = (__bf16 *)D->h[l + 1].ptr;
+            g.ldh = kHidden;
+            gemm<false, false, EPI_GELU>(g, s);
+        } else {
+            g.cout = (float *)D->y.ptr;
+            g.ldc = kOutPad;
+            gemm<false, false, EPI_OUT>(g, s);
+        }
+    }
+}
+
+// Backward from dy over `rows` rows into the padded FP32 gradient.
+void backward(MlpData *D, int64_t rows, hipStream_t s)
+{
+    const __bf16 *wb = (const __bf16 *)D->wb.ptr;
+    float *grad = (float *)D->grad.ptr;
+    for (int l = kLayers - 1; l >= 0; l--) {
+        const __bf16 *dz = l == kLayers - 1 ? (const __bf16 *)D->dy.ptr : (const __bf16 *)D->dz[l].ptr;
+        const int64_t ldd = kOutP[l];
+        GemmArgs w{};  // dW = H^T dZ, db = column sums of dZ
+        w.a = (const __bf16 *)D->h[l].ptr;
+        w.lda = kInP[l];
+        w.b = dz;
+        w.ldb = ldd;
+        w.M = kInP[l];
+        w.N = kOutP[l];
+        w.K = (int32_t)rows;
+        w.cout = grad + D->L.w[l];
+        w.ldc = kOutP[l];
+        w.colsum = grad + D->L.b[l];
+        gemm<true, true, EPI_STORE>(w, s);
+        if (l == 0) break;
+        GemmArgs d{};  // dH = dZ W^T, then dZ of the layer below = dH * GELU'(z)
+        d.a = dz;
+        d.lda = ldd;
+        d.b = wb + D->L.wt[l];
+        d.ldb = kOutP[l];
+        d.M = (int32_t)rows;
+        d.N = kInP[l];
+        d.K = kOutP[l];
+        d.zin = (const float *)D->z[l - 1].ptr;
+        d.ldzi = kHidden;
+        d.hout = (__bf16 *)D->dz[l - 1].ptr;
+        d.ldh = kHidden;
+        gemm<false, false, EPI_DGELU>(d, s);
+    }
+}
+
+void loss(MlpData *D, const float *t, int64_t rows, float *out, const int64_t *state, hipStream_t s)
+{
+    uint32_t nb = blocks(rows);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, (const float *)D->y.ptr, t, rows,
+                       (__bf16 *)D->dy.ptr, (double *)D->loss_part.ptr);
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, (const double *)D->loss_part.ptr,
+                       (int32_t)nb, rows, out, state);
+}
+
+int check_mlp(vh_ctx *c)
+{
+    if (!c->mlp || !c->mlp->ready) {
+        set_error("no network: call vh_mlp_init first");
+        return VH_ESTATE;
+    }
+    return VH_OK;
+}
+
+FeatArgs feat_args(MlpData *D, const BounceView &bv)
+{
+    FeatArgs f{};
+    f.origin_area = bv.origin_area;
+    f.normal_pd = bv.normal_pd;
+    for (int k = 0; k < 3; k++) {
+        f.lo[k] = bv.lo[k];
+        float e = bv.hi[k] - bv.lo[k];
+        f.ext[k] = e > 0.0f ? e : 1.0f;
+    }
+    for (int k = 0; k < 24; k++) f.freq[k] = D->freq[k];
+    f.length = D->length;
+    f.floor = D->floor;
+    return f;
+}
+
+// Leaf patches that are not sky, as rows (host list -> device).
+// Leaf patches that are not sky, as rows.  With `groups`, the rows are
+// ordered by cluster (then patch) and `groups` receives the clusters' row
+// ranges (ngroups + 1 offsets).
+int leaf_rows(vh_ctx *c, const BounceView &bv, DevBuf &out, int64_t *n, hipStream_t s,
+              DevBuf *groups = nullptr, int32_t *ngroups = nullptr)
+{
+    std::vector<int32_t> leaves(bv.nleaves), cl;
+    std::vector<uint8_t> sky(bv.npatch);
+    VH_HIP(hipMemcpyAsync(leaves.data(), bv.leaves, leaves.size() * 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipMemcpyAsync(sky.data(), bv.sky, sky.size(), hipMemcpyDeviceToHost, s));
+    if (groups) {
+        cl.resize(bv.npatch);
+        VH_HIP(hipMemcpyAsync(cl.data(), bv.cluster, cl.size() * 4, hipMemcpyDeviceToHost, s));
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> rows;
+    rows.reserve(leaves.size());
+    for (int32_t p : leaves)
+        if (!sky[p]) rows.push_back(p);
+    std::vector<int32_t> gp(1, 0);
+    if (groups) {
+        std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return cl[a] < cl[b]; });
+        for (size_t i = 1; i <= rows.size(); i++)
+            if (i == rows.size() || cl[rows[i]] != cl[rows[i - 1]]) gp.push_back((int32_t)i);
+        if (rows.empty()) gp.assign(1, 0);
+        *ngroups = (int32_t)gp.size() - 1;
+    }
+    *n = (int64_t)rows.size();
+    int rc = out.ensure(rows.size() * 4 + 16, "patch rows");
+    if (rc) return rc;
+    if (!rows.empty()) VH_HIP(hipMemcpyAsync(out.ptr, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, s));
+    if (groups) {
+        if ((rc = groups->ensure(gp.size() * 4 + 16, "cluster rows"))) return rc;
+        VH_HIP(hipMemcpyAsync(groups->ptr, gp.data(), gp.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+}  // namespace
+
+void mlp_release(vh_ctx *c)
+{
+    MlpData *D = c->mlp;
+    if (!D) return;
+    for (DevBuf *b : {&D->master, &D->m, &D->v, &D->grad, &D->wb, &D->wtb, &D->X, &D->T,
+                      &D->y, &D->dy, &D->xf, &D->tf, &D->loss_part, &D->losses, &D->state, &D->loss1})
+        b->release();
+    for (auto &b : D->h) b.release();
+    for (auto &b : D->z) b.release();
+    for (auto &b : D->dz) b.release();
+    delete D;
+    c->mlp = nullptr;
+}
+
+}  // namespace vh
+
+using namespace vh;
+
+namespace {
+
+// Packed (36,739) <-> padded parameter layouts.
+void pad_params(const Layout &L, const float *in, std::vector<float> &out)
+{
+    out.assign((size_t)L.nparam, 0.0f);
+    int64_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        for (int k = 0; k < kInDim[l]; k++)
+            for (int n = 0; n < kOutDim[l]; n++) out[L.w[l] + (int64_t)k * kOutP[l] + n] = in[o++];
+        for (int n = 0; n < kOutDim[l]; n++) out[L.b[l] + n] = in[o++];
+    }
+}
+
+void unpad_params(const Layout &L, const float *in, float *out)
+{
+    int64_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        for (int k = 0; k < kInDim[l]; k++)
+            for (int n = 0; n < kOutDim[l]; n++) out[o++] = in[L.w[l] + (int64_t)k * kOutP[l] + n];
+        for (int n = 0; n < kOutDim[l]; n++) out[o++] = in[L.b[l] + n];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vh_mlp_init(vh_ctx *c, const float *params, const float *freq, const vh_mlp_opts *o)
+{
+    clear_error();
+    if (!c || !params || !freq || !o || !(o->length > 0.0f) || !(o->floor > 0.0f) ||
+        !(o->scale > 0.0f) || !std::isfinite(o->gain)) {
+        set_error("NULL context, parameters, frequencies or options, or a length, floor or scale "
+                  "that is not positive");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    VH_HIP(hipSetDevice(c->device));
+    // a data set already built stays: new weights can train on it
+    DevBuf X, T;
+    int64_t rows = 0, cap = 0;
+    if (c->mlp) {
+        std::swap(X, c->mlp->X);
+        std::swap(T, c->mlp->T);
+        rows = c->mlp->rows;
+        cap = c->mlp->cap;
+    }
+    mlp_release(c);
+    MlpData *D = new MlpData();
+    c->mlp = D;
+    std::swap(D->X, X);
+    std::swap(D->T, T);
+    D->rows = rows;
+    D->cap = cap;
+    D->L = layout();
+    D->length = o->length;
+    D->floor = o->floor;
+    D->gain = o->gain;
+    D->scale = o->scale;
+    std::memcpy(D->freq, freq, sizeof(D->freq));
+    std::vector<float> w;
+    pad_params(D->L, params, w);
+    const size_t pb = (size_t)D->L.nparam * 4, wb = (size_t)D->L.nweight * 2;
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = D->master.ensure(pb, "weights")) || (rc = D->m.ensure(pb, "Adam moments")) ||
+        (rc = D->v.ensure(pb, "Adam moments")) || (rc = D->grad.ensure(pb, "gradients")) ||
+        (rc = D->wb.ensure(wb, "BF16 weights")) || (rc = D->wtb.ensure(wb, "BF16 weights")) ||
+        (rc = D->state.ensure(64, "training counters")) || (rc = D->loss1.ensure(64, "loss")))
+        return rc;
+    std::vector<float> zero((size_t)D->L.nparam, 0.0f);
+    VH_HIP(hipMemcpyAsync(D->master.ptr, w.data(), pb, hipMemcpyHostToDevice, s));
+    VH_HIP(hipMemcpyAsync(D->m.ptr, zero.data(), pb, hipMemcpyHostToDevice, s));
+    VH_HIP(hipMemcpyAsync(D->v.ptr, zero.data(), pb, hipMemcpyHostToDevice, s));
+    VH_HIP(hipMemcpyAsync(D->grad.ptr, zero.data(), pb, hipMemcpyHostToDevice, s));
+    AdamArgs a = adam_args(D);
+    hipLaunchKernelGGL(k_weights, dim3(blocks(a.n)), dim3(256), 0, s, a);
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipStreamSynchronize(s));
+    D->ready = true;
+    return VH_OK;
+}
+
+int vh_mlp_params(vh_ctx *c, float *out)
+{
+    clear_error();
+    if (!c || !out) {
+        set_error("NULL context or output");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    std::vector<float> w((size_t)D->L.nparam);
+    VH_HIP(hipMemcpyAsync(w.data(), D->master.ptr, w.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    VH_HIP(hipStreamSynchronize(c->stream));
+    unpad_params(D->L, w.data(), out);
+    return VH_OK;
+}
+
+int vh_mlp_forward(vh_ctx *c, const float *x, int64_t n, float *y)
+{
+    clear_error();
+    if (!c || n < 0 || (n > 0 && (!x || !y))) {
+        set_error("bad forward arguments");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    if (n == 0) return VH_OK;
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    hipStream_t s = c->stream;
+    const int64_t npad = (n + 15) / 16 * 16;
+    if ((rc = ensure_acts(D, npad)) || (rc = D->xf.ensure((size_t)n * kIn * 4, "inputs"))) return rc;
+    VH_HIP(hipMemcpyAsync(D->xf.ptr, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pack, dim3(blocks(npad)), dim3(256), 0, s, (const float *)D->xf.ptr, n, npad,
+                       (__bf16 *)D->h[0].ptr);
+    {
+        KernelTimer timer(c, VH_KERNEL_MLP, s);
+        forward(D, npad, false, s);
+        timer.done();
+    }
+    VH_HIP(hipGetLastError());
+    std::vector<float> buf((size_t)n * kOutPad);
+    VH_HIP(hipMemcpyAsync(buf.data(), D->y.ptr, buf.size() * 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < kOut; k++) y[i * kOut + k] = buf[(size_t)i * kOutPad + k];
+    return VH_OK;
+}
+
+int vh_mlp_grad(vh_ctx *c, const float *x, const float *t, int64_t n, float *grad, float *loss_out)
+{
+    clear_error();
+    if (!c || !x || !t || !grad || n <= 0 || n % 32 != 0) {
+        set_error("bad gradient arguments: n must be a positive multiple of 32");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    hipStream_t s = c->stream;
+    if ((rc = ensure_acts(D, n)) || (rc = D->xf.ensure((size_t)n * kIn * 4, "inputs"))) return rc;
+    VH_HIP(hipMemcpyAsync(D->xf.ptr, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice, s));
+    VH_HIP(hipMemcpyAsync(D->tf.ptr, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pack, dim3(blocks(n)), dim3(256), 0, s, (const float *)D->xf.ptr, n, n,
+                       (__bf16 *)D->h[0].ptr);
+    forward(D, n, true, s);
+    loss(D, (const float *)D->tf.ptr, n, (float *)D->loss1.ptr, nullptr, s);
+    backward(D, n, s);
+    VH_HIP(hipGetLastError());
+    std::vector<float> g((size_t)D->L.nparam);
+    float l = 0.0f;
+    VH_HIP(hipMemcpyAsync(g.data(), D->grad.ptr, g.size() * 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipMemcpyAsync(&l, D->loss1.ptr, 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    unpad_params(D->L, g.data(), grad);
+    if (loss_out) *loss_out = l;
+    return VH_OK;
+}
+
+int vh_mlp_adam(vh_ctx *c, const float *grad, float lr, float beta1, float beta2, float eps)
+{
+    clear_error();
+    if (!c || !grad) {
+        set_error("NULL context or gradient");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    hipStream_t s = c->stream;
+    std::vector<float> g;
+    pad_params(D->L, grad, g);
+    int64_t st[5] = {D->step, 0, 0, 0, 0};
+    VH_HIP(hipMemcpyAsync(D->grad.ptr, g.data(), g.size() * 4, hipMemcpyHostToDevice, s));
+    VH_HIP(hipMemcpyAsync(D->state.ptr, st, sizeof(st), hipMemcpyHostToDevice, s));
+    AdamArgs a = adam_args(D);
+    a.lr = lr;
+    a.beta1 = beta1;
+    a.beta2 = beta2;
+    a.eps = eps;
+    a.state = (const int64_t *)D->state.ptr;
+    hipLaunchKernelGGL(k_adam, dim3(blocks(a.n)), dim3(256), 0, s, a);
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipStreamSynchronize(s));
+    D->step += 1;
+    return VH_OK;
+}
+
+int vh_mlp_features(vh_ctx *c, const float *emit0, float *x)
+{
+    clear_error();
+    if (!c || !x) {
+        set_error("NULL context or output");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    BounceView bv;
+    if (!bounce_view(c, &bv)) {
+        set_error("no patches loaded: call vh_patches_load first");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    hipStream_t s = c->stream;
+    const int64_t P = bv.npatch;
+    LocalBuf e;
+    if ((rc = ensure_acts(D, (P + 15) / 16 * 16)) || (rc = D->xf.ensure((size_t)P * kIn * 4, "features")))
+        return rc;
+    FeatArgs f = feat_args(D, bv);
+    if (emit0) {
+        std::vector<float> buf((size_t)P * 4, 0.0f);
+        for (int64_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
+        if ((rc = e.ensure(buf.size() * 4, "emission"))) return rc;
+        VH_HIP(hipMemcpyAsync(e.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+        f.e0 = (const float4 *)e.ptr;
+    } else if (bv.direct) {
+        f.e0 = bv.direct;
+    } else {
+        set_error("NULL emission and no vh_patch_lights result");
+        return VH_ESTATE;
+    }
+    f.n = P;
+    f.x = (__bf16 *)D->h[0].ptr;
+    f.xf = (float *)D->xf.ptr;
+    hipLaunchKernelGGL(k_features, dim3(blocks(P)), dim3(256), 0, s, f);
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipMemcpyAsync(x, D->xf.ptr, (size_t)P * kIn * 4, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+int vh_mlp_dataset(vh_ctx *c, int32_t op, int64_t *rows)
+{
+    clear_error();
+    if (!c || (op != VH_MLP_DATA_CLEAR && op != VH_MLP_DATA_ADD)) {
+        set_error("NULL context or bad data-set operation");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    MlpData *D = c->mlp;
+    if (op == VH_MLP_DATA_CLEAR) {
+        D->rows = 0;
+        if (rows) *rows = 0;
+        return VH_OK;
+    }
+    BounceView bv;
+    if (!bounce_view(c, &bv) || !bv.e0 || !bv.total) {
+        set_error("no finished bounce: call vh_bounce first (its emission and light become the row)");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    LocalBuf idx;
+    int64_t n = 0;
+    if ((rc = leaf_rows(c, bv, idx, &n, s))) return rc;
+    const int64_t need = D->rows + n;
+    if (need >= ((int64_t)1 << 31)) {
+        set_error("the data set would pass 2^31 rows");
+        return VH_EINVAL;
+    }
+    if (need > D->cap) {
+        // grow by half again, keeping the rows already held
+        int64_t cap = std::max<int64_t>(need, D->cap + D->cap / 2);
+        LocalBuf X2, T2;
+        if ((rc = X2.ensure((size_t)cap * kInPad * 2, "data set")) ||
+            (rc = T2.ensure((size_t)cap * kLight * 4, "data set")))
+            return rc;
+        if (D->rows) {
+            VH_HIP(hipMemcpyAsync(X2.ptr, D->X.ptr, (size_t)D->rows * kInPad * 2, hipMemcpyDeviceToDevice, s));
+            VH_HIP(hipMemcpyAsync(T2.ptr, D->T.ptr, (size_t)D->rows * kLight * 4, hipMemcpyDeviceToDevice, s));
+        }
+        VH_HIP(hipStreamSynchronize(s));
+        D->X.release();
+        D->T.release();
+        std::swap(D->X.ptr, X2.ptr);
+        std::swap(D->X.bytes, X2.bytes);
+        std::swap(D->T.ptr, T2.ptr);
+        std::swap(D->T.bytes, T2.bytes);
+        D->cap = cap;
+    }
+    if (n > 0) {
+        FeatArgs f = feat_args(D, bv);
+        f.rows = (const int32_t *)idx.ptr;
+        f.n = n;
+        f.e0 = bv.e0;
+        f.total = bv.total;
+        f.x = (__bf16 *)D->X.ptr + D->rows * kInPad;
+        f.t = (float *)D->T.ptr + D->rows * kLight;
+        hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, s, f);
+        VH_HIP(hipGetLastError());
+        VH_HIP(hipStreamSynchronize(s));
+    }
+    D->rows = need;
+    if (rows) *rows = need;
+    return VH_OK;
+}
+
+int vh_mlp_train(vh_ctx *c, const vh_mlp_train_opts *o, float *epoch_loss)
+{
+    clear_error();
+    if (!c || !o || o->epochs < 1 || o->batch < 32 || o->batch % 32 != 0 || !(o->lr > 0.0f)) {
+        set_error("bad training options: epochs >= 1, batch a positive multiple of 32, lr > 0");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    MlpData *D = c->mlp;
+    const int64_t nb = o->batch, n = D->rows;
+    const int64_t spe = n / nb;
+    if (spe < 1) {
+        set_error("the data set holds " + std::to_string(n) + " rows, fewer than one batch");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t total = spe * o->epochs;
+    if ((rc = ensure_acts(D, nb)) || (rc = D->losses.ensure((size_t)total * 4 + 16, "losses"))) return rc;
+    AdamArgs aa = adam_args(D);
+    aa.lr = o->lr;
+    aa.beta1 = o->beta1;
+    aa.beta2 = o->beta2;
+    aa.eps = o->eps;
+    aa.state = (const int64_t *)D->state.ptr;
+    int64_t *state = (int64_t *)D->state.ptr;
+    auto step = [&]() {
+        hipLaunchKernelGGL(k_batch, dim3(blocks(nb)), dim3(256), 0, s, (const __bf16 *)D->X.ptr,
+                           (const float *)D->T.ptr, n, (const int64_t *)state, (int32_t)nb, D->gain,
+                           1.0f / D->scale, (__bf16 *)D->h[0].ptr, (float *)D->tf.ptr);
+        forward(D, nb, true, s);
+        loss(D, (const float *)D->tf.ptr, nb, (float *)D->losses.ptr, state, s);
+        backward(D, nb, s);
+        hipLaunchKernelGGL(k_adam, dim3(blocks(aa.n)), dim3(256), 0, s, aa);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, state);
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (o->use_graph) {
+        VH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        step();
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            set_error(std::string("training-step graph failed: ") + hipGetErrorString(e));
+            return VH_EDEVICE;
+        }
+    }
+    std::mt19937_64 rng((uint64_t)o->seed * 0x9E3779B97F4A7C15ull + 1);
+    hipError_t failure = hipSuccess;
+    std::vector<int64_t> st(5);
+    for (int32_t ep = 0; ep < o->epochs && failure == hipSuccess; ep++) {
+        // a multiplier coprime to n and an offset: a fresh permutation of the rows
+        uint64_t a;
+        do {
+            a = rng() % (uint64_t)n;
+        } while (a == 0 || std::gcd(a, (uint64_t)n) != 1);
+        if (n == 1) a = 1;
+        st = {D->step + (int64_t)ep * spe, 0, (int64_t)a, (int64_t)(rng() % (uint64_t)n), (int64_t)ep * spe};
+        failure = hipMemcpyAsync(state, st.data(), st.size() * 8, hipMemcpyHostToDevice, s);
+        if (failure == hipSuccess) failure = hipStreamSynchronize(s);
+        KernelTimer timer(c, VH_KERNEL_MLP, s);
+        for (int64_t k = 0; k < spe && failure == hipSuccess; k++) {
+            if (exec) {
+                failure = hipGraphLaunch(exec, s);
+            } else {
+                step();
+                failure = hipGetLastError();
+            }
+        }
+        timer.done();
+    }
+    if (failure == hipSuccess) failure = hipStreamSynchronize(s);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (failure != hipSuccess) {
+        set_error(std::string("training failed: ") + hipGetErrorString(failure));
+        return VH_EDEVICE;
+    }
+    D->step += total;
+    if (epoch_loss) {
+        std::vector<float> l((size_t)total);
+        VH_HIP(hipMemcpy(l.data(), D->losses.ptr, l.size() * 4, hipMemcpyDeviceToHost));
+        for (int32_t ep = 0; ep < o->epochs; ep++) {
+            double sum = 0.0;
+            for (int64_t k = 0; k < spe; k++) sum += l[(size_t)(ep * spe + k)];
+            epoch_loss[ep] = (float)(sum / (double)spe);
+        }
+    }
+    return VH_OK;
+}
+
+int vh_mlp_predict(vh_ctx *c, const float *emit0, float *start)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_mlp(c);
+    if (rc) return rc;
+    BounceView bv;
+    if (!bounce_view(c, &bv)) {
+        set_error("no patches loaded: call vh_patches_load first");
+        return VH_ESTATE;
+    }
+    if (!emit0 && !bv.direct) {
+        set_error("NULL emission and no vh_patch_lights result");
+        return VH_ESTATE;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    MlpData *D = c->mlp;
+    hipStream_t s = c->stream;
+    const int64_t P = bv.npatch;
+    LocalBuf idx, groups, e;
+    int64_t n = 0;
+    int32_t ngroups = 0;
+    if ((rc = leaf_rows(c, bv, idx, &n, s, &groups, &ngroups))) return rc;
+    const int64_t npad = (n + 15) / 16 * 16;
+    if ((rc = ensure_acts(D, npad > 0 ? npad : 16))) return rc;
+    FeatArgs f = feat_args(D, bv);
+    if (emit0) {
+        std::vector<float> buf((size_t)P * 4, 0.0f);
+        for (int64_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
+        if ((rc = e.ensure(buf.size() * 4, "emission"))) return rc;
+        VH_HIP(hipMemcpyAsync(e.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
+        VH_HIP(hipStreamSynchronize(s));
+        f.e0 = (const float4 *)e.ptr;
+    } else {
+        f.e0 = bv.direct;
+    }
+    float4 *pred = bounce_pred_buffer(c);
+    if (!pred) return VH_EDEVICE;
+    hipLaunchKernelGGL(k_zero4, dim3(blocks(P)), dim3(256), 0, s, pred, P);
+    if (n > 0) {
+        // rows past n (padding) are zero inputs; their outputs are not read
+        hipLaunchKernelGGL(k_pack, dim3(blocks(npad)), dim3(256), 0, s, (const float *)nullptr, (int64_t)0,
+                           npad, (__bf16 *)D->h[0].ptr);
+        f.rows = (const int32_t *)idx.ptr;
+        f.n = n;
+        f.x = (__bf16 *)D->h[0].ptr;
+        hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, s, f);
+        {
+            KernelTimer timer(c, VH_KERNEL_MLP, s);
+            forward(D, npad, false, s);
+            timer.done();
+        }
+        hipLaunchKernelGGL(k_cluster_start, dim3((uint32_t)ngroups), dim3(256), 0, s, (const float *)D->y.ptr,
+                           (const int32_t *)idx.ptr, (const int32_t *)groups.ptr, f.e0, D->gain, D->scale,
+                           pred);
+    }
+    bounce_pull_up(c, pred, s);
+    VH_HIP(hipGetLastError());
+    if (start) {
+        std::vector<float> buf((size_t)P * 4);
+        VH_HIP(hipMemcpyAsync(buf.data(), pred, buf.size() * 4, hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        for (int64_t p = 0; p < P; p++)
+            for (int k = 0; k < 3; k++) start[3 * p + k] = buf[4 * p + k];
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    bounce_pred_ready(c);
+    return VH_OK;
+}
+
+}  // extern "C"

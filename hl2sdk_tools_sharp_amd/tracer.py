@@ -156,6 +156,15 @@ class HipRayTracer:
             _lib.lib().vh_supports(self._h, float(options.min_distance), skip, int(options.sky_does_not_block))
         )
 
+    def load_scene(self, casters: ShadowCasterSet) -> None:
+        """Replaces the scene on this context (a new map on the same device
+        state: the warm-start network and its data set stay)."""
+        self._alive()
+        st = _lib.lib().vh_scene_load(self._h, _ptr(casters.vertices), _ptr(casters.ids), _ptr(casters.flags),
+                                      casters.count)
+        self._raise(st)
+        self._triangle_count = casters.count
+
     @property
     def handle(self) -> ctypes.c_void_p:
         self._alive()

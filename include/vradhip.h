@@ -197,6 +197,7 @@ int vh_transfers_load(vh_ctx *ctx, const int64_t *row_ptr, const int32_t *src,
 #define VH_WEIGHTS_F16 1 /* FP16 weights with a per-row FP32 scale, FP32 accumulation */
 #define VH_BOUNCE_PROGRESSIVE 0 /* Radiosity.BounceAsync: emit = the last bounce's light */
 #define VH_BOUNCE_FROM_START 1  /* start from an estimate of the bounced light */
+#define VH_BOUNCE_FROM_PREDICTION 2 /* from a start: the device-resident vh_mlp_predict result */
 
 typedef struct vh_bounce_opts {
     int32_t weights;   /* VH_WEIGHTS_* */
@@ -318,6 +319,72 @@ int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacit
  * patch, may be NULL) receives a copy. */
 int vh_patch_lights(vh_ctx *ctx, int32_t which, const float *sample_area, float *direct);
 
+/* ---- The warm-start network (no counterpart in the reference) ----
+ *
+ * A per-patch MLP, 25 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form) on
+ * the hidden layers, that predicts a patch's converged bounced light; the
+ * bounce then iterates from the prediction (VH_BOUNCE_FROM_PREDICTION) to the
+ * same residual, so the network changes where the iteration starts, never
+ * what it converges to.  Inputs of patch p: 2 (origin - lo) / (hi - lo) - 1
+ * over the bounds of the loaded patch origins (3), the plane normal (3),
+ * log(max(direct light, 0) + floor) (3), and sin then cos of
+ * 2 pi (F origin) / length for the 8 rows of the frequency matrix F (16).
+ * With output y the prediction is max(gain * direct light + scale * y, 0):
+ * the network learns, in linear light, the correction to a start
+ * proportional to the patch's own direct light.  Parameters are FP32, packed per
+ * layer as W [in][out] row-major then b: 36,739 floats.  GEMMs run on BF16
+ * MFMA with FP32 accumulation; Adam keeps FP32 moments. */
+#define VH_MLP_PARAMS 36739
+#define VH_MLP_INPUTS 25
+
+typedef struct vh_mlp_opts {
+    float length; /* the Fourier features' length scale (> 0) */
+    float floor;  /* the direct-light input is log(max(x, 0) + floor) (> 0) */
+    float gain;   /* prediction = max(gain * direct + scale * y, 0) */
+    float scale;  /* (> 0) */
+} vh_mlp_opts;
+/* Sets the parameters (VH_MLP_PARAMS floats), the 8 x 3 frequency matrix and
+ * the options, and clears the optimiser state; a data set already built
+ * (vh_mlp_dataset) is kept. */
+int vh_mlp_init(vh_ctx *ctx, const float *params, const float *freq, const vh_mlp_opts *opts);
+/* Reads the FP32 parameters (VH_MLP_PARAMS floats). */
+int vh_mlp_params(vh_ctx *ctx, float *out);
+/* The raw network output y for n host rows of VH_MLP_INPUTS. */
+int vh_mlp_forward(vh_ctx *ctx, const float *x, int64_t n, float *y);
+/* Mean squared error over n rows x 3 outputs against t (n x 3) and its
+ * gradient (VH_MLP_PARAMS floats); n a positive multiple of 32. */
+int vh_mlp_grad(vh_ctx *ctx, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* One Adam step with the given gradient (VH_MLP_PARAMS floats). */
+int vh_mlp_adam(vh_ctx *ctx, const float *grad, float lr, float beta1, float beta2, float eps);
+/* The inputs of every loaded patch (npatch x VH_MLP_INPUTS) from emit0
+ * (3 per patch; NULL = the last vh_patch_lights result). */
+int vh_mlp_features(vh_ctx *ctx, const float *emit0, float *x);
+
+#define VH_MLP_DATA_CLEAR 0
+#define VH_MLP_DATA_ADD 1 /* the last vh_bounce's leaf, non-sky patches: inputs from its
+                             direct light; targets (bounced - gain * direct) / scale,
+                             formed per batch from the options of vh_mlp_init */
+/* The device-resident training set; rows (may be NULL) receives its size. */
+int vh_mlp_dataset(vh_ctx *ctx, int32_t op, int64_t *rows);
+
+typedef struct vh_mlp_train_opts {
+    int32_t epochs;
+    int32_t batch;     /* rows per step, a multiple of 32 */
+    int32_t seed;      /* the per-epoch row permutations */
+    int32_t use_graph; /* replay one captured training step (hipGraph) */
+    float lr, beta1, beta2, eps;
+} vh_mlp_train_opts;
+/* Trains on the data set: per epoch a fresh permutation of the rows, whole
+ * batches only; epoch_loss (epochs floats, may be NULL) receives each
+ * epoch's mean batch loss. */
+int vh_mlp_train(vh_ctx *ctx, const vh_mlp_train_opts *opts, float *epoch_loss);
+/* Predicts the bounced light of the loaded patches (leaves that are not sky;
+ * parents pulled up) from emit0 (3 per patch; NULL = the last
+ * vh_patch_lights result) and keeps it on the device for the next
+ * VH_BOUNCE_FROM_PREDICTION bounce; start (3 per patch, may be NULL)
+ * receives a copy. */
+int vh_mlp_predict(vh_ctx *ctx, const float *emit0, float *start);
+
 /* Kernel timing with HIP events recorded on the launching stream around each
  * launch of the named kernel (no counterpart in the reference: the bench and
  * the profiler read it).  Enable, run, then read: vh_timing_read waits for the
@@ -329,6 +396,7 @@ int vh_patch_lights(vh_ctx *ctx, int32_t which, const float *sample_area, float 
 #define VH_KERNEL_GATHER 3
 #define VH_KERNEL_TRANSFERS 4
 #define VH_KERNEL_FACELIGHT 5
+#define VH_KERNEL_MLP 6 /* the warm-start network's GEMMs (inference) or training steps */
 #define VH_KERNEL_COUNT 8
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
