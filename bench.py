@@ -283,6 +283,96 @@ def plain_vrad_leg(a, log):
     return out
 
 
+def _grid_for_seed(seed: int):
+    """The warm-start maps: the bounce config's rooms and patches on an
+    n x n grid, n = 6 + seed mod 7 (50k-200k patches)."""
+    from hl2sdk_tools_sharp_amd import rooms
+
+    return rooms.bounce_grid(6 + seed % 7, 6 + seed % 7, seed=seed)
+
+
+def warm_start_leg(a, log):
+    """The learned warm start: bounces of seeded maps (the bounce config's
+    generator, seeds 100.., 50k-200k patches) become the network's data set
+    on the GPU; the network trains there (BF16 MFMA, hipGraph steps, Adam);
+    on held-out maps (seeds 200..) the bounce to residual 1e-4 runs from a
+    zero start, from the direct-only prior and from the network, with FP32
+    and FP16 transfer weights.  Maps are generated on the host in worker
+    processes."""
+    import multiprocessing as mp
+
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, _lib
+    from hl2sdk_tools_sharp_amd.mlp import Prior, WarmStartMLP, bounce_map, init_params
+
+    train_seeds = list(range(100, 100 + a.mlp_maps))
+    test_seeds = list(range(200, 200 + a.mlp_test_maps))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
+        maps = pool.map(_grid_for_seed, train_seeds + test_seeds)
+    gen_s = time.perf_counter() - t0
+    train, test = maps[:len(train_seeds)], maps[len(train_seeds):]
+    log(f"[bench] warm start: {len(train)} training maps ({sum(g.npatch for g in train)} patches), "
+        f"{len(test)} held out, generated in {gen_s:.1f} s")
+    offer = HipTracerFactory(device=0).try_create(train[0].casters)
+    if offer.tracer is None:
+        raise RuntimeError(f"HIP tracer declined: {offer.decline_reason}")
+    tr = offer.tracer
+    net = WarmStartMLP(tr)
+    net.dataset_clear()
+    prior = Prior()
+    t0 = time.perf_counter()
+    rows = 0
+    for g in train:
+        _st, e0, r = bounce_map(tr, g, tol=1e-6)
+        rows = net.dataset_add()
+        prior.add(g, e0, r.total_light)
+    data_s = time.perf_counter() - t0
+    net = WarmStartMLP(tr, params=init_params(0), gain=prior.gain, scale=prior.scale)
+    L = _lib.lib()
+    _lib.check(L.vh_timing_enable(tr.handle, 1))
+    t0 = time.perf_counter()
+    losses = net.train(epochs=a.mlp_epochs, batch=a.mlp_batch, lr=a.mlp_lr, seed=1)
+    train_s = time.perf_counter() - t0
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(L.vh_timing_read(tr.handle, _lib.VH_KERNEL_MLP, ctypes.byref(ms), ctypes.byref(n)))
+    _lib.check(L.vh_timing_enable(tr.handle, 0))
+    steps = (rows // a.mlp_batch) * a.mlp_epochs
+    flop = 6.0 * 36352 * a.mlp_batch * steps  # 2 forward + 4 backward FLOP per weight per row
+    log(f"[bench] warm start: {rows} rows from {len(train)} maps in {data_s:.1f} s; {steps} steps in "
+        f"{train_s:.2f} s ({flop / (ms.value / 1e3) / 1e12:.1f} TFLOP/s over the training launches); "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    out = {"train_maps": len(train), "test_maps": len(test), "rows": rows, "epochs": a.mlp_epochs,
+           "batch": a.mlp_batch, "train_s": train_s, "train_tflops": flop / (ms.value / 1e3) / 1e12,
+           "data_s": data_s, "generate_s": gen_s, "prior_gain": prior.gain, "prior_scale": prior.scale,
+           "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "maps": []}
+    for g in test:
+        st, e0, _r = bounce_map(tr, g, tol=1e-4, max_iter=2)
+        leaf = (g.leaf & ~g.sky.astype(bool))[:, None]
+        res = {"patches": g.npatch}
+        ref = st.bounce(e0, tol=1e-7, max_iter=300).total_light.astype(np.float64)
+        for wname in ("f32", "f16"):
+            for start in ("zero", "direct", "network"):
+                t0 = time.perf_counter()
+                if start == "network":
+                    net.predict(e0, read=False)
+                    r = st.bounce(e0, tol=1e-4, max_iter=300, weights=wname, from_prediction=True)
+                else:
+                    x0 = np.zeros_like(e0) if start == "zero" else (prior.gain * e0 * leaf).astype(np.float32)
+                    r = st.bounce(e0, start=x0, tol=1e-4, max_iter=300, weights=wname)
+                wall = time.perf_counter() - t0
+                err = float(np.abs(r.total_light - ref).sum() / np.abs(ref).sum())
+                res[f"{wname}_{start}"] = {"iterations": r.iterations, "wall_s": wall, "rel_l1_vs_tight": err}
+        out["maps"].append(res)
+        log(f"[bench] held-out map {g.npatch} patches: iterations f32 zero/direct/network "
+            f"{res['f32_zero']['iterations']}/{res['f32_direct']['iterations']}/{res['f32_network']['iterations']}, "
+            f"f16 {res['f16_zero']['iterations']}/{res['f16_direct']['iterations']}/{res['f16_network']['iterations']}")
+    for key in ("f32_zero", "f32_direct", "f32_network", "f16_zero", "f16_direct", "f16_network"):
+        out[f"mean_iterations_{key}"] = float(np.mean([m[key]["iterations"] for m in out["maps"]]))
+        out[f"mean_wall_s_{key}"] = float(np.mean([m[key]["wall_s"] for m in out["maps"]]))
+    tr.dispose()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +388,12 @@ def main(argv=None) -> int:
     ap.add_argument("--bounce-timed-iters", type=int, default=10)
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--mlp-maps", type=int, default=64)
+    ap.add_argument("--mlp-test-maps", type=int, default=8)
+    ap.add_argument("--mlp-epochs", type=int, default=8)
+    ap.add_argument("--mlp-batch", type=int, default=4096)
+    ap.add_argument("--mlp-lr", type=float, default=2e-3)
     ap.add_argument("--batch-faces", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-cpu-fraction", type=float, default=0.1)
@@ -423,6 +519,7 @@ def main(argv=None) -> int:
     bounce = None if a.no_bounce else bounce_leg(a, log)
     batching = None if a.no_batching else batching_leg(a, log)
     plain = None if a.no_plain else plain_vrad_leg(a, log)
+    warm = None if a.no_warm_start else warm_start_leg(a, log)
 
     out = {
         "metric": METRIC,
@@ -451,6 +548,7 @@ def main(argv=None) -> int:
         "bounce": bounce,
         "batching": batching,
         "plain_vrad": plain,
+        "warm_start": warm,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -46,6 +47,7 @@ struct BounceData {
     bool have_direct = false;
     // the last bounce's emission and totals are valid; a predicted start
     bool have_e0 = false, have_total = false, have_pred = false;
+    uint64_t generation = 0;  // distinct for every vh_patches_load in the process
     vh::DevBuf pred;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of the patch origins
     double *h_result = nullptr;  // pinned
@@ -654,6 +656,7 @@ bool bounce_view(vh_ctx *c, BounceView *v)
     v->leaves = (const int32_t *)B->leaves.ptr;
     v->sky = (const uint8_t *)B->sky.ptr;
     v->cluster = (const int32_t *)B->cluster.ptr;
+    v->generation = B->generation;
     for (int k = 0; k < 3; k++) {
         v->lo[k] = B->lo[k];
         v->hi[k] = B->hi[k];
@@ -758,6 +761,8 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
 
     bounce_release(c);
     BounceData *B = new BounceData();
+    static std::atomic<uint64_t> generation{0};
+    B->generation = ++generation;
     c->bounce = B;
     B->npatch = P;
     B->nface = F;

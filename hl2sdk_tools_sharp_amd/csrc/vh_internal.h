@@ -142,6 +142,7 @@ struct BounceView {
     const int32_t *leaves = nullptr;      // nleaves leaf patches, ascending
     const uint8_t *sky = nullptr;         // per patch
     const int32_t *cluster = nullptr;     // per patch
+    uint64_t generation = 0;              // changes with every vh_patches_load
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of the patch origins
 };
 bool bounce_view(vh_ctx *c, BounceView *v);

@@ -105,6 +105,11 @@ struct MlpData {
     vh::DevBuf h[4], z[3], y, dy, dz[3], xf, tf, loss_part, losses, state, loss1;
     int64_t act_rows = 0;
     int64_t step = 0;  // Adam steps taken
+    // the prediction's rows (leaf, non-sky patches by cluster) for one patch system
+    vh::DevBuf pred_rows, pred_groups;
+    int64_t pred_n = 0;
+    int32_t pred_ngroups = 0;
+    uint64_t pred_generation = 0;
 };
 
 namespace vh {
@@ -649,7 +654,8 @@ void mlp_release(vh_ctx *c)
     MlpData *D = c->mlp;
     if (!D) return;
     for (DevBuf *b : {&D->master, &D->m, &D->v, &D->grad, &D->wb, &D->wtb, &D->X, &D->T,
-                      &D->y, &D->dy, &D->xf, &D->tf, &D->loss_part, &D->losses, &D->state, &D->loss1})
+                      &D->y, &D->dy, &D->xf, &D->tf, &D->loss_part, &D->losses, &D->state, &D->loss1,
+                      &D->pred_rows, &D->pred_groups})
         b->release();
     for (auto &b : D->h) b.release();
     for (auto &b : D->z) b.release();
@@ -1093,10 +1099,17 @@ int vh_mlp_predict(vh_ctx *c, const float *emit0, float *start)
     MlpData *D = c->mlp;
     hipStream_t s = c->stream;
     const int64_t P = bv.npatch;
-    LocalBuf idx, groups, e;
-    int64_t n = 0;
-    int32_t ngroups = 0;
-    if ((rc = leaf_rows(c, bv, idx, &n, s, &groups, &ngroups))) return rc;
+    LocalBuf e;
+    if (D->pred_generation != bv.generation) {
+        // once per patch system: the rows, grouped by cluster
+        D->pred_generation = 0;
+        if ((rc = leaf_rows(c, bv, D->pred_rows, &D->pred_n, s, &D->pred_groups, &D->pred_ngroups)))
+            return rc;
+        D->pred_generation = bv.generation;
+    }
+    const DevBuf &idx = D->pred_rows, &groups = D->pred_groups;
+    const int64_t n = D->pred_n;
+    const int32_t ngroups = D->pred_ngroups;
     const int64_t npad = (n + 15) / 16 * 16;
     if ((rc = ensure_acts(D, npad > 0 ? npad : 16))) return rc;
     FeatArgs f = feat_args(D, bv);

@@ -389,11 +389,13 @@ def direct_light(g: RoomGrid, blocked: np.ndarray) -> np.ndarray:
     L = g.lights.astype(np.float64)
     blocked = np.asarray(blocked, bool).reshape(g.npatch, len(L))
     e = np.zeros(g.npatch, np.float64)
-    for l in range(len(L)):
-        d = L[l] - pts
-        dist = np.sqrt((d * d).sum(1))
-        dot = np.maximum(0.0, (d * g.normal).sum(1) / np.maximum(dist, 1e-9))
-        e += np.where(blocked[:, l], 0.0, g.light_intensity[l] * dot / np.maximum(dist, 1.0) ** 2)
+    # only the unblocked (light, patch) pairs, light-major, so each patch
+    # sums its lights in ascending order
+    lk, pk = np.nonzero(~blocked.T)
+    d = L[lk] - pts[pk]
+    dist = np.sqrt((d * d).sum(1))
+    dot = np.maximum(0.0, (d * g.normal[pk]).sum(1) / np.maximum(dist, 1e-9))
+    np.add.at(e, pk, g.light_intensity[lk] * dot / np.maximum(dist, 1.0) ** 2)
     e = np.where(g.leaf & ~g.sky.astype(bool), e, 0.0)
     e3 = np.repeat(e[:, None], 3, axis=1).astype(np.float32)
     return g.pull_up(e3)
