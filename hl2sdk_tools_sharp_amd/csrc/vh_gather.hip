@@ -141,28 +141,26 @@ __device__ __forceinline__ Ray segment(float px, float py, float pz, float ex, f
     return r;
 }
 
-__device__ __forceinline__ bool blocked(const GatherArgs &a, int *stk, const Ray &r, int &nr)
+// The lanes that call these together trace together (packet_any and
+// packet_closest over the wave's stack `wstk`): the rays of one light, one sun
+// or sky direction, or one AO direction of one face leave from the samples of
+// one face, so they travel as a packet.
+__device__ __forceinline__ bool blocked(const GatherArgs &a, int *wstk, const Ray &r, int &nr)
 {
-    if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
-          isfinite(r.oy) && isfinite(r.oz)))
-        return false;
-    nr++;
-    float t;
-    int32_t id;
-    return trace_one<kAny, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id) >= 0;
+    const bool fin = ray_finite(r);
+    nr += fin ? 1 : 0;
+    return packet_any<false>(a.nodes, a.tris, wstk, r, 0.0f, fin);
 }
 
 // LightRayLog.Occlusion: the first hit short of the end, unless it is sky.
-__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *stk, const Ray &r, int &nr)
+__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *wstk, const Ray &r, int &nr)
 {
-    if (!(isfinite(r.dx) && isfinite(r.dy) && isfinite(r.dz) && isfinite(r.ox) &&
-          isfinite(r.oy) && isfinite(r.oz)))
-        return 0.0f;
-    nr++;
+    const bool fin = ray_finite(r);
+    nr += fin ? 1 : 0;
     float t;
     int32_t id;
-    int32_t h = trace_one<kClosestShort, false, kWave>(a.nodes, a.tris, stk, r, 0.0f, -1, t, id);
-    return (h >= 0 && (id & VH_TRACE_ID_SKY) == 0) ? 1.0f : 0.0f;
+    const int32_t h = packet_closest<kClosestShort>(a.nodes, a.tris, wstk, r, 0.0f, fin, t, id);
+    return (fin && h >= 0 && (id & VH_TRACE_ID_SKY) == 0) ? 1.0f : 0.0f;
 }
 
 __device__ __forceinline__ float inverse_quadratic(const vh_light &l, float d)
@@ -408,8 +406,7 @@ template <bool PERSISTENT>
 __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first_chunk,
                                                      unsigned long long *__restrict__ ticket)
 {
-    __shared__ int stack_mem[kMaxDepth * kWave];
-    int *stk = stack_mem + threadIdx.x;
+    __shared__ int stk[3 * kMaxDepth];  // the wave's traversal stack
     if (!PERSISTENT) {
         light_chunk(a, first_chunk + blockIdx.x, stk);
         return;

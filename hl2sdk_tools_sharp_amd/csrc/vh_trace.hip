@@ -47,10 +47,10 @@ __global__ __launch_bounds__(kBlock) void k_visibility(const float4 *__restrict_
             float t;
             int32_t id;
             if (SKY) {
-                int32_t h = trace_one<kClosestShort>(nodes, tris, stk, r, tmin, skip_id, t, id);
+                int32_t h = trace_one<kClosestShort, false, kBlock>(nodes, tris, stk, r, tmin, skip_id, t, id);
                 blocked = h >= 0 && (id & VH_TRACE_ID_SKY) == 0;
             } else {
-                blocked = trace_one<kAny>(nodes, tris, stk, r, tmin, skip_id, t, id) >= 0;
+                blocked = trace_one<kAny, false, kBlock>(nodes, tris, stk, r, tmin, skip_id, t, id) >= 0;
             }
         }
     }
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_closest(const float4 *__restrict__ n
     if (ray_finite(r)) {
         float t;
         int32_t id;
-        int32_t h = trace_one<kClosestInclusive>(nodes, tris, stk, r, tmin, skip_id, t, id);
+        int32_t h = trace_one<kClosestInclusive, false, kBlock>(nodes, tris, stk, r, tmin, skip_id, t, id);
         if (h >= 0) {
             sid = id;
             // SlabBatcher's fraction rule: the divide by a reach of 1 is skipped
@@ -126,52 +126,51 @@ __device__ __forceinline__ void wave_add(unsigned long long *counters, int slot,
 // Shadow rays from points to point lights.  One lane owns one point (visited
 // in Morton order) and walks every light; all lanes of a wave trace the same
 // light at once, so the wave's 64 rays converge on one light from nearby
-// starts.  The lane assembles its own hit word: bit l of word (p, l/64).
+// starts, and they walk the tree together (packet_any: one node sequence per
+// wave, lane masks per node, the wave's stack in LDS).  The lane assembles its
+// own hit word: bit l of word (p, l/64).
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_point_lights(
     const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
     const float *__restrict__ pts, const int32_t *__restrict__ order, int64_t npoints,
     const float *__restrict__ lights, int32_t nlights, float tmin, uint64_t *__restrict__ bits,
-    unsigned long long *__restrict__ counters)
+    unsigned long long *__restrict__ counters, int32_t depth)
 {
-    __shared__ int stack_mem[kMaxDepth * kBlock];
-    int *stk = stack_mem + threadIdx.x;
-    int64_t slot = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    extern __shared__ int stack_mem[];
+    int *wstk = stack_mem + (threadIdx.x >> 6) * 3 * depth;
+    const int64_t slot = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t nvisit = 0, ntest = 0;
-    if (slot < npoints) {
-        int64_t p = order ? order[slot] : slot;
-        float px = pts[p * 3 + 0], py = pts[p * 3 + 1], pz = pts[p * 3 + 2];
-        bool whole_words = (nlights & 63) == 0;
-        for (int32_t g = 0; g < nlights; g += 64) {
-            int32_t cnt = nlights - g < 64 ? nlights - g : 64;
-            uint64_t w = 0;
+    const bool have = slot < npoints;
+    const int64_t p = have ? (order ? order[slot] : slot) : 0;
+    const float px = have ? pts[p * 3 + 0] : 0.0f, py = have ? pts[p * 3 + 1] : 0.0f,
+                pz = have ? pts[p * 3 + 2] : 0.0f;
+    const bool whole_words = (nlights & 63) == 0;
+    // every lane of the wave runs every light together
+    for (int32_t g = 0; g < nlights; g += 64) {
+        const int32_t cnt = nlights - g < 64 ? nlights - g : 64;
+        uint64_t w = 0;
+        for (int32_t l = 0; l < cnt; l++) {
+            const float *L = lights + (size_t)(g + l) * 3;
+            Ray r;
+            r.ox = px;
+            r.oy = py;
+            r.oz = pz;
+            r.dx = L[0] - px;
+            r.dy = L[1] - py;
+            r.dz = L[2] - pz;
+            r.reach = 1.0f;
+            const bool blocked =
+                packet_any<COUNT>(nodes, tris, wstk, r, tmin, have && ray_finite(r), &nvisit, &ntest);
+            w |= (uint64_t)blocked << l;
+        }
+        if (!have) continue;
+        if (whole_words) {
+            bits[p * (nlights >> 6) + (g >> 6)] = w;
+        } else {
             for (int32_t l = 0; l < cnt; l++) {
-                const float *L = lights + (size_t)(g + l) * 3;
-                Ray r;
-                r.ox = px;
-                r.oy = py;
-                r.oz = pz;
-                r.dx = L[0] - px;
-                r.dy = L[1] - py;
-                r.dz = L[2] - pz;
-                r.reach = 1.0f;
-                bool blocked = false;
-                if (ray_finite(r)) {
-                    float t;
-                    int32_t id;
-                    blocked = trace_one<kAny, COUNT>(nodes, tris, stk, r, tmin, -1, t, id, &nvisit,
-                                                     &ntest) >= 0;
-                }
-                w |= (uint64_t)blocked << l;
-            }
-            if (whole_words) {
-                bits[p * (nlights >> 6) + (g >> 6)] = w;
-            } else {
-                for (int32_t l = 0; l < cnt; l++) {
-                    if ((w >> l) & 1) {
-                        int64_t rix = p * nlights + g + l;
-                        atomicOr((unsigned long long *)&bits[rix >> 6], 1ull << (rix & 63));
-                    }
+                if ((w >> l) & 1) {
+                    int64_t rix = p * nlights + g + l;
+                    atomicOr((unsigned long long *)&bits[rix >> 6], 1ull << (rix & 63));
                 }
             }
         }
@@ -255,17 +254,20 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
         VH_HIP(hipMemsetAsync(d_bits, 0, words * 8, s));
     }
     int64_t blocks = (npoints + kBlock - 1) / kBlock;
+    // each wave's stack holds one (node, lane mask) entry per internal level
+    int32_t depth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;
+    size_t lds = (size_t)(kBlock / 64) * 3 * depth * sizeof(int);
     KernelTimer timer(c, VH_KERNEL_POINT_LIGHTS, s);
     if (c->counting)
-        hipLaunchKernelGGL(k_point_lights<true>, dim3((uint32_t)blocks), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_point_lights<true>, dim3((uint32_t)blocks), dim3(kBlock), lds, s,
                            (const float4 *)c->nodes.ptr, (const float4 *)c->tris.ptr, d_points,
                            (const int32_t *)c->perm_b.ptr, npoints, d_lights, nlights, tmin,
-                           d_bits, (unsigned long long *)c->counters.ptr);
+                           d_bits, (unsigned long long *)c->counters.ptr, depth);
     else
-        hipLaunchKernelGGL(k_point_lights<false>, dim3((uint32_t)blocks), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_point_lights<false>, dim3((uint32_t)blocks), dim3(kBlock), lds, s,
                            (const float4 *)c->nodes.ptr, (const float4 *)c->tris.ptr, d_points,
                            (const int32_t *)c->perm_b.ptr, npoints, d_lights, nlights, tmin,
-                           d_bits, nullptr);
+                           d_bits, nullptr, depth);
     timer.done();
     VH_HIP(hipGetLastError());
     return VH_OK;

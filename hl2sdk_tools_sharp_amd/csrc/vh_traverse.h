@@ -75,7 +75,13 @@ __device__ __forceinline__ bool tri_test(const float4 t0, const float4 t1, const
 // One ray through the BVH.  MODE kAny returns at the first hit strictly short
 // of the reach; the closest modes keep the nearest (ties to the lower
 // original triangle index).  Returns the original triangle index or -1.
-template <int MODE, bool COUNT = false, int STRIDE = 128>
+// STRIDE is the LDS stack's row stride: the block size of the caller.
+//
+// The walk is "while-while": a lane descends through internal nodes until it
+// reaches a leaf or runs out of nodes, and only then tests the leaf's
+// triangles, so the lanes of a wave test leaves together instead of one
+// lane's leaf stalling the others' node steps.
+template <int MODE, bool COUNT, int STRIDE>
 __device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
                                              const float4 *__restrict__ tris, int *stk,
                                              const Ray &r, float tmin, int32_t skip_id,
@@ -90,7 +96,7 @@ __device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
     int sp = 0;
     int32_t ref = 0;
     for (;;) {
-        if (ref >= 0) {
+        while (ref >= 0) {
             if (COUNT) ++*nvisit;
             const float4 *nd = nodes + (size_t)ref * 4;
             float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
@@ -119,7 +125,15 @@ __device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
                 ref = h0 ? r0 : r1;
                 continue;
             }
-        } else {
+            if (sp == 0) {
+                ref = kEmptyRef;  // done: no leaf to test
+                break;
+            }
+            sp--;
+            ref = stk[sp * STRIDE];
+        }
+        if (ref == kEmptyRef) break;
+        {
             int32_t k = ~ref;
             for (;;) {
                 if (COUNT) ++*ntest;
@@ -157,6 +171,220 @@ __device__ __forceinline__ int32_t trace_one(const float4 *__restrict__ nodes,
         if (sp == 0) break;
         sp--;
         ref = stk[sp * STRIDE];
+    }
+    t_out = best_t;
+    id_out = best_id;
+    return best_orig;
+}
+
+// The wave's lane index and the wave-uniform form of a value every lane holds.
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Any hit for the 64 rays of a wave that travel together (one light from
+// nearby starts, one sky or sun direction, one AO direction of one face).
+// Every lane of the wave calls it at once; `want` is false for a lane with no
+// ray.  The wave walks one node sequence: each node is tested by every lane
+// whose ray entered its parent's box (the node's lane mask); a child is
+// visited when any of those lanes' rays enters it, with the lanes that do; a
+// lane leaves the walk when its ray is blocked.  A lane therefore tests
+// exactly the nodes and triangles its own traversal would (up to when it
+// stops), so the answer is the one-ray answer, while the control flow is the
+// wave's, not the lane's: no divergence, node fetches shared by the wave.
+// The stack is the wave's (`wstk`, 3 ints a level in LDS: node, lane mask).
+// The near child is the one the mask's first lane enters first.
+template <bool COUNT>
+__device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
+                                           const float4 *__restrict__ tris, int *wstk, const Ray &r,
+                                           float tmin, bool want, uint32_t *nvisit = nullptr,
+                                           uint32_t *ntest = nullptr)
+{
+    const uint64_t me = 1ull << lane_id();
+    const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
+    const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
+    const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
+    const float thi = r.reach;
+    uint64_t mask = uniform64(__ballot(want));
+    uint64_t done = 0;  // lanes already blocked
+    bool blocked = false;
+    int sp = 0;
+    int32_t ref = 0;
+    while (mask) {
+        if (ref >= 0) {
+            const bool in = (mask & me) != 0;
+            if (COUNT && in) ++*nvisit;
+            const float4 *nd = nodes + (size_t)ref * 4;
+            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+            float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
+            float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
+            float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
+            float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
+            float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
+            float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
+            float tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
+            float tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
+            float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
+            float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
+            const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
+            // both children's tests in straight-line code, masks by ballot
+            const bool e0 = in && (r0 != kEmptyRef) && tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
+            const bool e1 = in && (r1 != kEmptyRef) && tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1);
+            const uint64_t m0 = uniform64(__ballot(e0));
+            const uint64_t m1 = uniform64(__ballot(e1));
+            if (m0 && m1) {
+                const uint64_t both = m0 & m1;
+                const int fl = __builtin_ctzll(both ? both : mask);
+                const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
+                const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
+                const bool swap = n1 < n0;
+                const int32_t far = swap ? r0 : r1;
+                const uint64_t fm = swap ? m0 : m1;
+                wstk[3 * sp] = far;
+                wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
+                wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
+                sp++;
+                ref = swap ? r1 : r0;
+                mask = swap ? m1 : m0;
+                continue;
+            }
+            if (m0 | m1) {
+                ref = m0 ? r0 : r1;
+                mask = m0 | m1;
+                continue;
+            }
+        } else {
+            // the leaf's triangles, fetched once for the wave; each lane of the
+            // mask tests them until its ray is blocked
+            const bool in = (mask & me) != 0;
+            for (int32_t k = ~ref;; k++) {
+                const float4 *tp = tris + (size_t)k * 3;
+                const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
+                if (in && !blocked) {
+                    if (COUNT) ++*ntest;
+                    float t;
+                    blocked = tri_test(t0, t1, t2, meta, r, t) && t >= tmin && t < r.reach;
+                }
+                if (meta & kTriLast) break;
+            }
+            done = uniform64(__ballot(blocked));
+        }
+        // the next pending subtree that still has unblocked lanes
+        mask = 0;
+        while (sp > 0) {
+            sp--;
+            ref = uniform(wstk[3 * sp]);
+            const uint64_t lo = (uint32_t)uniform(wstk[3 * sp + 1]);
+            const uint64_t hi = (uint32_t)uniform(wstk[3 * sp + 2]);
+            mask = ((hi << 32) | lo) & ~done;
+            if (mask) break;
+        }
+    }
+    return blocked;
+}
+
+// The closest hit (MODE kClosestShort: t < reach; kClosestInclusive:
+// t <= reach) for the 64 rays of a wave that travel together, walked as in
+// packet_any.  A lane's box tests are bounded by its own best hit so far, so
+// masks shrink as lanes find hits; ties go to the lower original triangle
+// index, so the answer is the one-ray answer.  Returns the lane's original
+// triangle index or -1, with its distance and caller id.
+template <int MODE>
+__device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nodes,
+                                                  const float4 *__restrict__ tris, int *wstk,
+                                                  const Ray &r, float tmin, bool want, float &t_out,
+                                                  int32_t &id_out)
+{
+    const uint64_t me = 1ull << lane_id();
+    const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
+    const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
+    const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
+    float best_t = r.reach;
+    int32_t best_orig = -1, best_id = -1;
+    uint64_t mask = uniform64(__ballot(want));
+    int sp = 0;
+    int32_t ref = 0;
+    while (mask) {
+        if (ref >= 0) {
+            const bool in = (mask & me) != 0;
+            const float4 *nd = nodes + (size_t)ref * 4;
+            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+            const float thi = best_t;
+            float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
+            float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
+            float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
+            float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
+            float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
+            float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
+            float tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
+            float tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
+            float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
+            float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
+            const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
+            const bool e0 = in && (r0 != kEmptyRef) && tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
+            const bool e1 = in && (r1 != kEmptyRef) && tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1);
+            const uint64_t m0 = uniform64(__ballot(e0));
+            const uint64_t m1 = uniform64(__ballot(e1));
+            if (m0 && m1) {
+                const uint64_t both = m0 & m1;
+                const int fl = __builtin_ctzll(both ? both : mask);
+                const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
+                const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
+                const bool swap = n1 < n0;
+                const int32_t far = swap ? r0 : r1;
+                const uint64_t fm = swap ? m0 : m1;
+                wstk[3 * sp] = far;
+                wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
+                wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
+                sp++;
+                ref = swap ? r1 : r0;
+                mask = swap ? m1 : m0;
+                continue;
+            }
+            if (m0 | m1) {
+                ref = m0 ? r0 : r1;
+                mask = m0 | m1;
+                continue;
+            }
+        } else {
+            const bool in = (mask & me) != 0;
+            for (int32_t k = ~ref;; k++) {
+                const float4 *tp = tris + (size_t)k * 3;
+                const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
+                if (in) {
+                    float t;
+                    if (tri_test(t0, t1, t2, meta, r, t) && t >= tmin) {
+                        const int32_t orig = (int32_t)(meta >> kOrigShift);
+                        const bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);
+                        const bool better = best_orig < 0
+                                                ? within
+                                                : (t < best_t || (t == best_t && orig < best_orig));
+                        if (better) {
+                            best_t = t;
+                            best_orig = orig;
+                            best_id = __float_as_int(t2.z);
+                        }
+                    }
+                }
+                if (meta & kTriLast) break;
+            }
+        }
+        mask = 0;
+        while (sp > 0) {
+            sp--;
+            ref = uniform(wstk[3 * sp]);
+            const uint64_t lo = (uint32_t)uniform(wstk[3 * sp + 1]);
+            const uint64_t hi = (uint32_t)uniform(wstk[3 * sp + 2]);
+            mask = (hi << 32) | lo;
+            if (mask) break;
+        }
     }
     t_out = best_t;
     id_out = best_id;
