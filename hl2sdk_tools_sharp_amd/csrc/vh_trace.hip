@@ -11,6 +11,7 @@ namespace vh {
 namespace {
 
 constexpr int kBlock = 128;
+constexpr int kOccluders = 3;  // occluders a point-light lane remembers
 
 __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays, int rw, float shared_reach,
                                         int64_t i)
@@ -127,8 +128,11 @@ __device__ __forceinline__ void wave_add(unsigned long long *counters, int slot,
 // in Morton order) and walks every light; all lanes of a wave trace the same
 // light at once, so the wave's 64 rays converge on one light from nearby
 // starts, and they walk the tree together (packet_any: one node sequence per
-// wave, lane masks per node, the wave's stack in LDS).  The lane assembles its
-// own hit word: bit l of word (p, l/64).
+// wave, lane masks per node, the wave's stack in LDS).  Each lane first tests
+// the few triangles that blocked its earlier rays (a point's own face blocks
+// every light behind it): a hit is an exact hit short of the end, so the bit
+// is the traversal's, and a lane it settles leaves the packet.  The lane
+// assembles its own hit word: bit l of word (p, l/64).
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_point_lights(
     const float4 *__restrict__ nodes, const float4 *__restrict__ tris,
@@ -145,6 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
     const float px = have ? pts[p * 3 + 0] : 0.0f, py = have ? pts[p * 3 + 1] : 0.0f,
                 pz = have ? pts[p * 3 + 2] : 0.0f;
     const bool whole_words = (nlights & 63) == 0;
+    int32_t occ[kOccluders];  // the lane's remembered occluders (leaf-order triangles)
+    for (int c = 0; c < kOccluders; c++) occ[c] = -1;
     // every lane of the wave runs every light together
     for (int32_t g = 0; g < nlights; g += 64) {
         const int32_t cnt = nlights - g < 64 ? nlights - g : 64;
@@ -159,8 +165,29 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
             r.dy = L[1] - py;
             r.dz = L[2] - pz;
             r.reach = 1.0f;
-            const bool blocked =
-                packet_any<COUNT>(nodes, tris, wstk, r, tmin, have && ray_finite(r), &nvisit, &ntest);
+            const bool fin = have && ray_finite(r);
+            bool blocked = false;
+            // the remembered occluders, kept as a transpose list: a hit moves
+            // one place forward, so the often-blocking ones stay in front
+            for (int c = 0; c < kOccluders && fin && !blocked; c++) {
+                if (occ[c] < 0) break;
+                if (COUNT) ++ntest;
+                if (occluder_blocks(tris, occ[c], r, tmin)) {
+                    blocked = true;
+                    if (c > 0) {
+                        const int32_t x = occ[c - 1];
+                        occ[c - 1] = occ[c];
+                        occ[c] = x;
+                    }
+                }
+            }
+            int32_t hit = -1;
+            if (packet_any<COUNT>(nodes, tris, wstk, r, tmin, fin && !blocked, &nvisit, &ntest, &hit)) {
+                blocked = true;
+                int c = 0;
+                while (c < kOccluders - 1 && occ[c] >= 0) c++;
+                occ[c] = hit;  // an empty slot, else the last
+            }
             w |= (uint64_t)blocked << l;
         }
         if (!have) continue;

@@ -203,14 +203,14 @@ template <bool COUNT>
 __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, int *wstk, const Ray &r,
                                            float tmin, bool want, uint32_t *nvisit = nullptr,
-                                           uint32_t *ntest = nullptr)
+                                           uint32_t *ntest = nullptr, int32_t *blocker = nullptr)
 {
     const uint64_t me = 1ull << lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
     const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
     const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
     const float thi = r.reach;
-    uint64_t mask = uniform64(__ballot(want));
+    uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     uint64_t done = 0;  // lanes already blocked
     bool blocked = false;
     int sp = 0;
@@ -233,10 +233,13 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
             // both children's tests in straight-line code, masks by ballot
-            const bool e0 = in && (r0 != kEmptyRef) && tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
-            const bool e1 = in && (r1 != kEmptyRef) && tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1);
-            const uint64_t m0 = uniform64(__ballot(e0));
-            const uint64_t m1 = uniform64(__ballot(e1));
+            // a child's lanes: the node's lanes whose rays enter its box
+            const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
+                                    tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
+                                mask & (r0 != kEmptyRef ? ~0ull : 0ull);
+            const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
+                                    tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
+                                mask & (r1 != kEmptyRef ? ~0ull : 0ull);
             if (m0 && m1) {
                 const uint64_t both = m0 & m1;
                 const int fl = __builtin_ctzll(both ? both : mask);
@@ -270,10 +273,11 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                     if (COUNT) ++*ntest;
                     float t;
                     blocked = tri_test(t0, t1, t2, meta, r, t) && t >= tmin && t < r.reach;
+                    if (blocked && blocker) *blocker = k;
                 }
                 if (meta & kTriLast) break;
             }
-            done = uniform64(__ballot(blocked));
+            done = uniform64(__builtin_amdgcn_ballot_w64(blocked));
         }
         // the next pending subtree that still has unblocked lanes
         mask = 0;
@@ -287,6 +291,17 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
         }
     }
     return blocked;
+}
+
+// Whether triangle k (leaf order) blocks the ray strictly short of its reach:
+// the exact test for one remembered occluder.
+__device__ __forceinline__ bool occluder_blocks(const float4 *__restrict__ tris, int32_t k, const Ray &r,
+                                                float tmin)
+{
+    const float4 *tp = tris + (size_t)k * 3;
+    const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+    float t;
+    return tri_test(t0, t1, t2, __float_as_uint(t2.w), r, t) && t >= tmin && t < r.reach;
 }
 
 // The closest hit (MODE kClosestShort: t < reach; kClosestInclusive:
@@ -307,7 +322,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
     const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
     float best_t = r.reach;
     int32_t best_orig = -1, best_id = -1;
-    uint64_t mask = uniform64(__ballot(want));
+    uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     int sp = 0;
     int32_t ref = 0;
     while (mask) {
@@ -327,10 +342,13 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
             float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
-            const bool e0 = in && (r0 != kEmptyRef) && tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0);
-            const bool e1 = in && (r1 != kEmptyRef) && tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1);
-            const uint64_t m0 = uniform64(__ballot(e0));
-            const uint64_t m1 = uniform64(__ballot(e1));
+            // a child's lanes: the node's lanes whose rays enter its box
+            const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
+                                    tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
+                                mask & (r0 != kEmptyRef ? ~0ull : 0ull);
+            const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
+                                    tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
+                                mask & (r1 != kEmptyRef ? ~0ull : 0ull);
             if (m0 && m1) {
                 const uint64_t both = m0 & m1;
                 const int fl = __builtin_ctzll(both ? both : mask);
