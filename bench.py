@@ -440,12 +440,13 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     sptr = ctypes.c_void_p(stream.cuda_stream)
     if a.copy_probe_mib > 0:
-        # a plain copy of known size, the scale for rocprofv3's FETCH_SIZE/WRITE_SIZE
+        # a plain copy of known size (an elementwise kernel reading and writing
+        # the bytes once), the scale for rocprofv3's FETCH_SIZE/WRITE_SIZE
         src = torch.empty(a.copy_probe_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         with torch.cuda.stream(stream):
             for _ in range(3):
-                dst.copy_(src)
+                torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     P = ctypes.c_void_p
