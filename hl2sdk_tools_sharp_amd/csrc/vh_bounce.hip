@@ -331,7 +331,11 @@ __global__ __launch_bounds__(kGB) void k_half_rows(const int32_t *__restrict__ r
 }
 
 // GatherLight (Radiosity.cs:216-252), flat: one wave per receiver row, FP32
-// accumulation of shoot[src] * weight, the row reduced across the wave.
+// accumulation of shoot[src] * weight, the row reduced across the wave.  Lane
+// l takes transfers l, l + 64, ... so each load instruction of the wave reads
+// 64 consecutive transfers: 256 contiguous bytes of sources and of weights,
+// and shooters of about 64 consecutive patches (rows are sorted by source),
+// a few cache lines.  Four such steps are in flight at once.
 template <bool F16>
 __global__ __launch_bounds__(kGB) void k_gather(const int32_t *__restrict__ rows, int32_t nrows,
                                                 const int64_t *__restrict__ row_ptr,
@@ -342,19 +346,54 @@ __global__ __launch_bounds__(kGB) void k_gather(const int32_t *__restrict__ rows
                                                 const float4 *__restrict__ shoot,
                                                 float4 *__restrict__ add)
 {
-    int lane = threadIdx.x & 63;
-    int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
     if (k >= nrows) return;
-    int32_t r = rows[k];
-    int64_t b = row_ptr[r], e = row_ptr[r + 1];
+    const int32_t r = rows[k];
+    const int64_t b = row_ptr[r], e = row_ptr[r + 1];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (int64_t j = b + lane; j < e; j += 64) {
-        int32_t s = src[j];
-        float wk = F16 ? __half2float(w16[j]) : w32[j];
-        float4 sh = shoot[s];
-        ax = __builtin_fmaf(sh.x, wk, ax);
-        ay = __builtin_fmaf(sh.y, wk, ay);
-        az = __builtin_fmaf(sh.z, wk, az);
+    auto acc = [&](const float4 &h, float w) {
+        ax = __builtin_fmaf(h.x, w, ax);
+        ay = __builtin_fmaf(h.y, w, ay);
+        az = __builtin_fmaf(h.z, w, az);
+    };
+    int64_t j = b + lane;
+    // whole steps of four: the four loads of sources and weights, then the four
+    // shooter gathers, are all in flight before the first is used
+    for (; j + 192 < e; j += 256) {
+        const int32_t s0 = src[j], s1 = src[j + 64], s2 = src[j + 128], s3 = src[j + 192];
+        float w0, w1, w2, w3;
+        if (F16) {
+            w0 = __half2float(w16[j]);
+            w1 = __half2float(w16[j + 64]);
+            w2 = __half2float(w16[j + 128]);
+            w3 = __half2float(w16[j + 192]);
+        } else {
+            w0 = w32[j];
+            w1 = w32[j + 64];
+            w2 = w32[j + 128];
+            w3 = w32[j + 192];
+        }
+        const float4 h0 = shoot[s0], h1 = shoot[s1], h2 = shoot[s2], h3 = shoot[s3];
+        acc(h0, w0);
+        acc(h1, w1);
+        acc(h2, w2);
+        acc(h3, w3);
+    }
+    // the last, partial step: up to three slots, each read only inside the row
+    if (j < e) {
+        float4 h[3];
+        float w[3];
+        _Pragma("unroll") for (int u = 0; u < 3; u++) {
+            const int64_t ju = j + 64 * u;
+            h[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            w[u] = 0.0f;
+            if (ju < e) {
+                w[u] = F16 ? __half2float(w16[ju]) : w32[ju];
+                h[u] = shoot[src[ju]];
+            }
+        }
+        _Pragma("unroll") for (int u = 0; u < 3; u++) acc(h[u], w[u]);
     }
     for (int off = 32; off > 0; off >>= 1) {
         ax += __shfl_xor(ax, off, 64);
