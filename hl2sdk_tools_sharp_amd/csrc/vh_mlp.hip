@@ -57,6 +57,7 @@ constexpr int kLayers = 4;
 constexpr int kIn = 25, kHidden = 128, kOut = 3, kFourier = 8;
 constexpr int kInPad = 32, kOutPad = 32;
 constexpr int kLight = 6;  // data-set light per row: direct rgb, bounced rgb
+constexpr int kMaxSplits = 32;  // row splits of a weight-gradient GEMM
 constexpr int kInDim[kLayers] = {kIn, kHidden, kHidden, kHidden};
 constexpr int kOutDim[kLayers] = {kHidden, kHidden, kHidden, kOut};
 constexpr int kInP[kLayers] = {kInPad, kHidden, kHidden, kHidden};
@@ -103,6 +104,11 @@ struct MlpData {
     int64_t rows = 0, cap = 0;
     // activations of one pass over `act_rows` rows
     vh::DevBuf h[4], z[3], y, dy, dz[3], xf, tf, loss_part, losses, state, loss1;
+    // the weight-gradient GEMMs' operands, K-blocked: element (row b, column c)
+    // of an R x C matrix at ((b / 8) * C + c) * 8 + b % 8, so eight consecutive
+    // rows of one column are one 16-byte load; hk[l] holds layer l's input,
+    // dzk[l] the gradient at its output; dw_part the split-K partial sums
+    vh::DevBuf hk[4], dzk[4], dw_part, cs_part;
     int64_t act_rows = 0;
     int64_t step = 0;  // Adam steps taken
     // the prediction's rows (leaf, non-sky patches by cluster) for one patch system
@@ -131,6 +137,7 @@ struct GemmArgs {
     const float *zin;    // EPI_DGELU: the pre-activation of this layer's input
     int64_t ldzi;
     float *colsum;       // EPI_STORE (may be null): column sums of B
+    __bf16 *hblk;        // EPI_GELU, EPI_DGELU (may be null): hout again, K-blocked (N columns)
 };
 
 enum { EPI_GELU = 0, EPI_OUT = 1, EPI_DGELU = 2, EPI_STORE = 3 };
@@ -184,20 +191,28 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     }
     const int col = n0 + r;
     const int64_t row0 = m0 + (lane >> 4) * 4;
+    __bf16 hv[4];
     for (int i = 0; i < 4; i++) {
         const int64_t row = row0 + i;
         float v = acc[i];
         if (EPI == EPI_GELU) {
             float zz = v + g.bias[col];
             if (g.zout) g.zout[row * g.ldz + col] = zz;
-            g.hout[row * g.ldh + col] = (__bf16)gelu(zz);
+            hv[i] = (__bf16)gelu(zz);
+            g.hout[row * g.ldh + col] = hv[i];
         } else if (EPI == EPI_OUT) {
             g.cout[row * g.ldc + col] = v + g.bias[col];
         } else if (EPI == EPI_DGELU) {
-            g.hout[row * g.ldh + col] = (__bf16)(v * gelu_grad(g.zin[row * g.ldzi + col]));
+            hv[i] = (__bf16)(v * gelu_grad(g.zin[row * g.ldzi + col]));
+            g.hout[row * g.ldh + col] = hv[i];
         } else {
             g.cout[row * g.ldc + col] = v;
         }
+    }
+    if ((EPI == EPI_GELU || EPI == EPI_DGELU) && g.hblk) {
+        // rows row0 .. row0 + 3 of one column: four consecutive slots
+        __bf16 *dst = g.hblk + ((row0 >> 3) * g.N + col) * 8 + (row0 & 7);
+        *(uint2 *)dst = *(const uint2 *)hv;
     }
     if (EPI == EPI_STORE && g.colsum) {
         cs += __shfl_xor(cs, 16, 64);
@@ -209,7 +224,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 // Output-layer gradient and loss: d = y - t over the three outputs,
 // dy = 2 d / (3 n) as BF16 (padded columns 0), per-block sums of d^2.
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y, const float *__restrict__ t,
-                                              int64_t n, __bf16 *__restrict__ dy, double *__restrict__ part)
+                                              int64_t n, __bf16 *__restrict__ dy, __bf16 *__restrict__ dyk,
+                                              double *__restrict__ part)
 {
     __shared__ double red[256];
     const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -222,7 +238,9 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y, const
                 d = y[row * kOutPad + c] - t[row * kOut + c];
                 acc += (double)d * (double)d;
             }
-            dy[row * kOutPad + c] = (__bf16)(d * scale);
+            const __bf16 g = (__bf16)(d * scale);
+            dy[row * kOutPad + c] = g;
+            if (dyk) dyk[((row >> 3) * kOutPad + c) * 8 + (row & 7)] = g;
         }
     }
     red[threadIdx.x] = acc;
@@ -316,7 +334,7 @@ __global__ void k_advance(int64_t *state)
 __global__ __launch_bounds__(256) void k_batch(const __bf16 *__restrict__ X, const float *__restrict__ T,
                                                int64_t n, const int64_t *__restrict__ state, int32_t nb,
                                                float gain, float inv_scale, __bf16 *__restrict__ xb,
-                                               float *__restrict__ tb)
+                                               __bf16 *__restrict__ xk, float *__restrict__ tb)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
@@ -327,6 +345,7 @@ __global__ __launch_bounds__(256) void k_batch(const __bf16 *__restrict__ X, con
     const uint4 *src = (const uint4 *)(X + j * kInPad);
     uint4 *dst = (uint4 *)(xb + i * kInPad);
     for (int q = 0; q < kInPad * 2 / 16; q++) dst[q] = src[q];
+    for (int c = 0; c < kInPad; c++) xk[((i >> 3) * kInPad + c) * 8 + (i & 7)] = X[j * kInPad + c];
     // the target: (bounced - gain * direct) / scale
     for (int k = 0; k < kOut; k++)
         tb[i * kOut + k] = (T[j * kLight + kOut + k] - gain * T[j * kLight + k]) * inv_scale;
@@ -386,11 +405,73 @@ __global__ __launch_bounds__(256) void k_features(FeatArgs a)
 
 // Host rows (n x 25 FP32) to BF16 operands (n x 32), padded rows zero.
 __global__ __launch_bounds__(256) void k_pack(const float *__restrict__ xf, int64_t n, int64_t npad,
-                                              __bf16 *__restrict__ x)
+                                              __bf16 *__restrict__ x, __bf16 *__restrict__ xk)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= npad) return;
-    for (int k = 0; k < kInPad; k++) x[i * kInPad + k] = (__bf16)((i < n && k < kIn) ? xf[i * kIn + k] : 0.0f);
+    for (int k = 0; k < kInPad; k++) {
+        const __bf16 v = (__bf16)((i < n && k < kIn) ? xf[i * kIn + k] : 0.0f);
+        x[i * kInPad + k] = v;
+        if (xk) xk[((i >> 3) * kInPad + k) * 8 + (i & 7)] = v;
+    }
+}
+
+// Weight gradient, split over K: dW (M x N) = A^T B over rows [k0, k1) of a
+// split, A (K x M) and B (K x N) K-blocked.  One wave per (16 x 16 tile,
+// split); lane l holds column (l & 15) of A^T and of B at rows 8 (l >> 4) .. + 7
+// of each step of 32, one 16-byte load each.  Partial tiles and the partial
+// column sums of B (the bias gradient) go to the workspace, summed in split
+// order by k_dw_reduce.
+__global__ __launch_bounds__(256) void k_gemm_dw(const __bf16 *__restrict__ a, int32_t M,
+                                                 const __bf16 *__restrict__ b, int32_t N, int32_t K,
+                                                 int32_t kc, int32_t splits, float *__restrict__ part,
+                                                 float *__restrict__ cpart)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_n = N >> 4;
+    const int64_t tiles = (int64_t)(M >> 4) * tiles_n;
+    if (w >= tiles * splits) return;  // whole wave
+    const int split = (int)(w / tiles);
+    const int64_t tile = w % tiles;
+    const int m0 = (int)(tile / tiles_n) * 16, n0 = (int)(tile % tiles_n) * 16;
+    const int r = lane & 15, kg = lane >> 4;
+    const int32_t ka = split * kc, kb = min(K, ka + kc);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float cs = 0.0f;
+    for (int32_t k0 = ka; k0 < kb; k0 += 32) {
+        const int64_t blk = (k0 >> 3) + kg;
+        const bf16x8 af = *(const bf16x8 *)(a + (blk * M + m0 + r) * 8);
+        const bf16x8 bfr = *(const bf16x8 *)(b + (blk * N + n0 + r) * 8);
+        for (int j = 0; j < 8; j++) cs += (float)bfr[j];
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc, 0, 0, 0);
+    }
+    float *out = part + (int64_t)split * M * N;
+    const int row0 = m0 + kg * 4;
+    for (int i = 0; i < 4; i++) out[(int64_t)(row0 + i) * N + n0 + r] = acc[i];
+    cs += __shfl_xor(cs, 16, 64);
+    cs += __shfl_xor(cs, 32, 64);
+    if (m0 == 0 && lane < 16) cpart[(int64_t)split * N + n0 + r] = cs;
+}
+
+// dW and db from the split partials, in split order.
+__global__ __launch_bounds__(256) void k_dw_reduce(const float *__restrict__ part,
+                                                   const float *__restrict__ cpart, int32_t M, int32_t N,
+                                                   int32_t splits, float *__restrict__ dw,
+                                                   float *__restrict__ db)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t mn = (int64_t)M * N;
+    if (i < mn) {
+        float s = 0.0f;
+        for (int k = 0; k < splits; k++) s += part[(int64_t)k * mn + i];
+        dw[i] = s;
+    } else if (i < mn + N) {
+        const int64_t c = i - mn;
+        float s = 0.0f;
+        for (int k = 0; k < splits; k++) s += cpart[(int64_t)k * N + c];
+        db[c] = s;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ v, int64_t n)
@@ -498,6 +579,13 @@ int ensure_acts(MlpData *D, int64_t rows)
         if ((rc = D->z[l].ensure((size_t)rows * kHidden * 4, "pre-activations"))) return rc;
         if ((rc = D->dz[l].ensure((size_t)rows * kHidden * 2, "activation gradients"))) return rc;
     }
+    for (int l = 0; l < 4; l++)
+        if ((rc = D->hk[l].ensure((size_t)rows * kInP[l] * 2, "blocked activations")) ||
+            (rc = D->dzk[l].ensure((size_t)rows * kOutP[l] * 2, "blocked gradients")))
+            return rc;
+    if ((rc = D->dw_part.ensure((size_t)kMaxSplits * kHidden * kHidden * 4, "weight-gradient partials")) ||
+        (rc = D->cs_part.ensure((size_t)kMaxSplits * kHidden * 4, "bias-gradient partials")))
+        return rc;
     if ((rc = D->y.ensure((size_t)rows * kOutPad * 4, "outputs")) ||
         (rc = D->dy.ensure((size_t)rows * kOutPad * 2, "output gradients")) ||
         (rc = D->tf.ensure((size_t)rows * kOut * 4, "targets")) ||
@@ -527,6 +615,7 @@ void forward(MlpData *D, int64_t rows, bool keep, hipStream_t s)
             g.ldz = kHidden;
             g.hout = (__bf16 *)D->h[l + 1].ptr;
             g.ldh = kHidden;
+            g.hblk = keep ? (__bf16 *)D->hk[l + 1].ptr : nullptr;
             gemm<false, false, EPI_GELU>(g, s);
         } else {
             g.cout = (float *)D->y.ptr;
@@ -544,18 +633,21 @@ void backward(MlpData *D, int64_t rows, hipStream_t s)
     for (int l = kLayers - 1; l >= 0; l--) {
         const __bf16 *dz = l == kLayers - 1 ? (const __bf16 *)D->dy.ptr : (const __bf16 *)D->dz[l].ptr;
         const int64_t ldd = kOutP[l];
-        GemmArgs w{};  // dW = H^T dZ, db = column sums of dZ
-        w.a = (const __bf16 *)D->h[l].ptr;
-        w.lda = kInP[l];
-        w.b = dz;
-        w.ldb = ldd;
-        w.M = kInP[l];
-        w.N = kOutP[l];
-        w.K = (int32_t)rows;
-        w.cout = grad + D->L.w[l];
-        w.ldc = kOutP[l];
-        w.colsum = grad + D->L.b[l];
-        gemm<true, true, EPI_STORE>(w, s);
+        {
+            // dW = H^T dZ and db = column sums of dZ, split over the rows
+            const int32_t M = kInP[l], N = kOutP[l], K = (int32_t)rows;
+            int32_t splits = K / 256;
+            splits = splits < 1 ? 1 : (splits > kMaxSplits ? kMaxSplits : splits);
+            const int32_t kc = ((K + splits - 1) / splits + 31) / 32 * 32;
+            splits = (K + kc - 1) / kc;
+            const int64_t waves = (int64_t)(M / 16) * (N / 16) * splits;
+            hipLaunchKernelGGL(k_gemm_dw, dim3(blocks(waves, 4)), dim3(256), 0, s,
+                               (const __bf16 *)D->hk[l].ptr, M, (const __bf16 *)D->dzk[l].ptr, N, K, kc,
+                               splits, (float *)D->dw_part.ptr, (float *)D->cs_part.ptr);
+            hipLaunchKernelGGL(k_dw_reduce, dim3(blocks((int64_t)M * N + N)), dim3(256), 0, s,
+                               (const float *)D->dw_part.ptr, (const float *)D->cs_part.ptr, M, N, splits,
+                               grad + D->L.w[l], grad + D->L.b[l]);
+        }
         if (l == 0) break;
         GemmArgs d{};  // dH = dZ W^T, then dZ of the layer below = dH * GELU'(z)
         d.a = dz;
@@ -569,6 +661,7 @@ void backward(MlpData *D, int64_t rows, hipStream_t s)
         d.ldzi = kHidden;
         d.hout = (__bf16 *)D->dz[l - 1].ptr;
         d.ldh = kHidden;
+        d.hblk = (__bf16 *)D->dzk[l - 1].ptr;
         gemm<false, false, EPI_DGELU>(d, s);
     }
 }
@@ -577,7 +670,7 @@ void loss(MlpData *D, const float *t, int64_t rows, float *out, const int64_t *s
 {
     uint32_t nb = blocks(rows);
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, (const float *)D->y.ptr, t, rows,
-                       (__bf16 *)D->dy.ptr, (double *)D->loss_part.ptr);
+                       (__bf16 *)D->dy.ptr, (__bf16 *)D->dzk[kLayers - 1].ptr, (double *)D->loss_part.ptr);
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, (const double *)D->loss_part.ptr,
                        (int32_t)nb, rows, out, state);
 }
@@ -655,9 +748,11 @@ void mlp_release(vh_ctx *c)
     if (!D) return;
     for (DevBuf *b : {&D->master, &D->m, &D->v, &D->grad, &D->wb, &D->wtb, &D->X, &D->T,
                       &D->y, &D->dy, &D->xf, &D->tf, &D->loss_part, &D->losses, &D->state, &D->loss1,
-                      &D->pred_rows, &D->pred_groups})
+                      &D->pred_rows, &D->pred_groups, &D->dw_part, &D->cs_part})
         b->release();
     for (auto &b : D->h) b.release();
+    for (auto &b : D->hk) b.release();
+    for (auto &b : D->dzk) b.release();
     for (auto &b : D->z) b.release();
     for (auto &b : D->dz) b.release();
     delete D;
@@ -789,7 +884,7 @@ int vh_mlp_forward(vh_ctx *c, const float *x, int64_t n, float *y)
     if ((rc = ensure_acts(D, npad)) || (rc = D->xf.ensure((size_t)n * kIn * 4, "inputs"))) return rc;
     VH_HIP(hipMemcpyAsync(D->xf.ptr, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack, dim3(blocks(npad)), dim3(256), 0, s, (const float *)D->xf.ptr, n, npad,
-                       (__bf16 *)D->h[0].ptr);
+                       (__bf16 *)D->h[0].ptr, (__bf16 *)nullptr);
     {
         KernelTimer timer(c, VH_KERNEL_MLP, s);
         forward(D, npad, false, s);
@@ -821,7 +916,7 @@ int vh_mlp_grad(vh_ctx *c, const float *x, const float *t, int64_t n, float *gra
     VH_HIP(hipMemcpyAsync(D->xf.ptr, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice, s));
     VH_HIP(hipMemcpyAsync(D->tf.ptr, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack, dim3(blocks(n)), dim3(256), 0, s, (const float *)D->xf.ptr, n, n,
-                       (__bf16 *)D->h[0].ptr);
+                       (__bf16 *)D->h[0].ptr, (__bf16 *)D->hk[0].ptr);
     forward(D, n, true, s);
     loss(D, (const float *)D->tf.ptr, n, (float *)D->loss1.ptr, nullptr, s);
     backward(D, n, s);
@@ -1012,7 +1107,8 @@ int vh_mlp_train(vh_ctx *c, const vh_mlp_train_opts *o, float *epoch_loss)
     auto step = [&]() {
         hipLaunchKernelGGL(k_batch, dim3(blocks(nb)), dim3(256), 0, s, (const __bf16 *)D->X.ptr,
                            (const float *)D->T.ptr, n, (const int64_t *)state, (int32_t)nb, D->gain,
-                           1.0f / D->scale, (__bf16 *)D->h[0].ptr, (float *)D->tf.ptr);
+                           1.0f / D->scale, (__bf16 *)D->h[0].ptr, (__bf16 *)D->hk[0].ptr,
+                           (float *)D->tf.ptr);
         forward(D, nb, true, s);
         loss(D, (const float *)D->tf.ptr, nb, (float *)D->losses.ptr, state, s);
         backward(D, nb, s);
@@ -1130,7 +1226,7 @@ int vh_mlp_predict(vh_ctx *c, const float *emit0, float *start)
     if (n > 0) {
         // rows past n (padding) are zero inputs; their outputs are not read
         hipLaunchKernelGGL(k_pack, dim3(blocks(npad)), dim3(256), 0, s, (const float *)nullptr, (int64_t)0,
-                           npad, (__bf16 *)D->h[0].ptr);
+                           npad, (__bf16 *)D->h[0].ptr, (__bf16 *)nullptr);
         f.rows = (const int32_t *)idx.ptr;
         f.n = n;
         f.x = (__bf16 *)D->h[0].ptr;
