@@ -369,7 +369,59 @@ def warm_start_leg(a, log):
     for key in ("f32_zero", "f32_direct", "f32_network", "f16_zero", "f16_direct", "f16_network"):
         out[f"mean_iterations_{key}"] = float(np.mean([m[key]["iterations"] for m in out["maps"]]))
         out[f"mean_wall_s_{key}"] = float(np.mean([m[key]["wall_s"] for m in out["maps"]]))
+    model = {"params": net.params(), "gain": prior.gain, "scale": prior.scale}
     tr.dispose()
+    return out, model
+
+
+def bake_leg(a, log, model):
+    """The 200k-patch bake end to end, both forms on the bounce config's map
+    (12 x 12 rooms, 202,752 patches, 288 point lights behind room PVS rows,
+    luxel 16): scene, gather (HDR pass), patch lights, transfer build, bounce to
+    residual 1e-4, lump.
+      plain:     one gather launch per face, FP32 weights, bounce from zero;
+      optimised: the persistent gather queue, FP16 weights, bounce from the
+                 warm-start network's prediction (trained in the warm-start leg).
+    Each form's bounced light is compared with the oracle's FP64 bounce on the
+    same transfers and emission."""
+    import oracle.bounce as ob
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, rooms
+    from hl2sdk_tools_sharp_amd.bounce import HipBounceStage
+    from hl2sdk_tools_sharp_amd.vrad import light_pass
+
+    g = rooms.bounce_grid(12, 12)
+    faces = rooms.room_faces(g, luxel=16.0)
+    lights, pvs = rooms.room_lights(g, sun=False)
+    log(f"[bench] bake: {g.npatch} patches, {faces.nsamples} luxels, {len(lights)} lights")
+    kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True)
+    light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
+    plain = light_pass(g, faces, lights, pvs, schedule="per_face", weights="f32", **kw)
+    fast = light_pass(g, faces, lights, pvs, schedule="queue", weights="f16", warm_start=model, **kw)
+    out = {"patches": g.npatch, "luxels": faces.nsamples, "lights": len(lights)}
+    for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+        out[name] = {"wall_s": r.wall_s, "iterations": r.bounce.iterations,
+                     "stage_s": {k: round(v, 5) for k, v in r.stage_s.items()}}
+        log(f"[bench] bake {name}: {r.wall_s:.3f} s, {r.bounce.iterations} iterations: "
+            + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in r.stage_s.items()))
+    out["speedup"] = plain.wall_s / fast.wall_s
+    out["lump_equal"] = bool(np.array_equal(plain.lump, fast.lump))
+    if not a.no_cpu:
+        offer = HipTracerFactory(device=0).try_create(g.casters)
+        with offer.tracer as tr:
+            st = HipBounceStage(tr, g)
+            st.build_transfers()
+            rp, src, w = st.read_transfers()
+        t0 = time.perf_counter()
+        ref, _res, _en, it = ob.bounce(g, rp, src, w, plain.direct, tol=1e-6, max_iter=300, use_double=True)
+        out["fp64_oracle_s"] = time.perf_counter() - t0
+        out["fp64_oracle_iterations"] = it
+        leaf = (g.leaf & ~g.sky.astype(bool))[:, None]
+        for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+            d = np.abs((r.bounce.total_light.astype(np.float64) - ref) * leaf).sum()
+            out[name]["rel_l1_vs_fp64"] = float(d / max(np.abs(ref * leaf).sum(), 1e-300))
+        log(f"[bench] bake vs FP64 oracle ({it} iterations, {out['fp64_oracle_s']:.1f} s): plain "
+            f"{out['plain_f32']['rel_l1_vs_fp64']:.2e}, optimised {out['optimised_f16_warm']['rel_l1_vs_fp64']:.2e}")
+    log(f"[bench] bake speed-up {out['speedup']:.1f}x; lumps equal: {out['lump_equal']}")
     return out
 
 
@@ -389,6 +441,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--no-bake", action="store_true")
     ap.add_argument("--mlp-maps", type=int, default=64)
     ap.add_argument("--mlp-test-maps", type=int, default=8)
     ap.add_argument("--mlp-epochs", type=int, default=8)
@@ -520,7 +573,8 @@ def main(argv=None) -> int:
     bounce = None if a.no_bounce else bounce_leg(a, log)
     batching = None if a.no_batching else batching_leg(a, log)
     plain = None if a.no_plain else plain_vrad_leg(a, log)
-    warm = None if a.no_warm_start else warm_start_leg(a, log)
+    warm, model = (None, None) if a.no_warm_start else warm_start_leg(a, log)
+    bake = None if (a.no_bake or model is None) else bake_leg(a, log, model)
 
     out = {
         "metric": METRIC,
@@ -550,6 +604,7 @@ def main(argv=None) -> int:
         "batching": batching,
         "plain_vrad": plain,
         "warm_start": warm,
+        "bake_200k": bake,
     }
     print(json.dumps(out), flush=True)
     return 0

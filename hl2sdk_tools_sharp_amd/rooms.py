@@ -412,39 +412,40 @@ def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
     no sky surface) and so have no samples."""
     from .light import FaceSet, bump_basis
 
-    ptr = [0]
-    pos = []
-    cl = []
-    ar = []
-    nc = np.ones(g.nface, np.int32)
-    normals = np.zeros((g.nface, 4, 3), np.float32)
-    for f in range(g.nface):
-        lo, hi = g.face_lo[f].astype(np.float64), g.face_hi[f].astype(np.float64)
-        n = g.face_normal[f]
-        normals[f, 0] = n
-        if g.face_sky[f]:
-            ptr.append(ptr[-1])
-            continue
-        ax = int(np.argmax(np.abs(n)))
-        u, v = (ax + 1) % 3, (ax + 2) % 3
-        nu = max(1, int(round((hi[u] - lo[u]) / luxel)))
-        nv = max(1, int(round((hi[v] - lo[v]) / luxel)))
-        ar.append(np.full(nu * nv, (hi[u] - lo[u]) / nu * (hi[v] - lo[v]) / nv))
-        a = lo[u] + (np.arange(nu) + 0.5) * (hi[u] - lo[u]) / nu
-        b = lo[v] + (np.arange(nv) + 0.5) * (hi[v] - lo[v]) / nv
-        A, B = np.meshgrid(a, b, indexing="ij")
-        p = np.zeros((nu * nv, 3))
-        p[:, ax] = lo[ax]
-        p[:, u] = A.ravel()
-        p[:, v] = B.ravel()
-        pos.append(p)
-        cl.append(np.full(nu * nv, g.face_cluster[f], np.int32))
-        ptr.append(ptr[-1] + nu * nv)
-        if bumped_every and f % bumped_every == 0:
-            nc[f] = 4
-            normals[f, 1:] = bump_basis(n)
-    return FaceSet(np.array(ptr, np.int32), nc, normals, np.concatenate(pos).astype(np.float32),
-                   np.concatenate(cl), np.concatenate(ar).astype(np.float32))
+    F = g.nface
+    fi = np.arange(F)
+    lo = g.face_lo.astype(np.float64)
+    hi = g.face_hi.astype(np.float64)
+    n = g.face_normal
+    ax = np.argmax(np.abs(n), axis=1)
+    u, v = (ax + 1) % 3, (ax + 2) % 3
+    ext_u = hi[fi, u] - lo[fi, u]
+    ext_v = hi[fi, v] - lo[fi, v]
+    nu = np.maximum(1, np.round(ext_u / luxel).astype(np.int64))
+    nv = np.maximum(1, np.round(ext_v / luxel).astype(np.int64))
+    cnt = np.where(g.face_sky.astype(bool), 0, nu * nv)
+    ptr = np.zeros(F + 1, np.int64)
+    np.cumsum(cnt, out=ptr[1:])
+    S = int(ptr[-1])
+    fs = np.repeat(fi, cnt)  # each sample's face, in face order
+    k = np.arange(S) - ptr[fs]
+    i, j = k // nv[fs], k % nv[fs]  # cell (i, j), j fastest
+    pos = np.zeros((S, 3))
+    rows = np.arange(S)
+    pos[rows, ax[fs]] = lo[fs, ax[fs]]
+    pos[rows, u[fs]] = lo[fs, u[fs]] + (i + 0.5) * ext_u[fs] / nu[fs]
+    pos[rows, v[fs]] = lo[fs, v[fs]] + (j + 0.5) * ext_v[fs] / nv[fs]
+    area = (ext_u / nu * ext_v / nv)[fs]
+    nc = np.ones(F, np.int32)
+    normals = np.zeros((F, 4, 3), np.float32)
+    normals[:, 0] = n
+    if bumped_every:
+        for f in range(0, F, bumped_every):
+            if not g.face_sky[f]:
+                nc[f] = 4
+                normals[f, 1:] = bump_basis(n[f])
+    return FaceSet(ptr.astype(np.int32), nc, normals, pos.astype(np.float32),
+                   g.face_cluster[fs].astype(np.int32), area.astype(np.float32))
 
 
 def room_lights(g: RoomGrid, sun: bool = True, sky_ambient: bool = False, spot_every: int = 0,

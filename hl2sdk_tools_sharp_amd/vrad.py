@@ -43,10 +43,14 @@ class PassResult:
 def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bool = False,
                sun_extent: float = 0.0, sun_samples: int = 30, sky_dirs=None, ao_count: int = 0,
                ao_radius: float = 40.0, weights: str = "f32", tol: float = 1e-4, max_iter: int = 100,
-               device: int = 0, read_direct: bool = False) -> PassResult:
+               device: int = 0, read_direct: bool = False, schedule: str = "queue",
+               warm_start: Optional[dict] = None) -> PassResult:
     """Runs one pass (LDR, or HDR with ``hdr``) over room grid ``g`` whose
-    faces, in the patch system's order, are ``faces``.  Raises if the HIP
-    tracer declines (there is no CPU path)."""
+    faces, in the patch system's order, are ``faces``.  ``schedule`` is the
+    gather's ('queue', or 'per_face' for one launch per face); ``warm_start``
+    (the network's ``params``, ``gain`` and ``scale``) starts the bounce from
+    the network's prediction instead of from zero.  Raises if the HIP tracer
+    declines (there is no CPU path)."""
     out = PassResult()
     clock = time.perf_counter
 
@@ -61,7 +65,7 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
         fl = HipFaceLighter(tr)
         fl.load(faces, lights, light_pvs=light_pvs, sun_extent=sun_extent, sun_samples=sun_samples,
                 sky_dirs=sky_dirs, ao_count=ao_count, ao_radius=ao_radius)
-        fl.run("queue")
+        fl.run(schedule)
         out.stage_s["facelight"] = clock() - t
 
         t = clock()
@@ -75,7 +79,15 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
         out.stage_s["transfers"] = clock() - t
 
         t = clock()
-        out.bounce = st.bounce(None, weights=weights, tol=tol, max_iter=max_iter)
+        if warm_start is not None:
+            from .mlp import WarmStartMLP
+
+            net = WarmStartMLP(tr, params=warm_start["params"], gain=warm_start["gain"],
+                               scale=warm_start["scale"])
+            net.predict(None, read=False)
+            out.bounce = st.bounce(None, weights=weights, tol=tol, max_iter=max_iter, from_prediction=True)
+        else:
+            out.bounce = st.bounce(None, weights=weights, tol=tol, max_iter=max_iter)
         out.stage_s["bounce"] = clock() - t
 
         t = clock()
