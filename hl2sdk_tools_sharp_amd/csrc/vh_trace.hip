@@ -182,7 +182,12 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
                 }
             }
             int32_t hit = -1;
-            if (packet_any<COUNT>(nodes, tris, wstk, r, tmin, fin && !blocked, &nvisit, &ntest, &hit)) {
+            const bool want = fin && !blocked;
+            const bool any = by_octant(wave_octant(want, r), [&](auto oct) {
+                return packet_any<COUNT, decltype(oct)::value>(nodes, tris, wstk, r, tmin, want,
+                                                               &nvisit, &ntest, &hit);
+            });
+            if (any) {
                 blocked = true;
                 int c = 0;
                 while (c < kOccluders - 1 && occ[c] >= 0) c++;

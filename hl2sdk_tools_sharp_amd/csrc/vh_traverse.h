@@ -13,6 +13,8 @@
 // kMaxDepth entries deep, which the scene builder guarantees is enough.
 #pragma once
 
+#include <type_traits>
+
 #include "vh_internal.h"
 
 #pragma clang fp contract(off)
@@ -38,8 +40,10 @@ __device__ __forceinline__ bool ray_finite(const Ray &r)
 __device__ __forceinline__ float inv_saturate(float d)
 {
     // zeros become +-2^-60 with their sign kept (ReciprocalSaturateExact,
-    // KdRayTracer.cs:1446-1451); the box test tolerates an approximate inverse.
-    if (d == 0.0f) d = copysignf(kZeroSub, d);
+    // KdRayTracer.cs:1446-1451), and so do nonzero components smaller than
+    // that, so the inverse is always finite; the box test tolerates an
+    // approximate inverse.
+    if (fabsf(d) < kZeroSub) d = copysignf(kZeroSub, d);
     return __builtin_amdgcn_rcpf(d);
 }
 
@@ -187,6 +191,96 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Both children's parametric spans [tn, tf] for one lane's ray, from a node's
+// boxes: q0 = child 0's x and y slabs, q1 = child 1's, q2 = both z slabs.
+// OCT < 0 takes each slab's near and far plane by min/max.  OCT 0..7 is the
+// direction octant every lane of the wave shares (bit 0 set: x component
+// negative, bit 1: y, bit 2: z); the planes are then picked at compile time,
+// which leaves 12 of the general form's 24 min/max.  fmaf(q, ix, nox) is
+// monotonic in q for a finite inverse of fixed sign, so with lo <= hi the
+// picked plane is the one min/max would take and every span, and so every
+// lane mask, is bit-identical to the general form's.  (An empty child's box
+// is inverted, lo = +inf > hi = -inf, so its spans differ; its mask is
+// cleared by its empty ref in both forms.)
+template <int OCT>
+__device__ __forceinline__ void child_spans(const float4 q0, const float4 q1, const float4 q2,
+                                            float ix, float iy, float iz, float nox, float noy,
+                                            float noz, float tlo, float thi, float &tn0,
+                                            float &tf0, float &tn1, float &tf1)
+{
+    if constexpr (OCT < 0) {
+        const float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
+        const float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
+        const float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
+        const float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
+        const float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
+        const float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
+        tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
+        tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
+        tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
+        tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
+    } else {
+        constexpr bool nx = (OCT & 1) != 0, ny = (OCT & 2) != 0, nz = (OCT & 4) != 0;
+        const float ax_n = __builtin_fmaf(nx ? q0.y : q0.x, ix, nox);
+        const float ax_f = __builtin_fmaf(nx ? q0.x : q0.y, ix, nox);
+        const float ay_n = __builtin_fmaf(ny ? q0.w : q0.z, iy, noy);
+        const float ay_f = __builtin_fmaf(ny ? q0.z : q0.w, iy, noy);
+        const float az_n = __builtin_fmaf(nz ? q2.y : q2.x, iz, noz);
+        const float az_f = __builtin_fmaf(nz ? q2.x : q2.y, iz, noz);
+        const float bx_n = __builtin_fmaf(nx ? q1.y : q1.x, ix, nox);
+        const float bx_f = __builtin_fmaf(nx ? q1.x : q1.y, ix, nox);
+        const float by_n = __builtin_fmaf(ny ? q1.w : q1.z, iy, noy);
+        const float by_f = __builtin_fmaf(ny ? q1.z : q1.w, iy, noy);
+        const float bz_n = __builtin_fmaf(nz ? q2.w : q2.z, iz, noz);
+        const float bz_f = __builtin_fmaf(nz ? q2.z : q2.w, iz, noz);
+        tn0 = fmaxf(fmaxf(ax_n, ay_n), fmaxf(az_n, tlo));
+        tf0 = fminf(fminf(ax_f, ay_f), fminf(az_f, thi));
+        tn1 = fmaxf(fmaxf(bx_n, by_n), fmaxf(bz_n, tlo));
+        tf1 = fminf(fminf(bx_f, by_f), fminf(bz_f, thi));
+    }
+}
+
+// The direction octant shared by every lane with `want` set (the OCT of
+// child_spans), or -1 when the lanes' signs differ on some axis.  The sign of
+// a component is that of its inverse (inv_saturate keeps it).  It is also -1
+// when some lane's ray is not tame: a direction component of 2^100 or more
+// (whose inverse could be zero) or an origin component of 2^60 or more (whose
+// product with an inverse up to 2^60 could overflow).  For a tame ray an empty
+// child's inverted box gives tn = +inf and a NaN widened tf, so it fails the
+// box test by itself and the octant walks skip the empty-ref check.
+__device__ __forceinline__ int wave_octant(bool want, const Ray &r)
+{
+    const float dmax = fmaxf(fmaxf(fabsf(r.dx), fabsf(r.dy)), fabsf(r.dz));
+    const float omax = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz));
+    const bool tame = dmax < 0x1p100f && omax < 0x1p60f;
+    if (__builtin_amdgcn_ballot_w64(want && !tame)) return -1;
+    const float dx = r.dx, dy = r.dy, dz = r.dz;
+    const uint64_t w = __builtin_amdgcn_ballot_w64(want);
+    const uint64_t sx = __builtin_amdgcn_ballot_w64(want && signbit(dx));
+    const uint64_t sy = __builtin_amdgcn_ballot_w64(want && signbit(dy));
+    const uint64_t sz = __builtin_amdgcn_ballot_w64(want && signbit(dz));
+    if ((sx != 0 && sx != w) || (sy != 0 && sy != w) || (sz != 0 && sz != w)) return -1;
+    return uniform((sx != 0 ? 1 : 0) | (sy != 0 ? 2 : 0) | (sz != 0 ? 4 : 0));
+}
+
+// Calls f(std::integral_constant<int, OCT>) with the octant as a compile-time
+// constant (-1 for mixed signs), so each octant gets its own walk.
+template <typename F>
+__device__ __forceinline__ auto by_octant(int oct, F &&f)
+{
+    switch (oct) {
+    case 0: return f(std::integral_constant<int, 0>{});
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    case 7: return f(std::integral_constant<int, 7>{});
+    default: return f(std::integral_constant<int, -1>{});
+    }
+}
+
 // Any hit for the 64 rays of a wave that travel together (one light from
 // nearby starts, one sky or sun direction, one AO direction of one face).
 // Every lane of the wave calls it at once; `want` is false for a lane with no
@@ -199,7 +293,7 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
 // wave's, not the lane's: no divergence, node fetches shared by the wave.
 // The stack is the wave's (`wstk`, 3 ints a level in LDS: node, lane mask).
 // The near child is the one the mask's first lane enters first.
-template <bool COUNT>
+template <bool COUNT, int OCT = -1>
 __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, int *wstk, const Ray &r,
                                            float tmin, bool want, uint32_t *nvisit = nullptr,
@@ -208,7 +302,7 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
     const uint64_t me = 1ull << lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
     const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
-    const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
+    const float tlo = fmaxf(tmin * (1.0f - kSlack), 0.0f);  // 0 for tmin <= 0 or NaN
     const float thi = r.reach;
     uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     uint64_t done = 0;  // lanes already blocked
@@ -221,25 +315,17 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             if (COUNT && in) ++*nvisit;
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
-            float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
-            float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
-            float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
-            float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
-            float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
-            float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
-            float tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
-            float tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
-            float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
-            float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
+            float tn0, tf0, tn1, tf1;
+            child_spans<OCT>(q0, q1, q2, ix, iy, iz, nox, noy, noz, tlo, thi, tn0, tf0, tn1, tf1);
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
             // both children's tests in straight-line code, masks by ballot
             // a child's lanes: the node's lanes whose rays enter its box
             const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
-                                mask & (r0 != kEmptyRef ? ~0ull : 0ull);
+                                mask & (OCT >= 0 || r0 != kEmptyRef ? ~0ull : 0ull);
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
-                                mask & (r1 != kEmptyRef ? ~0ull : 0ull);
+                                mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
             if (m0 && m1) {
                 const uint64_t both = m0 & m1;
                 const int fl = __builtin_ctzll(both ? both : mask);
@@ -310,7 +396,7 @@ __device__ __forceinline__ bool occluder_blocks(const float4 *__restrict__ tris,
 // masks shrink as lanes find hits; ties go to the lower original triangle
 // index, so the answer is the one-ray answer.  Returns the lane's original
 // triangle index or -1, with its distance and caller id.
-template <int MODE>
+template <int MODE, int OCT = -1>
 __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nodes,
                                                   const float4 *__restrict__ tris, int *wstk,
                                                   const Ray &r, float tmin, bool want, float &t_out,
@@ -319,7 +405,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
     const uint64_t me = 1ull << lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
     const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
-    const float tlo = tmin > 0.0f ? tmin * (1.0f - kSlack) : 0.0f;
+    const float tlo = fmaxf(tmin * (1.0f - kSlack), 0.0f);  // 0 for tmin <= 0 or NaN
     float best_t = r.reach;
     int32_t best_orig = -1, best_id = -1;
     uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
@@ -331,24 +417,16 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             const float thi = best_t;
-            float ax0 = __builtin_fmaf(q0.x, ix, nox), ax1 = __builtin_fmaf(q0.y, ix, nox);
-            float ay0 = __builtin_fmaf(q0.z, iy, noy), ay1 = __builtin_fmaf(q0.w, iy, noy);
-            float az0 = __builtin_fmaf(q2.x, iz, noz), az1 = __builtin_fmaf(q2.y, iz, noz);
-            float bx0 = __builtin_fmaf(q1.x, ix, nox), bx1 = __builtin_fmaf(q1.y, ix, nox);
-            float by0 = __builtin_fmaf(q1.z, iy, noy), by1 = __builtin_fmaf(q1.w, iy, noy);
-            float bz0 = __builtin_fmaf(q2.z, iz, noz), bz1 = __builtin_fmaf(q2.w, iz, noz);
-            float tn0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tlo));
-            float tf0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), thi));
-            float tn1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tlo));
-            float tf1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), thi));
+            float tn0, tf0, tn1, tf1;
+            child_spans<OCT>(q0, q1, q2, ix, iy, iz, nox, noy, noz, tlo, thi, tn0, tf0, tn1, tf1);
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
             // a child's lanes: the node's lanes whose rays enter its box
             const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
-                                mask & (r0 != kEmptyRef ? ~0ull : 0ull);
+                                mask & (OCT >= 0 || r0 != kEmptyRef ? ~0ull : 0ull);
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
-                                mask & (r1 != kEmptyRef ? ~0ull : 0ull);
+                                mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
             if (m0 && m1) {
                 const uint64_t both = m0 & m1;
                 const int fl = __builtin_ctzll(both ? both : mask);
