@@ -523,8 +523,12 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     nv, nt = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(L.vh_count_read(h, ctypes.byref(nv), ctypes.byref(nt)))
+    ns, ts = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(L.vh_count_read_slots(h, ctypes.byref(ns), ctypes.byref(ts)))
     _lib.check(L.vh_count_enable(h, 0))
-    log(f"[bench] per ray: {nv.value / nrays:.2f} node visits, {nt.value / nrays:.2f} triangle tests")
+    node_eff = nv.value / max(ns.value, 1)
+    log(f"[bench] per ray: {nv.value / nrays:.2f} node visits, {nt.value / nrays:.2f} triangle tests; "
+        f"packet efficiency {node_eff:.3f} (node visits per lane slot)")
 
     _lib.check(L.vh_timing_enable(h, 1))
     torch.cuda.synchronize()
@@ -558,6 +562,9 @@ def main(argv=None) -> int:
         "flop_per_launch": flops,
         "node_visits_per_ray": round(nv.value / nrays, 3),
         "triangle_tests_per_ray": round(nt.value / nrays, 3),
+        "node_slots_per_ray": round(ns.value / nrays, 3),
+        "triangle_slots_per_ray": round(ts.value / nrays, 3),
+        "packet_efficiency": round(node_eff, 4),
     }
 
     gpu_bits = bits.cpu().numpy().view(np.uint64)

@@ -145,6 +145,7 @@ SIGNATURES = {
     "vh_timing_read": (ctypes.c_int, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64)]),
     "vh_count_enable": (ctypes.c_int, [P, I32]),
     "vh_count_read": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+    "vh_count_read_slots": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
 }
 
 VH_KERNEL_POINT_LIGHTS = 0

@@ -602,7 +602,7 @@ int vh_count_enable(vh_ctx *c, int32_t on)
     VH_HIP(hipSetDevice(c->device));
     int rc = c->counters.ensure(64, "work counters");
     if (rc) return rc;
-    VH_HIP(hipMemsetAsync(c->counters.ptr, 0, 16, c->stream));
+    VH_HIP(hipMemsetAsync(c->counters.ptr, 0, 32, c->stream));
     VH_HIP(hipStreamSynchronize(c->stream));
     c->counting = on != 0;
     return VH_OK;
@@ -624,6 +624,25 @@ int vh_count_read(vh_ctx *c, int64_t *node_visits, int64_t *triangle_tests)
     }
     if (node_visits) *node_visits = v[0];
     if (triangle_tests) *triangle_tests = v[1];
+    return VH_OK;
+}
+
+int vh_count_read_slots(vh_ctx *c, int64_t *node_slots, int64_t *triangle_slots)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    Guard g(c);
+    VH_HIP(hipSetDevice(c->device));
+    int64_t v[4] = {0, 0, 0, 0};
+    if (c->counters.ptr) {
+        VH_HIP(hipDeviceSynchronize());
+        VH_HIP(hipMemcpy(v, c->counters.ptr, 32, hipMemcpyDeviceToHost));
+    }
+    if (node_slots) *node_slots = v[2];
+    if (triangle_slots) *triangle_slots = v[3];
     return VH_OK;
 }
 

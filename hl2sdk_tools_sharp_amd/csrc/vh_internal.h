@@ -84,7 +84,8 @@ struct vh_ctx {
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[VH_KERNEL_COUNT];
     std::vector<hipEvent_t> event_pool;
-    // work counters (vh_count_*): [node visits, triangle tests]
+    // work counters (vh_count_*): [node visits, triangle tests, node slots,
+    // triangle slots]
     bool counting = false;
     vh::DevBuf counters;
     // patches, transfers and bounce state (vh_bounce.hip)

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
     extern __shared__ int stack_mem[];
     int *wstk = stack_mem + (threadIdx.x >> 6) * 3 * depth;
     const int64_t slot = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    uint32_t nvisit = 0, ntest = 0;
+    uint32_t nvisit = 0, ntest = 0, slots[2] = {0, 0};
     const bool have = slot < npoints;
     const int64_t p = have ? (order ? order[slot] : slot) : 0;
     const float px = have ? pts[p * 3 + 0] : 0.0f, py = have ? pts[p * 3 + 1] : 0.0f,
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
             const bool want = fin && !blocked;
             const bool any = by_octant(wave_octant(want, r), [&](auto oct) {
                 return packet_any<COUNT, decltype(oct)::value>(nodes, tris, wstk, r, tmin, want,
-                                                               &nvisit, &ntest, &hit);
+                                                               &nvisit, &ntest, &hit, slots);
             });
             if (any) {
                 blocked = true;
@@ -211,6 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_point_lights(
     if (COUNT) {
         wave_add(counters, 0, nvisit);
         wave_add(counters, 1, ntest);
+        wave_add(counters, 2, slots[0]);
+        wave_add(counters, 3, slots[1]);
     }
 }
 

@@ -76,6 +76,32 @@ __device__ __forceinline__ bool tri_test(const float4 t0, const float4 t1, const
            (b0 >= kParallelEps) && (b1 >= kParallelEps) && ((b1 + b0) <= 1.0f);
 }
 
+// tri_test's answer with every condition evaluated (no short circuit), for
+// the packet walks, where all lanes run the test together anyway: the same
+// operations in the same order, so the same t and the same answer.
+__device__ __forceinline__ bool tri_test_flat(const float4 t0, const float4 t1, const float4 t2,
+                                              uint32_t meta, const Ray &r, float &t)
+{
+    float ddotn = r.dx * t0.x;
+    ddotn = (r.dy * t0.y) + ddotn;
+    ddotn = (r.dz * t0.z) + ddotn;
+    float odotn = r.ox * t0.x;
+    odotn = (r.oy * t0.y) + odotn;
+    odotn = (r.oz * t0.z) + odotn;
+    t = (t0.w - odotn) / ddotn;
+    int cs0 = meta & 3, cs1 = (meta >> 2) & 3;
+    float h1 = sel3(cs0, r.ox, r.oy, r.oz) + (t * sel3(cs0, r.dx, r.dy, r.dz));
+    float h2 = sel3(cs1, r.ox, r.oy, r.oz) + (t * sel3(cs1, r.dx, r.dy, r.dz));
+    float b0 = t1.x * h1;
+    b0 = b0 + (t1.y * h2);
+    b0 = b0 + t1.z;
+    float b1 = t1.w * h1;
+    b1 = b1 + (t2.x * h2);
+    b1 = b1 + t2.y;
+    return ((ddotn > kParallelEps) | (ddotn < -kParallelEps)) & (t > kParallelEps) &
+           (b0 >= kParallelEps) & (b1 >= kParallelEps) & ((b1 + b0) <= 1.0f);
+}
+
 // One ray through the BVH.  MODE kAny returns at the first hit strictly short
 // of the reach; the closest modes keep the nearest (ties to the lower
 // original triangle index).  Returns the original triangle index or -1.
@@ -297,28 +323,34 @@ template <bool COUNT, int OCT = -1>
 __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, int *wstk, const Ray &r,
                                            float tmin, bool want, uint32_t *nvisit = nullptr,
-                                           uint32_t *ntest = nullptr, int32_t *blocker = nullptr)
+                                           uint32_t *ntest = nullptr, int32_t *blocker = nullptr,
+                                           uint32_t *slots = nullptr)
 {
-    const uint64_t me = 1ull << lane_id();
+    const int lane = lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
     const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
     const float tlo = fmaxf(tmin * (1.0f - kSlack), 0.0f);  // 0 for tmin <= 0 or NaN
     const float thi = r.reach;
     uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     uint64_t done = 0;  // lanes already blocked
-    bool blocked = false;
+    // Per-lane state is kept in integers and every lane runs every test, with
+    // the lane masks applied arithmetically: a wave's VALU instruction costs
+    // the same whatever lanes are on, and divergent branches would cost
+    // scalar exec-mask bookkeeping on every visit.
+    uint32_t blk = 0;  // this lane's ray is blocked
     int sp = 0;
     int32_t ref = 0;
-    while (mask) {
+    while (mask != 0) {
         if (ref >= 0) {
-            const bool in = (mask & me) != 0;
-            if (COUNT && in) ++*nvisit;
+            if (COUNT) {
+                *nvisit += (uint32_t)(mask >> lane) & 1u;
+                if (slots) ++slots[0];  // the visit occupies the whole wave
+            }
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             float tn0, tf0, tn1, tf1;
             child_spans<OCT>(q0, q1, q2, ix, iy, iz, nox, noy, noz, tlo, thi, tn0, tf0, tn1, tf1);
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
-            // both children's tests in straight-line code, masks by ballot
             // a child's lanes: the node's lanes whose rays enter its box
             const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
@@ -326,9 +358,22 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
                                 mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
-            if (m0 && m1) {
+            // one uniform branch per case
+            if (m0 == 0) {
+                if (m1 != 0) {
+                    ref = r1;
+                    mask = m1;
+                    continue;
+                }
+            } else if (m1 == 0) {
+                ref = r0;
+                mask = m0;
+                continue;
+            } else {
+                // both: the near child is the one the first lane of both
+                // masks (else of the node's mask) enters first
                 const uint64_t both = m0 & m1;
-                const int fl = __builtin_ctzll(both ? both : mask);
+                const int fl = __builtin_ctzll(both != 0 ? both : mask);
                 const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
                 const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
                 const bool swap = n1 < n0;
@@ -342,28 +387,27 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                 mask = swap ? m1 : m0;
                 continue;
             }
-            if (m0 | m1) {
-                ref = m0 ? r0 : r1;
-                mask = m0 | m1;
-                continue;
-            }
         } else {
             // the leaf's triangles, fetched once for the wave; each lane of the
             // mask tests them until its ray is blocked
-            const bool in = (mask & me) != 0;
+            const uint32_t act = (uint32_t)(mask >> lane) & 1u;
             for (int32_t k = ~ref;; k++) {
                 const float4 *tp = tris + (size_t)k * 3;
                 const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                 const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
-                if (in && !blocked) {
-                    if (COUNT) ++*ntest;
-                    float t;
-                    blocked = tri_test(t0, t1, t2, meta, r, t) && t >= tmin && t < r.reach;
-                    if (blocked && blocker) *blocker = k;
+                const uint32_t live = act & (blk ^ 1u);
+                if (COUNT) {
+                    if (slots) ++slots[1];
+                    *ntest += live;
                 }
+                float t;
+                const bool hit = tri_test_flat(t0, t1, t2, meta, r, t) & (t >= tmin) & (t < r.reach);
+                const uint32_t take = live & (uint32_t)hit;
+                if (blocker) *blocker = take ? k : *blocker;
+                blk |= take;
                 if (meta & kTriLast) break;
             }
-            done = uniform64(__builtin_amdgcn_ballot_w64(blocked));
+            done = uniform64(__builtin_amdgcn_ballot_w64(blk != 0));
         }
         // the next pending subtree that still has unblocked lanes
         mask = 0;
@@ -373,10 +417,10 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             const uint64_t lo = (uint32_t)uniform(wstk[3 * sp + 1]);
             const uint64_t hi = (uint32_t)uniform(wstk[3 * sp + 2]);
             mask = ((hi << 32) | lo) & ~done;
-            if (mask) break;
+            if (mask != 0) break;
         }
     }
-    return blocked;
+    return blk != 0;
 }
 
 // Whether triangle k (leaf order) blocks the ray strictly short of its reach:

@@ -406,6 +406,11 @@ int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launc
  * tests; the roofline's algorithmic FLOPs are computed from them. */
 int vh_count_enable(vh_ctx *ctx, int32_t on);
 int vh_count_read(vh_ctx *ctx, int64_t *node_visits, int64_t *triangle_tests);
+/* The lane slots the point-light packet walks occupied while counting: 64 per
+ * node visit or triangle fetch of a wave, whatever lanes took part, so
+ * node_visits / node_slots is the walks' packet efficiency (no counterpart in
+ * the reference). */
+int vh_count_read_slots(vh_ctx *ctx, int64_t *node_slots, int64_t *triangle_slots);
 
 #ifdef __cplusplus
 }
