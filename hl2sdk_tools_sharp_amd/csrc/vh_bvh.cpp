@@ -354,6 +354,21 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
         int32_t r1 = (bn == root && root_leaf) ? kEmptyRef : ref_of(c1);
         std::memcpy(&o[12], &r0, 4);
         std::memcpy(&o[13], &r1, 4);
+        // the children's order along the axis their centres differ most on:
+        // bits 0-1 the axis, bit 2 set when child 0's centre is the higher
+        int32_t ord = 0;
+        if (!(bn == root && root_leaf)) {
+            float best = -1.0f;
+            for (int k = 0; k < 3; k++) {
+                float c0 = 0.5f * (b0.lo[k] + b0.hi[k]), c1 = 0.5f * (b1.lo[k] + b1.hi[k]);
+                float dk = std::fabs(c1 - c0);
+                if (dk > best) {
+                    best = dk;
+                    ord = k | (c0 > c1 ? 4 : 0);
+                }
+            }
+        }
+        std::memcpy(&o[14], &ord, 4);
     }
     out.ntri = n;
     out.depth = b.max_depth + (root_leaf ? 1 : 0);

@@ -370,13 +370,21 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                 mask = m0;
                 continue;
             } else {
-                // both: the near child is the one the first lane of both
-                // masks (else of the node's mask) enters first
-                const uint64_t both = m0 & m1;
-                const int fl = __builtin_ctzll(both != 0 ? both : mask);
-                const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
-                const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
-                const bool swap = n1 < n0;
+                bool swap;
+                if constexpr (OCT >= 0) {
+                    // the wave's shared direction picks the near child along
+                    // the node's separating axis
+                    const int32_t ord = uniform(__float_as_int(q3.z));
+                    swap = (((OCT >> (ord & 3)) ^ (ord >> 2)) & 1) != 0;
+                } else {
+                    // both: the near child is the one the first lane of both
+                    // masks (else of the node's mask) enters first
+                    const uint64_t both = m0 & m1;
+                    const int fl = __builtin_ctzll(both != 0 ? both : mask);
+                    const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
+                    const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
+                    swap = n1 < n0;
+                }
                 const int32_t far = swap ? r0 : r1;
                 const uint64_t fm = swap ? m0 : m1;
                 wstk[3 * sp] = far;
@@ -446,7 +454,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
                                                   const Ray &r, float tmin, bool want, float &t_out,
                                                   int32_t &id_out)
 {
-    const uint64_t me = 1ull << lane_id();
+    const int lane = lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
     const float nox = -(r.ox * ix), noy = -(r.oy * iy), noz = -(r.oz * iz);
     const float tlo = fmaxf(tmin * (1.0f - kSlack), 0.0f);  // 0 for tmin <= 0 or NaN
@@ -455,9 +463,8 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
     uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     int sp = 0;
     int32_t ref = 0;
-    while (mask) {
+    while (mask != 0) {
         if (ref >= 0) {
-            const bool in = (mask & me) != 0;
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             const float thi = best_t;
@@ -471,9 +478,19 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
                                 mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
-            if (m0 && m1) {
+            if (m0 == 0) {
+                if (m1 != 0) {
+                    ref = r1;
+                    mask = m1;
+                    continue;
+                }
+            } else if (m1 == 0) {
+                ref = r0;
+                mask = m0;
+                continue;
+            } else {
                 const uint64_t both = m0 & m1;
-                const int fl = __builtin_ctzll(both ? both : mask);
+                const int fl = __builtin_ctzll(both != 0 ? both : mask);
                 const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
                 const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
                 const bool swap = n1 < n0;
@@ -487,32 +504,25 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
                 mask = swap ? m1 : m0;
                 continue;
             }
-            if (m0 | m1) {
-                ref = m0 ? r0 : r1;
-                mask = m0 | m1;
-                continue;
-            }
         } else {
-            const bool in = (mask & me) != 0;
+            // every lane runs the test; the mask and the order rule are
+            // applied by selects (see packet_any)
+            const bool act = ((mask >> lane) & 1ull) != 0;
             for (int32_t k = ~ref;; k++) {
                 const float4 *tp = tris + (size_t)k * 3;
                 const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                 const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
-                if (in) {
-                    float t;
-                    if (tri_test(t0, t1, t2, meta, r, t) && t >= tmin) {
-                        const int32_t orig = (int32_t)(meta >> kOrigShift);
-                        const bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);
-                        const bool better = best_orig < 0
-                                                ? within
-                                                : (t < best_t || (t == best_t && orig < best_orig));
-                        if (better) {
-                            best_t = t;
-                            best_orig = orig;
-                            best_id = __float_as_int(t2.z);
-                        }
-                    }
-                }
+                const int32_t orig = (int32_t)(meta >> kOrigShift);
+                float t;
+                const bool h = tri_test_flat(t0, t1, t2, meta, r, t) & (t >= tmin);
+                const bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);
+                const bool better = best_orig < 0
+                                        ? within
+                                        : ((t < best_t) | ((t == best_t) & (orig < best_orig)));
+                const bool take = act & h & better;
+                best_t = take ? t : best_t;
+                best_orig = take ? orig : best_orig;
+                best_id = take ? __float_as_int(t2.z) : best_id;
                 if (meta & kTriLast) break;
             }
         }
@@ -523,7 +533,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             const uint64_t lo = (uint32_t)uniform(wstk[3 * sp + 1]);
             const uint64_t hi = (uint32_t)uniform(wstk[3 * sp + 2]);
             mask = (hi << 32) | lo;
-            if (mask) break;
+            if (mask != 0) break;
         }
     }
     t_out = best_t;
