@@ -167,12 +167,14 @@ __global__ __launch_bounds__(kTB) void k_transfers(PatchDev pd, const int32_t *_
                                                    float *__restrict__ st_w,
                                                    int64_t *__restrict__ made,
                                                    unsigned long long *__restrict__ stats,
-                                                   int *__restrict__ err)
+                                                   int *__restrict__ err, int32_t tdepth,
+                                                   int32_t pdepth)
 {
-    __shared__ int trav_mem[kMaxDepth * kTB];
-    __shared__ int pstk_mem[kPatchStack * kTB];
-    int *trav = trav_mem + threadIdx.x;
-    int *pstk = pstk_mem + threadIdx.x;
+    // the lanes' stacks, [depth][lane]: traversal (tdepth rows, MAKE only),
+    // then the patch descent (pdepth rows)
+    extern __shared__ int stacks[];
+    int *trav = stacks + threadIdx.x;
+    int *pstk = stacks + (MAKE ? tdepth * kTB : 0) + threadIdx.x;
     int64_t k = (int64_t)blockIdx.x * kTB + threadIdx.x;
     uint32_t nblocked = 0;
     if (k < nrecv) {
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(kTB) void k_transfers(PatchDev pd, const int32_t *_
                     if (ch.x != -1) {
                         float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
                         if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
-                            if (sp + 2 > kPatchStack) {
+                            if (sp + 2 > pdepth) {
                                 atomicOr(err, 1);
                                 continue;
                             }
@@ -910,12 +912,16 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     VH_HIP(hipMemsetAsync(stats.ptr, 0, 16, s));
     VH_HIP(hipMemsetAsync(err.ptr, 0, 16, s));
     uint32_t grid = (uint32_t)((R + kTB - 1) / kTB);
+    // stack rows: the tree's depth bounds a traversal stack; a patch descent
+    // holds at most height + 1 entries
+    const int32_t tdepth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;
+    const int32_t pdepth = B->height + 2 < kPatchStack ? B->height + 2 : kPatchStack;
     KernelTimer timer(c, VH_KERNEL_TRANSFERS, s);
     if (R > 0)
-        hipLaunchKernelGGL(k_transfers<false>, dim3(grid), dim3(kTB), 0, s, pd,
+        hipLaunchKernelGGL(k_transfers<false>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
                            (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
                            (const float4 *)c->tris.ptr, (int64_t *)counts.ptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, (int *)err.ptr);
+                           nullptr, nullptr, nullptr, (int *)err.ptr, 0, pdepth);
     VH_HIP(hipGetLastError());
     std::vector<int64_t> hcount(R + 1, 0), hoff(R + 1, 0);
     if (R > 0)
@@ -930,11 +936,11 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
         return rc;
     }
     if (R > 0)
-        hipLaunchKernelGGL(k_transfers<true>, dim3(grid), dim3(kTB), 0, s, pd,
+        hipLaunchKernelGGL(k_transfers<true>, dim3(grid), dim3(kTB), (size_t)(tdepth + pdepth) * kTB * 4, s, pd,
                            (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
                            (const float4 *)c->tris.ptr, nullptr, (const int64_t *)cand_off.ptr,
                            (int32_t *)st_src.ptr, (float *)st_w.ptr, (int64_t *)made.ptr,
-                           (unsigned long long *)stats.ptr, (int *)err.ptr);
+                           (unsigned long long *)stats.ptr, (int *)err.ptr, tdepth, pdepth);
     timer.done();
     VH_HIP(hipGetLastError());
     std::vector<int64_t> hmade(R + 1, 0);
