@@ -9,6 +9,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
+#include <system_error>
+#include <thread>
 
 #include "vh_internal.h"
 
@@ -99,16 +102,53 @@ struct Box {
     }
 };
 
+// A triangle as the build sees it: bounds, centroid, index.  The build
+// partitions these in place, so each node's scans read memory in order.
+struct Prim {
+    Box box;
+    float c[3];
+    int32_t id;
+};
+
 struct BNode {
     Box box;
     int32_t left = -1, right = -1;  // build-node indices, -1 for a leaf
     int32_t first = 0, count = 0;   // leaf range in the order array
 };
 
+// Host threads for the build: at most 16, and what the host has.
+unsigned build_threads()
+{
+    return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+// f(begin, end) over [0, n) in contiguous chunks, one per thread; serially
+// when no thread can be had.  Chunks write disjoint outputs.
+template <class F>
+void parallel_chunks(int64_t n, F &&f)
+{
+    const int64_t nt = std::min<int64_t>(build_threads(), std::max<int64_t>(1, n / 4096));
+    std::vector<std::thread> pool;
+    int64_t done = 0;  // chunks [0, done) run on threads
+    try {
+        for (int64_t t = 1; t < nt; t++) {
+            pool.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+            done = t;
+        }
+    } catch (const std::system_error &) {
+    }
+    f(0, n / nt);
+    for (int64_t t = done + 1; t < nt; t++) f(n * t / nt, n * (t + 1) / nt);
+    for (auto &th : pool) th.join();
+}
+
+// Binned SAH, top-down.  The upper levels fork: a node's right subtree is
+// built on another thread into its own node list and appended after the
+// left subtree, so the nodes come out in the serial build's preorder and the
+// tree is the serial build's whatever the timing.  Subtrees own disjoint
+// ranges of prims, which is all they write.
 struct Builder {
-    std::vector<Box> tb;
-    std::vector<float> cen;  // 3 per triangle
-    std::vector<int32_t> idx;
+    std::vector<Prim> &prims;
     std::vector<BNode> nodes;
     int max_depth = 0;
 
@@ -116,8 +156,12 @@ struct Builder {
     static constexpr float kTraverse = 1.0f;
     static constexpr float kIntersect = 1.5f;
     static constexpr int kLeafMax = 8;
+    static constexpr int32_t kForkMin = 8192;  // smallest subtree worth a thread
 
-    int32_t build(int32_t first, int32_t count, int depth)
+    explicit Builder(std::vector<Prim> &p) : prims(p) {}
+
+    // fork > 0: this many more levels may hand their right subtree to a thread
+    int32_t build(int32_t first, int32_t count, int depth, int fork)
     {
         int32_t me = (int32_t)nodes.size();
         nodes.emplace_back();
@@ -125,9 +169,9 @@ struct Builder {
         b.reset();
         cb.reset();
         for (int32_t i = first; i < first + count; i++) {
-            b.grow(tb[idx[i]]);
+            b.grow(prims[i].box);
             Box c;
-            for (int k = 0; k < 3; k++) c.lo[k] = c.hi[k] = cen[3 * idx[i] + k];
+            for (int k = 0; k < 3; k++) c.lo[k] = c.hi[k] = prims[i].c[k];
             cb.grow(c);
         }
         nodes[me].box = b;
@@ -136,26 +180,36 @@ struct Builder {
         int best_axis = -1, best_bin = 0;
         float best_cost = INFINITY;
         if (count > 1 && depth < kMaxDepth) {
+            // one pass bins the centroids along all three axes
+            float ext[3];
+            bool live[3];
+            int cnt[3][kBins] = {};
+            Box bb[3][kBins];
             for (int ax = 0; ax < 3; ax++) {
-                float ext = cb.hi[ax] - cb.lo[ax];
-                if (!(ext > 0)) continue;
-                int cnt[kBins] = {0};
-                Box bb[kBins];
-                for (auto &x : bb) x.reset();
-                for (int32_t i = first; i < first + count; i++) {
-                    int bin = (int)((cen[3 * idx[i] + ax] - cb.lo[ax]) / ext * kBins);
+                ext[ax] = cb.hi[ax] - cb.lo[ax];
+                live[ax] = ext[ax] > 0;
+                for (auto &x : bb[ax]) x.reset();
+            }
+            for (int32_t i = first; i < first + count; i++) {
+                const Prim &t = prims[i];
+                for (int ax = 0; ax < 3; ax++) {
+                    if (!live[ax]) continue;
+                    int bin = (int)((t.c[ax] - cb.lo[ax]) / ext[ax] * kBins);
                     bin = std::min(std::max(bin, 0), kBins - 1);
-                    cnt[bin]++;
-                    bb[bin].grow(tb[idx[i]]);
+                    cnt[ax][bin]++;
+                    bb[ax][bin].grow(t.box);
                 }
+            }
+            for (int ax = 0; ax < 3; ax++) {
+                if (!live[ax]) continue;
                 float rarea[kBins];
                 int rcnt[kBins];
                 Box acc;
                 acc.reset();
                 int n = 0;
                 for (int bin = kBins - 1; bin > 0; bin--) {
-                    acc.grow(bb[bin]);
-                    n += cnt[bin];
+                    acc.grow(bb[ax][bin]);
+                    n += cnt[ax][bin];
                     rarea[bin] = acc.area();
                     rcnt[bin] = n;
                 }
@@ -163,8 +217,8 @@ struct Builder {
                 n = 0;
                 float parea = b.area();
                 for (int bin = 0; bin < kBins - 1; bin++) {
-                    acc.grow(bb[bin]);
-                    n += cnt[bin];
+                    acc.grow(bb[ax][bin]);
+                    n += cnt[ax][bin];
                     if (n == 0 || rcnt[bin + 1] == 0) continue;
                     float cost = kTraverse + kIntersect *
                                                  (acc.area() * n + rarea[bin + 1] * rcnt[bin + 1]) /
@@ -189,19 +243,46 @@ struct Builder {
             mid = first + count / 2;  // coincident centroids: split the list
         } else {
             float ext = cb.hi[best_axis] - cb.lo[best_axis];
-            auto it = std::partition(idx.begin() + first, idx.begin() + first + count,
-                                     [&](int32_t t) {
-                                         int bin = (int)((cen[3 * t + best_axis] -
-                                                          cb.lo[best_axis]) /
+            auto it = std::partition(prims.begin() + first, prims.begin() + first + count,
+                                     [&](const Prim &t) {
+                                         int bin = (int)((t.c[best_axis] - cb.lo[best_axis]) /
                                                          ext * kBins);
                                          bin = std::min(std::max(bin, 0), kBins - 1);
                                          return bin <= best_bin;
                                      });
-            mid = (int32_t)(it - idx.begin());
+            mid = (int32_t)(it - prims.begin());
             if (mid == first || mid == first + count) mid = first + count / 2;
         }
-        int32_t l = build(first, mid - first, depth + 1);
-        int32_t r = build(mid, first + count - mid, depth + 1);
+        const int32_t rcount = first + count - mid;
+        if (fork > 0 && rcount >= kForkMin && mid - first >= kForkMin) {
+            Builder rb(prims);
+            std::future<int32_t> right;
+            try {
+                right = std::async(std::launch::async, [&rb, mid, rcount, depth, fork]() {
+                    return rb.build(mid, rcount, depth + 1, fork - 1);
+                });
+            } catch (const std::system_error &) {
+                fork = 0;  // no thread to be had: this subtree builds serially
+            }
+            if (fork > 0) {
+                int32_t l = build(first, mid - first, depth + 1, fork - 1);
+                int32_t r = right.get();
+                const int32_t off = (int32_t)nodes.size();
+                for (BNode nd : rb.nodes) {
+                    if (nd.left >= 0) {
+                        nd.left += off;
+                        nd.right += off;
+                    }
+                    nodes.push_back(nd);
+                }
+                nodes[me].left = l;
+                nodes[me].right = r + off;
+                max_depth = std::max(max_depth, rb.max_depth);
+                return me;
+            }
+        }
+        int32_t l = build(first, mid - first, depth + 1, fork);
+        int32_t r = build(mid, rcount, depth + 1, fork);
         nodes[me].left = l;
         nodes[me].right = r;
         return me;
@@ -222,10 +303,8 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
         set_error("too many triangles for one scene (limit 2^27)");
         return VH_EINVAL;
     }
-    Builder b;
-    b.tb.resize(n);
-    b.cen.resize((size_t)n * 3);
-    b.idx.resize(n);
+    std::vector<Prim> prims(n);
+    Builder b(prims);
     float m = 0.0f;
     for (int32_t i = 0; i < n; i++) {
         const float *v = v9 + (size_t)i * 9;
@@ -235,15 +314,20 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
                 set_error("triangle " + std::to_string(i) + " has a non-finite vertex");
                 return VH_EINVAL;
             }
-            b.tb[i].lo[k] = std::min(a, std::min(c, e));
-            b.tb[i].hi[k] = std::max(a, std::max(c, e));
-            b.cen[3 * (size_t)i + k] = 0.5f * (b.tb[i].lo[k] + b.tb[i].hi[k]);
-            m = std::max(m, std::max(std::fabs(b.tb[i].lo[k]), std::fabs(b.tb[i].hi[k])));
+            Prim &q = prims[i];
+            q.box.lo[k] = std::min(a, std::min(c, e));
+            q.box.hi[k] = std::max(a, std::max(c, e));
+            q.c[k] = 0.5f * (q.box.lo[k] + q.box.hi[k]);
+            m = std::max(m, std::max(std::fabs(q.box.lo[k]), std::fabs(q.box.hi[k])));
         }
-        b.idx[i] = i;
+        prims[i].id = i;
     }
     b.nodes.reserve((size_t)n * 2);
-    b.build(0, n, 0);
+    // up to 16 threads at the fourth level, fewer on a smaller host
+    const unsigned hw = build_threads();
+    int fork = 0;
+    while ((2u << fork) <= hw) fork++;
+    b.build(0, n, 0, fork);
 
     // Widening: the exact test's hit point carries a few ulps of the scene's
     // coordinate magnitude; 2^-16 of it is far beyond that.
@@ -267,36 +351,42 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
             queue.push_back(b.nodes[bn].right);
         }
     }
-    // triangles in leaf order
+    // triangles in leaf order: each leaf's start in a serial pass, then the
+    // conversions, which are independent, in parallel
     out.tris.assign((size_t)n * 12, 0.0f);
     std::vector<int32_t> leaf_start(b.nodes.size(), -1);
+    std::vector<int32_t> leaves;
     int32_t cursor = 0;
-    auto emit_leaf = [&](int32_t bn) {
-        const BNode &L = b.nodes[bn];
-        leaf_start[bn] = cursor;
-        for (int32_t i = 0; i < L.count; i++) {
-            int32_t orig = b.idx[L.first + i];
-            Tri48 t = convert(v9 + (size_t)orig * 9, ids ? ids[orig] : orig);
-            float *o = &out.tris[(size_t)cursor * 12];
-            o[0] = t.n[0];
-            o[1] = t.n[1];
-            o[2] = t.n[2];
-            o[3] = t.d;
-            o[4] = t.e[0];
-            o[5] = t.e[1];
-            o[6] = t.e[2];
-            o[7] = t.e[3];
-            o[8] = t.e[4];
-            o[9] = t.e[5];
-            std::memcpy(&o[10], &t.id, 4);
-            uint32_t meta = (uint32_t)t.cs0 | ((uint32_t)t.cs1 << 2) |
-                            (i == L.count - 1 ? kTriLast : 0u) | ((uint32_t)orig << kOrigShift);
-            std::memcpy(&o[11], &meta, 4);
-            cursor++;
-        }
-    };
     for (size_t bn = 0; bn < b.nodes.size(); bn++)
-        if (b.nodes[bn].left < 0 && b.nodes[bn].count > 0) emit_leaf((int32_t)bn);
+        if (b.nodes[bn].left < 0 && b.nodes[bn].count > 0) {
+            leaf_start[bn] = cursor;
+            cursor += b.nodes[bn].count;
+            leaves.push_back((int32_t)bn);
+        }
+    parallel_chunks((int64_t)leaves.size(), [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+            const BNode &L = b.nodes[leaves[q]];
+            for (int32_t i = 0; i < L.count; i++) {
+                int32_t orig = prims[L.first + i].id;
+                Tri48 t = convert(v9 + (size_t)orig * 9, ids ? ids[orig] : orig);
+                float *o = &out.tris[(size_t)(leaf_start[leaves[q]] + i) * 12];
+                o[0] = t.n[0];
+                o[1] = t.n[1];
+                o[2] = t.n[2];
+                o[3] = t.d;
+                o[4] = t.e[0];
+                o[5] = t.e[1];
+                o[6] = t.e[2];
+                o[7] = t.e[3];
+                o[8] = t.e[4];
+                o[9] = t.e[5];
+                std::memcpy(&o[10], &t.id, 4);
+                uint32_t meta = (uint32_t)t.cs0 | ((uint32_t)t.cs1 << 2) |
+                                (i == L.count - 1 ? kTriLast : 0u) | ((uint32_t)orig << kOrigShift);
+                std::memcpy(&o[11], &meta, 4);
+            }
+        }
+    });
 
     auto ref_of = [&](int32_t bn) -> int32_t {
         if (bn < 0) return kEmptyRef;
@@ -306,7 +396,9 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
     };
     out.nnodes = (int32_t)internal.size();
     out.nodes.assign((size_t)out.nnodes * 16, 0.0f);
-    for (int32_t g = 0; g < out.nnodes; g++) {
+    // each GPU node is written from its build node alone
+    parallel_chunks(out.nnodes, [&](int64_t glo, int64_t ghi) {
+    for (int64_t g = glo; g < ghi; g++) {
         int32_t bn = internal[g];
         int32_t c0, c1;
         Box b0, b1;
@@ -370,6 +462,7 @@ int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32
         }
         std::memcpy(&o[14], &ord, 4);
     }
+    });
     out.ntri = n;
     out.depth = b.max_depth + (root_leaf ? 1 : 0);
     Box sb = b.nodes[root].box;

@@ -722,7 +722,22 @@ int leaf_rows(vh_ctx *c, const BounceView &bv, DevBuf &out, int64_t *n, hipStrea
         if (!sky[p]) rows.push_back(p);
     std::vector<int32_t> gp(1, 0);
     if (groups) {
-        std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return cl[a] < cl[b]; });
+        // stable by cluster: a counting sort when the cluster range is compact
+        int32_t lo = rows.empty() ? 0 : cl[rows[0]], hi = lo;
+        for (int32_t p : rows) {
+            lo = std::min(lo, cl[p]);
+            hi = std::max(hi, cl[p]);
+        }
+        const int64_t span = (int64_t)hi - lo + 1;
+        if (!rows.empty() && span <= 4 * (int64_t)rows.size() + 1024) {
+            std::vector<int32_t> start((size_t)span + 1, 0), sorted(rows.size());
+            for (int32_t p : rows) start[(size_t)(cl[p] - lo) + 1]++;
+            for (int64_t k = 0; k < span; k++) start[k + 1] += start[k];
+            for (int32_t p : rows) sorted[start[(size_t)(cl[p] - lo)]++] = p;
+            rows.swap(sorted);
+        } else {
+            std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return cl[a] < cl[b]; });
+        }
         for (size_t i = 1; i <= rows.size(); i++)
             if (i == rows.size() || cl[rows[i]] != cl[rows[i - 1]]) gp.push_back((int32_t)i);
         if (rows.empty()) gp.assign(1, 0);
