@@ -152,7 +152,7 @@ struct Builder {
     std::vector<BNode> nodes;
     int max_depth = 0;
 
-    static constexpr int kBins = 16;
+    static constexpr int kBins = 64;
     static constexpr float kTraverse = 1.0f;
     static constexpr float kIntersect = 1.5f;
     static constexpr int kLeafMax = 8;

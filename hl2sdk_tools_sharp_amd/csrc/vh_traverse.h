@@ -102,6 +102,49 @@ __device__ __forceinline__ bool tri_test_flat(const float4 t0, const float4 t1, 
            (b0 >= kParallelEps) & (b1 >= kParallelEps) & ((b1 + b0) <= 1.0f);
 }
 
+// tri_test_flat with the two kept axes fixed at compile time (C0, C1 are the
+// triangle's cs0, cs1): the same operations on the same operands, without the
+// per-lane selects.
+template <int C0, int C1>
+__device__ __forceinline__ bool tri_test_axes(const float4 t0, const float4 t1, const float4 t2,
+                                              const Ray &r, float &t)
+{
+    float ddotn = r.dx * t0.x;
+    ddotn = (r.dy * t0.y) + ddotn;
+    ddotn = (r.dz * t0.z) + ddotn;
+    float odotn = r.ox * t0.x;
+    odotn = (r.oy * t0.y) + odotn;
+    odotn = (r.oz * t0.z) + odotn;
+    t = (t0.w - odotn) / ddotn;
+    const float o0 = C0 == 0 ? r.ox : (C0 == 1 ? r.oy : r.oz);
+    const float d0 = C0 == 0 ? r.dx : (C0 == 1 ? r.dy : r.dz);
+    const float o1 = C1 == 0 ? r.ox : (C1 == 1 ? r.oy : r.oz);
+    const float d1 = C1 == 0 ? r.dx : (C1 == 1 ? r.dy : r.dz);
+    float h1 = o0 + (t * d0);
+    float h2 = o1 + (t * d1);
+    float b0 = t1.x * h1;
+    b0 = b0 + (t1.y * h2);
+    b0 = b0 + t1.z;
+    float b1 = t1.w * h1;
+    b1 = b1 + (t2.x * h2);
+    b1 = b1 + t2.y;
+    return ((ddotn > kParallelEps) | (ddotn < -kParallelEps)) & (t > kParallelEps) &
+           (b0 >= kParallelEps) & (b1 >= kParallelEps) & ((b1 + b0) <= 1.0f);
+}
+
+// The exact test for a wave-uniform triangle: the scene builder only makes
+// (cs0, cs1) = (1, 2), (2, 0) or (0, 1), each taken by a uniform branch.
+__device__ __forceinline__ bool tri_test_uniform(const float4 t0, const float4 t1, const float4 t2,
+                                                 uint32_t meta, const Ray &r, float &t)
+{
+    switch (meta & 15u) {
+    case 1u | (2u << 2): return tri_test_axes<1, 2>(t0, t1, t2, r, t);
+    case 2u | (0u << 2): return tri_test_axes<2, 0>(t0, t1, t2, r, t);
+    case 0u | (1u << 2): return tri_test_axes<0, 1>(t0, t1, t2, r, t);
+    default: return tri_test_flat(t0, t1, t2, meta, r, t);
+    }
+}
+
 // One ray through the BVH.  MODE kAny returns at the first hit strictly short
 // of the reach; the closest modes keep the nearest (ties to the lower
 // original triangle index).  Returns the original triangle index or -1.
@@ -318,7 +361,11 @@ __device__ __forceinline__ auto by_octant(int oct, F &&f)
 // stops), so the answer is the one-ray answer, while the control flow is the
 // wave's, not the lane's: no divergence, node fetches shared by the wave.
 // The stack is the wave's (`wstk`, 3 ints a level in LDS: node, lane mask).
-// The near child is the one the mask's first lane enters first.
+// The near child is, for an octant walk, the one the shared direction meets
+// first along the node's separating axis, else the one the mask's first lane
+// enters first.  The loop is "descend, then leaf, then pop", like trace_one's
+// while-while walk: the inner descent has one exit, which keeps the scalar
+// control flow of a visit short.
 template <bool COUNT, int OCT = -1>
 __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                                            const float4 *__restrict__ tris, int *wstk, const Ray &r,
@@ -340,64 +387,58 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
     uint32_t blk = 0;  // this lane's ray is blocked
     int sp = 0;
     int32_t ref = 0;
-    while (mask != 0) {
-        if (ref >= 0) {
+    if (mask == 0) return false;
+    for (;;) {
+        // descend through internal nodes until a leaf, or until no child is entered
+        while (ref >= 0) {
             if (COUNT) {
                 *nvisit += (uint32_t)(mask >> lane) & 1u;
-                if (slots) ++slots[0];  // the visit occupies the whole wave
+                if (slots) ++slots[0];
             }
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             float tn0, tf0, tn1, tf1;
             child_spans<OCT>(q0, q1, q2, ix, iy, iz, nox, noy, noz, tlo, thi, tn0, tf0, tn1, tf1);
             const int32_t r0 = uniform(__float_as_int(q3.x)), r1 = uniform(__float_as_int(q3.y));
-            // a child's lanes: the node's lanes whose rays enter its box
             const uint64_t m0 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn0 <= __builtin_fmaf(fabsf(tf0), kSlack, tf0))) &
                                 mask & (OCT >= 0 || r0 != kEmptyRef ? ~0ull : 0ull);
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
                                 mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
-            // one uniform branch per case
-            if (m0 == 0) {
+            if (m0 != 0) {
                 if (m1 != 0) {
-                    ref = r1;
-                    mask = m1;
-                    continue;
-                }
-            } else if (m1 == 0) {
-                ref = r0;
-                mask = m0;
-                continue;
-            } else {
-                bool swap;
-                if constexpr (OCT >= 0) {
-                    // the wave's shared direction picks the near child along
-                    // the node's separating axis
-                    const int32_t ord = uniform(__float_as_int(q3.z));
-                    swap = (((OCT >> (ord & 3)) ^ (ord >> 2)) & 1) != 0;
+                    bool swap;
+                    if constexpr (OCT >= 0) {
+                        const int32_t ord = uniform(__float_as_int(q3.z));
+                        swap = (((OCT >> (ord & 3)) ^ (ord >> 2)) & 1) != 0;
+                    } else {
+                        const uint64_t both = m0 & m1;
+                        const int fl = __builtin_ctzll(both != 0 ? both : mask);
+                        const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
+                        const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
+                        swap = n1 < n0;
+                    }
+                    const int32_t far = swap ? r0 : r1;
+                    const uint64_t fm = swap ? m0 : m1;
+                    wstk[3 * sp] = far;
+                    wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
+                    wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
+                    sp++;
+                    ref = swap ? r1 : r0;
+                    mask = swap ? m1 : m0;
                 } else {
-                    // both: the near child is the one the first lane of both
-                    // masks (else of the node's mask) enters first
-                    const uint64_t both = m0 & m1;
-                    const int fl = __builtin_ctzll(both != 0 ? both : mask);
-                    const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
-                    const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
-                    swap = n1 < n0;
+                    ref = r0;
+                    mask = m0;
                 }
-                const int32_t far = swap ? r0 : r1;
-                const uint64_t fm = swap ? m0 : m1;
-                wstk[3 * sp] = far;
-                wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
-                wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
-                sp++;
-                ref = swap ? r1 : r0;
-                mask = swap ? m1 : m0;
-                continue;
+            } else if (m1 != 0) {
+                ref = r1;
+                mask = m1;
+            } else {
+                ref = kEmptyRef;
             }
-        } else {
-            // the leaf's triangles, fetched once for the wave; each lane of the
-            // mask tests them until its ray is blocked
+        }
+        if (ref != kEmptyRef) {
             const uint32_t act = (uint32_t)(mask >> lane) & 1u;
             for (int32_t k = ~ref;; k++) {
                 const float4 *tp = tris + (size_t)k * 3;
@@ -409,7 +450,7 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
                     *ntest += live;
                 }
                 float t;
-                const bool hit = tri_test_flat(t0, t1, t2, meta, r, t) & (t >= tmin) & (t < r.reach);
+                const bool hit = tri_test_uniform(t0, t1, t2, meta, r, t) & (t >= tmin) & (t < r.reach);
                 const uint32_t take = live & (uint32_t)hit;
                 if (blocker) *blocker = take ? k : *blocker;
                 blk |= take;
@@ -417,7 +458,6 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             }
             done = uniform64(__builtin_amdgcn_ballot_w64(blk != 0));
         }
-        // the next pending subtree that still has unblocked lanes
         mask = 0;
         while (sp > 0) {
             sp--;
@@ -427,6 +467,7 @@ __device__ __forceinline__ bool packet_any(const float4 *__restrict__ nodes,
             mask = ((hi << 32) | lo) & ~done;
             if (mask != 0) break;
         }
+        if (mask == 0) break;
     }
     return blk != 0;
 }
