@@ -504,8 +504,14 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
     uint64_t mask = uniform64(__builtin_amdgcn_ballot_w64(want));
     int sp = 0;
     int32_t ref = 0;
-    while (mask != 0) {
-        if (ref >= 0) {
+    if (mask == 0) {
+        t_out = best_t;
+        id_out = best_id;
+        return best_orig;
+    }
+    for (;;) {
+        // descend (see packet_any)
+        while (ref >= 0) {
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             const float thi = best_t;
@@ -519,33 +525,33 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             const uint64_t m1 = uniform64(__builtin_amdgcn_ballot_w64(
                                     tn1 <= __builtin_fmaf(fabsf(tf1), kSlack, tf1))) &
                                 mask & (OCT >= 0 || r1 != kEmptyRef ? ~0ull : 0ull);
-            if (m0 == 0) {
+            if (m0 != 0) {
                 if (m1 != 0) {
-                    ref = r1;
-                    mask = m1;
-                    continue;
+                    const uint64_t both = m0 & m1;
+                    const int fl = __builtin_ctzll(both != 0 ? both : mask);
+                    const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
+                    const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
+                    const bool swap = n1 < n0;
+                    const int32_t far = swap ? r0 : r1;
+                    const uint64_t fm = swap ? m0 : m1;
+                    wstk[3 * sp] = far;
+                    wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
+                    wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
+                    sp++;
+                    ref = swap ? r1 : r0;
+                    mask = swap ? m1 : m0;
+                } else {
+                    ref = r0;
+                    mask = m0;
                 }
-            } else if (m1 == 0) {
-                ref = r0;
-                mask = m0;
-                continue;
+            } else if (m1 != 0) {
+                ref = r1;
+                mask = m1;
             } else {
-                const uint64_t both = m0 & m1;
-                const int fl = __builtin_ctzll(both != 0 ? both : mask);
-                const float n0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn0), fl));
-                const float n1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn1), fl));
-                const bool swap = n1 < n0;
-                const int32_t far = swap ? r0 : r1;
-                const uint64_t fm = swap ? m0 : m1;
-                wstk[3 * sp] = far;
-                wstk[3 * sp + 1] = (int32_t)(uint32_t)fm;
-                wstk[3 * sp + 2] = (int32_t)(uint32_t)(fm >> 32);
-                sp++;
-                ref = swap ? r1 : r0;
-                mask = swap ? m1 : m0;
-                continue;
+                ref = kEmptyRef;
             }
-        } else {
+        }
+        if (ref != kEmptyRef) {
             // every lane runs the test; the mask and the order rule are
             // applied by selects (see packet_any)
             const bool act = ((mask >> lane) & 1ull) != 0;
@@ -555,7 +561,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
                 const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
                 const int32_t orig = (int32_t)(meta >> kOrigShift);
                 float t;
-                const bool h = tri_test_flat(t0, t1, t2, meta, r, t) & (t >= tmin);
+                const bool h = tri_test_uniform(t0, t1, t2, meta, r, t) & (t >= tmin);
                 const bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);
                 const bool better = best_orig < 0
                                         ? within
@@ -576,6 +582,7 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
             mask = (hi << 32) | lo;
             if (mask != 0) break;
         }
+        if (mask == 0) break;
     }
     t_out = best_t;
     id_out = best_id;
