@@ -10,7 +10,7 @@
 namespace vh {
 namespace {
 
-constexpr int kBlock = 128;
+constexpr int kBlock = 64;
 constexpr int kOccluders = 3;  // occluders a point-light lane remembers
 
 __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays, int rw, float shared_reach,
