@@ -155,129 +155,211 @@ __device__ __forceinline__ bool make_transfer(const PatchDev &pd, float4 ro, flo
     return (double)trans > kTransferEps;
 }
 
-// One lane per receiver.  MAKE = false counts the candidates; MAKE = true
-// traces, makes and scales them into the receiver's staging range.
-template <bool MAKE>
-__global__ __launch_bounds__(kTB) void k_transfers(PatchDev pd, const int32_t *__restrict__ recv,
-                                                   int32_t nrecv, const float4 *__restrict__ nodes,
-                                                   const float4 *__restrict__ tris,
-                                                   int64_t *__restrict__ counts,
-                                                   const int64_t *__restrict__ cand_off,
-                                                   int32_t *__restrict__ st_src,
-                                                   float *__restrict__ st_w,
-                                                   int64_t *__restrict__ made,
-                                                   unsigned long long *__restrict__ stats,
-                                                   int *__restrict__ err, int32_t tdepth,
-                                                   int32_t pdepth)
+// The candidates of one receiver in the reference's order (the loops of
+// VisMatrix.BuildAsync for one row): its cluster's PVS faces, skipping its own
+// face and faces whose plane it is not in front of; each face's patch trees
+// descended while the receiver is within four times a parent's size; a leaf
+// (or a patch not descended into) is a candidate when it is in front of the
+// receiver's plane.  f(s, so) sees each candidate.  The descent stack is the
+// lane's LDS column (pdepth rows); returns false if a descent outgrew it.
+template <class F>
+__device__ __forceinline__ bool for_each_candidate(const PatchDev &pd, float4 ro, float4 rn, int32_t cl,
+                                                   int32_t myface, int *pstk, int32_t pdepth, F &&f)
 {
-    // the lanes' stacks, [depth][lane]: traversal (tdepth rows, MAKE only),
-    // then the patch descent (pdepth rows)
-    extern __shared__ int stacks[];
-    int *trav = stacks + threadIdx.x;
-    int *pstk = stacks + (MAKE ? tdepth * kTB : 0) + threadIdx.x;
-    int64_t k = (int64_t)blockIdx.x * kTB + threadIdx.x;
-    uint32_t nblocked = 0;
-    if (k < nrecv) {
-        int32_t r = recv[k];
-        float4 ro = pd.origin_area[r];
-        float4 rn = pd.normal_pd[r];
-        int32_t cl = pd.cluster[r];
-        int32_t myface = pd.face[r];
-        int64_t base = MAKE ? cand_off[k] : 0;
-        int64_t limit = MAKE ? cand_off[k + 1] - base : 0;
-        int64_t cnt = 0, kept = 0;
-        // the transfer ray starts one unit off the receiver (VisMatrix.MakeRays)
-        float p1x = ro.x + rn.x, p1y = ro.y + rn.y, p1z = ro.z + rn.z;
-        for (int32_t q = pd.row_faces_ptr[cl]; q < pd.row_faces_ptr[cl + 1]; q++) {
-            int32_t f = pd.row_faces[q];
-            if (f == myface) continue;
-            int32_t head = pd.face_parent[f];
-            if (head < 0) continue;
-            float4 fn = pd.normal_pd[head];
-            if (!((double)dot3(ro.x, ro.y, ro.z, fn.x, fn.y, fn.z) > (double)fn.w + kPlaneTestEps))
-                continue;
-            for (int32_t p = head; p != -1; p = pd.next_parent[p]) {
-                int sp = 0;
-                pstk[0] = p;
-                sp = 1;
-                while (sp > 0) {
-                    sp--;
-                    int32_t s = pstk[sp * kTB];
-                    float4 so = pd.origin_area[s];
-                    int2 ch = pd.children[s];
-                    if (ch.x != -1) {
-                        float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
-                        if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
-                            if (sp + 2 > pdepth) {
-                                atomicOr(err, 1);
-                                continue;
-                            }
-                            pstk[sp * kTB] = ch.y;
-                            pstk[(sp + 1) * kTB] = ch.x;  // child1 first, as the recursion
-                            sp += 2;
+    bool ok = true;
+    for (int32_t q = pd.row_faces_ptr[cl]; q < pd.row_faces_ptr[cl + 1]; q++) {
+        int32_t fc = pd.row_faces[q];
+        if (fc == myface) continue;
+        int32_t head = pd.face_parent[fc];
+        if (head < 0) continue;
+        float4 fn = pd.normal_pd[head];
+        if (!((double)dot3(ro.x, ro.y, ro.z, fn.x, fn.y, fn.z) > (double)fn.w + kPlaneTestEps))
+            continue;
+        for (int32_t p = head; p != -1; p = pd.next_parent[p]) {
+            int sp = 0;
+            pstk[0] = p;
+            sp = 1;
+            while (sp > 0) {
+                sp--;
+                int32_t sc = pstk[sp * kTB];
+                float4 so = pd.origin_area[sc];
+                int2 ch = pd.children[sc];
+                if (ch.x != -1) {
+                    float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
+                    if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
+                        if (sp + 2 > pdepth) {
+                            ok = false;
                             continue;
                         }
-                    }
-                    if (!((double)dot3(so.x, so.y, so.z, rn.x, rn.y, rn.z) >
-                          (double)rn.w + kPlaneTestEps))
+                        pstk[sp * kTB] = ch.y;
+                        pstk[(sp + 1) * kTB] = ch.x;  // child1 first, as the recursion
+                        sp += 2;
                         continue;
-                    cnt++;
-                    if (!MAKE) continue;
-                    if (cnt > limit) {
-                        atomicOr(err, 2);
-                        continue;
-                    }
-                    // VisMatrix.TransferRay, exact divide
-                    float4 sn = pd.normal_pd[s];
-                    float p2x = so.x + sn.x, p2y = so.y + sn.y, p2z = so.z + sn.z;
-                    float dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
-                    float len = sqrtf((dx * dx) + (dy * dy) + (dz * dz));
-                    float safe = len == 0.0f ? 1.1920929E-07f : len;
-                    Ray ray;
-                    ray.ox = p1x;
-                    ray.oy = p1y;
-                    ray.oz = p1z;
-                    ray.dx = dx / safe;
-                    ray.dy = dy / safe;
-                    ray.dz = dz / safe;
-                    ray.reach = len;
-                    float th;
-                    int32_t hid;
-                    bool blocked = false;
-                    if (isfinite(ray.dx) && isfinite(ray.dy) && isfinite(ray.dz))
-                        blocked = trace_one<kAny, false, kTB>(nodes, tris, trav, ray, 0.0f, -1, th,
-                                                              hid) >= 0;
-                    if (blocked) {
-                        nblocked++;
-                        continue;
-                    }
-                    if (kept >= kMaxPatchesPerRow) continue;
-                    float w;
-                    if (make_transfer(pd, ro, rn, s, so, w)) {
-                        st_src[base + kept] = s;
-                        st_w[base + kept] = w;
-                        kept++;
                     }
                 }
+                if (!((double)dot3(so.x, so.y, so.z, rn.x, rn.y, rn.z) > (double)rn.w + kPlaneTestEps))
+                    continue;
+                f(sc, so);
             }
-        }
-        if (!MAKE) {
-            counts[k] = cnt;
-        } else {
-            // VisMatrix.MakeScales: the row's float sum, left to right
-            if (kept > 0) {
-                float tot = 0.0f;
-                for (int64_t j = 0; j < kept; j++) tot += st_w[base + j];
-                tot = (double)tot > M_PI ? 1.0f / tot : (float)(1.0f / M_PI);
-                for (int64_t j = 0; j < kept; j++) st_w[base + j] = st_w[base + j] * tot;
-            }
-            made[k] = kept;
         }
     }
-    if (MAKE) {
-        unsigned long long x = nblocked;
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if (threadIdx.x == 0) atomicAdd(&stats[0], x);
+    return ok;
+}
+
+// One lane per receiver: LIST = false counts its candidates; LIST = true
+// writes their sources, in order, from the receiver's offset.
+template <bool LIST>
+__global__ __launch_bounds__(kTB) void k_candidates(PatchDev pd, const int32_t *__restrict__ recv,
+                                                    int32_t nrecv, int64_t *__restrict__ counts,
+                                                    const int64_t *__restrict__ cand_off,
+                                                    int32_t *__restrict__ cand_src, int *__restrict__ err,
+                                                    int32_t pdepth)
+{
+    extern __shared__ int stacks[];  // the lanes' descent stacks, [pdepth][lane]
+    int *pstk = stacks + threadIdx.x;
+    int64_t k = (int64_t)blockIdx.x * kTB + threadIdx.x;
+    if (k >= nrecv) return;
+    int32_t r = recv[k];
+    float4 ro = pd.origin_area[r];
+    float4 rn = pd.normal_pd[r];
+    int64_t base = LIST ? cand_off[k] : 0;
+    int64_t limit = LIST ? cand_off[k + 1] - base : 0;
+    int64_t cnt = 0;
+    bool over = false;
+    bool ok = for_each_candidate(pd, ro, rn, pd.cluster[r], pd.face[r], pstk, pdepth,
+                                 [&](int32_t sc, float4) {
+                                     if (LIST) {
+                                         if (cnt < limit)
+                                             cand_src[base + cnt] = sc;
+                                         else
+                                             over = true;
+                                     }
+                                     cnt++;
+                                 });
+    if (!ok) atomicOr(err, 1);
+    if (LIST && (over || cnt != limit)) atomicOr(err, 2);
+    if (!LIST) counts[k] = cnt;
+}
+
+constexpr int kRayBatches = 16;  // 64-candidate batches a transfer-ray wave takes
+
+// The candidate rays, one lane per candidate over all receivers' candidates
+// in order (VisMatrix.TransferRay and MakeTransfer).  A wave takes
+// kRayBatches x 64 consecutive candidates, so a batch is mostly one
+// receiver's: 64 rays from one start to neighbouring patches, which walk the
+// tree as a packet.  cand_w gets the transfer's weight, or 0 when the ray is
+// blocked or MakeTransfer drops it.
+__global__ __launch_bounds__(64) void k_transfer_rays(PatchDev pd, const int32_t *__restrict__ recv,
+                                                      int32_t nrecv, const int64_t *__restrict__ cand_off,
+                                                      const int32_t *__restrict__ cand_src, int64_t ncand,
+                                                      const float4 *__restrict__ nodes,
+                                                      const float4 *__restrict__ tris,
+                                                      float *__restrict__ cand_w,
+                                                      unsigned long long *__restrict__ stats)
+{
+    extern __shared__ int wstk[];  // the wave's walk stack, 3 ints a level
+    const int lane = lane_id();
+    const int64_t first = (int64_t)blockIdx.x * (kRayBatches * 64);
+    // the receiver of the wave's first candidate: the last k with cand_off[k] <= first
+    int32_t lo = 0, hi = nrecv;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (cand_off[mid] <= first) lo = mid; else hi = mid;
+    }
+    int32_t kb = uniform(lo);
+    uint32_t nblocked = 0;
+    for (int b = 0; b < kRayBatches; b++) {
+        const int64_t c = first + (int64_t)b * 64 + lane;
+        const bool valid = c < ncand;
+        int32_t k = kb;
+        if (valid)
+            while (cand_off[k + 1] <= c) k++;
+        kb = uniform(__builtin_amdgcn_readlane(k, 63));
+        Ray ray{};
+        float4 ro{}, rn{}, so{};
+        int32_t sc = 0;
+        bool fin = false;
+        if (valid) {
+            const int32_t r = recv[k];
+            ro = pd.origin_area[r];
+            rn = pd.normal_pd[r];
+            sc = cand_src[c];
+            so = pd.origin_area[sc];
+            // the transfer ray starts one unit off the receiver and ends one unit
+            // off the source (VisMatrix.MakeRays, TransferRay), exact divide
+            const float p1x = ro.x + rn.x, p1y = ro.y + rn.y, p1z = ro.z + rn.z;
+            const float4 sn = pd.normal_pd[sc];
+            const float p2x = so.x + sn.x, p2y = so.y + sn.y, p2z = so.z + sn.z;
+            const float dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
+            const float len = sqrtf((dx * dx) + (dy * dy) + (dz * dz));
+            const float safe = len == 0.0f ? 1.1920929E-07f : len;
+            ray.ox = p1x;
+            ray.oy = p1y;
+            ray.oz = p1z;
+            ray.dx = dx / safe;
+            ray.dy = dy / safe;
+            ray.dz = dz / safe;
+            ray.reach = len;
+            fin = isfinite(ray.dx) && isfinite(ray.dy) && isfinite(ray.dz);
+        }
+        const bool blocked = by_octant(wave_octant(fin, ray), [&](auto oct) {
+            return packet_any<false, decltype(oct)::value>(nodes, tris, wstk, ray, 0.0f, fin);
+        });
+        if (valid) {
+            float w = 0.0f;
+            if (blocked) {
+                nblocked++;
+            } else {
+                float t;
+                if (make_transfer(pd, ro, rn, sc, so, t)) w = t;
+            }
+            cand_w[c] = w;
+        }
+    }
+    unsigned long long x = nblocked;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) atomicAdd(&stats[0], x);
+}
+
+// One wave per receiver: the kept candidates (weight > 0; MakeTransfer keeps
+// only weights above 1e-7), in order and at most kMaxPatchesPerRow of them,
+// packed to the front of the row's staging range, and VisMatrix.MakeScales'
+// factor from their float sum taken left to right.
+__global__ __launch_bounds__(kGB) void k_rowscan(int32_t nrecv, const int64_t *__restrict__ cand_off,
+                                                 const int32_t *__restrict__ cand_src,
+                                                 const float *__restrict__ cand_w,
+                                                 int32_t *__restrict__ st_src, float *__restrict__ st_w,
+                                                 int64_t *__restrict__ made, float *__restrict__ msc)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
+    if (k >= nrecv) return;  // whole wave
+    const int64_t base = cand_off[k], end = cand_off[k + 1];
+    int64_t kept = 0;
+    float tot = 0.0f;  // the same value in every lane
+    for (int64_t j0 = base; j0 < end && kept < kMaxPatchesPerRow; j0 += 64) {
+        const int64_t j = j0 + lane;
+        const float w = j < end ? cand_w[j] : 0.0f;
+        uint64_t bal = __builtin_amdgcn_ballot_w64(w > 0.0f);
+        const int64_t room = kMaxPatchesPerRow - kept;
+        if ((int64_t)__builtin_popcountll(bal) > room) {
+            // keep only the first `room` of this batch
+            uint64_t m = bal;
+            for (int64_t i = 0; i < room; i++) m &= m - 1;
+            bal &= ~m;
+        }
+        const int pos = __builtin_popcountll(bal & ((1ull << lane) - 1ull));
+        if ((bal >> lane) & 1ull) {
+            st_src[base + kept + pos] = cand_src[j];
+            st_w[base + kept + pos] = w;
+        }
+        for (uint64_t m = bal; m != 0; m &= m - 1)
+            tot += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), __builtin_ctzll(m)));
+        kept += __builtin_popcountll(bal);
+    }
+    if (lane == 0) {
+        made[k] = kept;
+        msc[k] = kept > 0 ? ((double)tot > M_PI ? 1.0f / tot : (float)(1.0f / M_PI)) : 1.0f;
     }
 }
 
@@ -288,6 +370,7 @@ __global__ __launch_bounds__(kGB) void k_compact(const int32_t *__restrict__ rec
                                                  const int64_t *__restrict__ row_ptr,
                                                  const int32_t *__restrict__ st_src,
                                                  const float *__restrict__ st_w,
+                                                 const float *__restrict__ msc,
                                                  int32_t *__restrict__ src, float *__restrict__ w32,
                                                  __half *__restrict__ w16,
                                                  float *__restrict__ rscale)
@@ -299,16 +382,17 @@ __global__ __launch_bounds__(kGB) void k_compact(const int32_t *__restrict__ rec
     int64_t from = cand_off[k];
     int64_t to = row_ptr[r];
     int64_t n = row_ptr[r + 1] - to;
+    const float f = msc[k];  // VisMatrix.MakeScales' factor for the row
     float mx = 0.0f;
     for (int64_t j = lane; j < n; j += 64) {
-        float x = st_w[from + j];
+        float x = st_w[from + j] * f;
         src[to + j] = st_src[from + j];
         w32[to + j] = x;
         mx = fmaxf(mx, fabsf(x));
     }
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
     float scale = mx > 0.0f ? mx / kF16Top : 1.0f;
-    for (int64_t j = lane; j < n; j += 64) w16[to + j] = __float2half_rn(st_w[from + j] / scale);
+    for (int64_t j = lane; j < n; j += 64) w16[to + j] = __float2half_rn((st_w[from + j] * f) / scale);
     if (lane == 0) rscale[r] = scale;
 }
 
@@ -898,13 +982,15 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     hipStream_t s = c->stream;
     int32_t R = (int32_t)B->h_receivers.size();
     PatchDev pd = patch_dev(B);
-    DevBuf counts, cand_off, st_src, st_w, made, stats, err;
+    DevBuf counts, cand_off, cand_src, cand_w, st_src, st_w, made, msc, stats, err;
     auto cleanup = [&]() {
-        for (DevBuf *b : {&counts, &cand_off, &st_src, &st_w, &made, &stats, &err}) b->release();
+        for (DevBuf *b : {&counts, &cand_off, &cand_src, &cand_w, &st_src, &st_w, &made, &msc, &stats, &err})
+            b->release();
     };
     if ((rc = counts.ensure((size_t)(R + 1) * 8, "candidate counts")) ||
         (rc = cand_off.ensure((size_t)(R + 1) * 8, "candidate offsets")) ||
         (rc = made.ensure((size_t)(R + 1) * 8, "row lengths")) ||
+        (rc = msc.ensure((size_t)(R + 1) * 4, "row scale factors")) ||
         (rc = stats.ensure(16, "build statistics")) || (rc = err.ensure(16, "build errors"))) {
         cleanup();
         return rc;
@@ -917,11 +1003,11 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     const int32_t tdepth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;
     const int32_t pdepth = B->height + 2 < kPatchStack ? B->height + 2 : kPatchStack;
     KernelTimer timer(c, VH_KERNEL_TRANSFERS, s);
+    // 1. candidates per receiver
     if (R > 0)
-        hipLaunchKernelGGL(k_transfers<false>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
-                           (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
-                           (const float4 *)c->tris.ptr, (int64_t *)counts.ptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, (int *)err.ptr, 0, pdepth);
+        hipLaunchKernelGGL(k_candidates<false>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
+                           (const int32_t *)B->receivers.ptr, R, (int64_t *)counts.ptr, nullptr, nullptr,
+                           (int *)err.ptr, pdepth);
     VH_HIP(hipGetLastError());
     std::vector<int64_t> hcount(R + 1, 0), hoff(R + 1, 0);
     if (R > 0)
@@ -929,18 +1015,34 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     VH_HIP(hipStreamSynchronize(s));
     for (int32_t k = 0; k < R; k++) hoff[k + 1] = hoff[k] + hcount[k];
     int64_t ncand = hoff[R];
+    const size_t nslot = (size_t)(ncand > 0 ? ncand : 1);
     if ((rc = upload(cand_off, hoff.data(), hoff.size(), "candidate offsets", s)) ||
-        (rc = st_src.ensure((size_t)(ncand > 0 ? ncand : 1) * 4, "staged sources")) ||
-        (rc = st_w.ensure((size_t)(ncand > 0 ? ncand : 1) * 4, "staged weights"))) {
+        (rc = cand_src.ensure(nslot * 4, "candidate sources")) ||
+        (rc = cand_w.ensure(nslot * 4, "candidate weights")) ||
+        (rc = st_src.ensure(nslot * 4, "staged sources")) ||
+        (rc = st_w.ensure(nslot * 4, "staged weights"))) {
         cleanup();
         return rc;
     }
+    if (R > 0 && ncand > 0) {
+        // 2. their sources, in the reference's order
+        hipLaunchKernelGGL(k_candidates<true>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
+                           (const int32_t *)B->receivers.ptr, R, nullptr, (const int64_t *)cand_off.ptr,
+                           (int32_t *)cand_src.ptr, (int *)err.ptr, pdepth);
+        // 3. every candidate's ray and weight, a lane each
+        const int64_t waves = (ncand + kRayBatches * 64 - 1) / (kRayBatches * 64);
+        hipLaunchKernelGGL(k_transfer_rays, dim3((uint32_t)waves), dim3(64), (size_t)3 * tdepth * 4, s, pd,
+                           (const int32_t *)B->receivers.ptr, R, (const int64_t *)cand_off.ptr,
+                           (const int32_t *)cand_src.ptr, ncand, (const float4 *)c->nodes.ptr,
+                           (const float4 *)c->tris.ptr, (float *)cand_w.ptr,
+                           (unsigned long long *)stats.ptr);
+    }
+    // 4. each row's kept transfers, packed in order, and its MakeScales factor
     if (R > 0)
-        hipLaunchKernelGGL(k_transfers<true>, dim3(grid), dim3(kTB), (size_t)(tdepth + pdepth) * kTB * 4, s, pd,
-                           (const int32_t *)B->receivers.ptr, R, (const float4 *)c->nodes.ptr,
-                           (const float4 *)c->tris.ptr, nullptr, (const int64_t *)cand_off.ptr,
-                           (int32_t *)st_src.ptr, (float *)st_w.ptr, (int64_t *)made.ptr,
-                           (unsigned long long *)stats.ptr, (int *)err.ptr, tdepth, pdepth);
+        hipLaunchKernelGGL(k_rowscan, dim3((uint32_t)((R + 3) / 4)), dim3(kGB), 0, s, R,
+                           (const int64_t *)cand_off.ptr, (const int32_t *)cand_src.ptr,
+                           (const float *)cand_w.ptr, (int32_t *)st_src.ptr, (float *)st_w.ptr,
+                           (int64_t *)made.ptr, (float *)msc.ptr);
     timer.done();
     VH_HIP(hipGetLastError());
     std::vector<int64_t> hmade(R + 1, 0);
@@ -970,8 +1072,8 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
         hipLaunchKernelGGL(k_compact, dim3((uint32_t)((R + 3) / 4)), dim3(kGB), 0, s,
                            (const int32_t *)B->receivers.ptr, R, (const int64_t *)cand_off.ptr,
                            (const int64_t *)B->row_ptr.ptr, (const int32_t *)st_src.ptr,
-                           (const float *)st_w.ptr, (int32_t *)B->src.ptr, (float *)B->w32.ptr,
-                           (__half *)B->w16.ptr, (float *)B->rscale.ptr);
+                           (const float *)st_w.ptr, (const float *)msc.ptr, (int32_t *)B->src.ptr,
+                           (float *)B->w32.ptr, (__half *)B->w16.ptr, (float *)B->rscale.ptr);
     VH_HIP(hipGetLastError());
     if ((rc = upload_gather_rows(B, rp, s))) {
         cleanup();

@@ -167,6 +167,73 @@ def test_point_lights_equal_materialised_rays(nlights):
     assert np.array_equal(pl, bvh.point_lights(w.points, w.lights))
 
 
+def _walk_form_points(w, rng):
+    """Points that send a wave's rays into several direction octants (points
+    all round each light), rays with an exactly zero component (a shared
+    coordinate with the light) and with components under 2^-60, plus the
+    workload's own points."""
+    parts = [w.points]
+    for L in w.lights[:6]:
+        parts.append((L + rng.normal(0.0, 40.0, size=(128, 3))).astype(np.float32))
+        same = np.repeat(L[None, :], 64, axis=0).astype(np.float32)
+        same[:, 1:] += rng.uniform(-300.0, 300.0, size=(64, 2)).astype(np.float32)  # d.x == 0
+        parts.append(same)
+    pts = np.concatenate(parts).astype(np.float32)
+    return pts
+
+
+def _tiny_component_case(w, rng):
+    """A light moved to x = 0 and points at x = +-1e-25 and +-1e-30 round it:
+    direction components under 2^-60 that are not zero."""
+    L = w.lights[0].copy()
+    L[0] = 0.0
+    pts = np.repeat(L[None, :], 256, axis=0).astype(np.float32)
+    pts[:, 1:] += rng.uniform(-300.0, 300.0, size=(256, 2)).astype(np.float32)
+    pts[:, 0] = np.tile(np.array([1e-25, -1e-25, 1e-30, -1e-30], np.float32), 64)
+    return pts, np.concatenate([w.lights, L[None, :]]).astype(np.float32)
+
+
+@pytest.mark.parametrize("form", ["octants", "tmin"])
+def test_point_lights_walk_forms(form):
+    """The point-light kernel walks the tree per direction octant when a
+    wave's rays share one and by the general form otherwise (points all round
+    a light); direction components that are zero or under 2^-60 take the
+    saturated inverse.  The bits equal the brute-force oracle's on the
+    materialised rays, with tmin 0 and with tmin 0.25."""
+    rng = np.random.default_rng(17)
+    w = scenes.visibility_workload(nboxes=150, npoints=1500, nlights=12, seed=13, volume=1500.0)
+    v9 = w.casters.vertices.copy()
+    pts = _walk_form_points(w, rng)
+    tiny_pts, lights = _tiny_component_case(w, rng)
+    pts = np.concatenate([pts, tiny_pts]).astype(np.float32)
+    tmin = 0.25 if form == "tmin" else 0.0
+    ids = np.arange(len(v9), dtype=np.int32)
+    rays = scenes.point_light_rays(pts, lights)
+    print(f"[walk forms] {form}: {len(v9)} tris, {len(rays)} rays, tmin {tmin}", flush=True)
+    tris = oracle.convert_triangles(v9, ids)
+    want = oracle.visibility_brute(tris, rays, tmin=tmin)
+    with make_tracer(v9, ids) as tr:
+        got = tr.trace_point_lights(pts, lights, RayTraceOptions(tmin))
+    assert np.array_equal(got, want)
+    frac = unpack_bits(got, len(rays)).mean()
+    assert 0.05 < frac < 0.95
+
+
+def test_point_lights_single_leaf_scene():
+    """A scene of one triangle is one leaf under the root, whose other child
+    is empty; the octant walks drop the empty-child check, which this covers."""
+    v9 = np.array([[0, 0, 0, 10, 0, 0, 0, 10, 0]], np.float32)
+    tris = oracle.convert_triangles(v9)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform([-5, -5, -10], [15, 15, 10], size=(500, 3)).astype(np.float32)
+    lights = np.array([[2, 2, 20], [2, 2, -20], [30, 1, 1], [1, 1, 0.5]], np.float32)
+    with make_tracer(v9) as tr:
+        got = tr.trace_point_lights(pts, lights)
+    want = oracle.visibility_brute(tris, scenes.point_light_rays(pts, lights))
+    assert np.array_equal(got, want)
+    assert unpack_bits(got, len(pts) * len(lights)).any()
+
+
 def test_edge_cases():
     v9 = np.array([[0, 0, 0, 10, 0, 0, 0, 10, 0]], np.float32)  # one triangle, z = 0
     tris = oracle.convert_triangles(v9)
