@@ -155,59 +155,53 @@ __device__ __forceinline__ bool make_transfer(const PatchDev &pd, float4 ro, flo
     return (double)trans > kTransferEps;
 }
 
-// The candidates of one receiver in the reference's order (the loops of
-// VisMatrix.BuildAsync for one row): its cluster's PVS faces, skipping its own
-// face and faces whose plane it is not in front of; each face's patch trees
+// The candidates one PVS face gives a receiver, in the reference's order
+// (the inner loops of VisMatrix.BuildAsync): the face's patch trees
 // descended while the receiver is within four times a parent's size; a leaf
 // (or a patch not descended into) is a candidate when it is in front of the
 // receiver's plane.  f(s, so) sees each candidate.  The descent stack is the
 // lane's LDS column (pdepth rows); returns false if a descent outgrew it.
 template <class F>
-__device__ __forceinline__ bool for_each_candidate(const PatchDev &pd, float4 ro, float4 rn, int32_t cl,
-                                                   int32_t myface, int *pstk, int32_t pdepth, F &&f)
+__device__ __forceinline__ bool face_candidates(const PatchDev &pd, float4 ro, float4 rn, int32_t head,
+                                                int *pstk, int32_t pdepth, F &&f)
 {
     bool ok = true;
-    for (int32_t q = pd.row_faces_ptr[cl]; q < pd.row_faces_ptr[cl + 1]; q++) {
-        int32_t fc = pd.row_faces[q];
-        if (fc == myface) continue;
-        int32_t head = pd.face_parent[fc];
-        if (head < 0) continue;
-        float4 fn = pd.normal_pd[head];
-        if (!((double)dot3(ro.x, ro.y, ro.z, fn.x, fn.y, fn.z) > (double)fn.w + kPlaneTestEps))
-            continue;
-        for (int32_t p = head; p != -1; p = pd.next_parent[p]) {
-            int sp = 0;
-            pstk[0] = p;
-            sp = 1;
-            while (sp > 0) {
-                sp--;
-                int32_t sc = pstk[sp * kTB];
-                float4 so = pd.origin_area[sc];
-                int2 ch = pd.children[sc];
-                if (ch.x != -1) {
-                    float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
-                    if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
-                        if (sp + 2 > pdepth) {
-                            ok = false;
-                            continue;
-                        }
-                        pstk[sp * kTB] = ch.y;
-                        pstk[(sp + 1) * kTB] = ch.x;  // child1 first, as the recursion
-                        sp += 2;
+    for (int32_t p = head; p != -1; p = pd.next_parent[p]) {
+        int sp = 0;
+        pstk[0] = p;
+        sp = 1;
+        while (sp > 0) {
+            sp--;
+            int32_t sc = pstk[sp * kTB];
+            float4 so = pd.origin_area[sc];
+            int2 ch = pd.children[sc];
+            if (ch.x != -1) {
+                float tx = ro.x - so.x, ty = ro.y - so.y, tz = ro.z - so.z;
+                if ((double)dot3(tx, ty, tz, tx, ty, tz) * 0.0625 < (double)so.w) {
+                    if (sp + 2 > pdepth) {
+                        ok = false;
                         continue;
                     }
-                }
-                if (!((double)dot3(so.x, so.y, so.z, rn.x, rn.y, rn.z) > (double)rn.w + kPlaneTestEps))
+                    pstk[sp * kTB] = ch.y;
+                    pstk[(sp + 1) * kTB] = ch.x;  // child1 first, as the recursion
+                    sp += 2;
                     continue;
-                f(sc, so);
+                }
             }
+            if (!((double)dot3(so.x, so.y, so.z, rn.x, rn.y, rn.z) > (double)rn.w + kPlaneTestEps))
+                continue;
+            f(sc, so);
         }
     }
     return ok;
 }
 
-// One lane per receiver: LIST = false counts its candidates; LIST = true
-// writes their sources, in order, from the receiver's offset.
+// One wave per receiver, a lane per PVS face of its cluster (64 faces at a
+// time, in row order): the faces it skips are its own and those whose plane
+// it is not in front of.  LIST = false counts the receiver's candidates;
+// LIST = true writes their sources in the reference's order from the
+// receiver's offset (each lane's face's candidates after those of the faces
+// before it, by a prefix sum over the wave).
 template <bool LIST>
 __global__ __launch_bounds__(kTB) void k_candidates(PatchDev pd, const int32_t *__restrict__ recv,
                                                     int32_t nrecv, int64_t *__restrict__ counts,
@@ -217,28 +211,55 @@ __global__ __launch_bounds__(kTB) void k_candidates(PatchDev pd, const int32_t *
 {
     extern __shared__ int stacks[];  // the lanes' descent stacks, [pdepth][lane]
     int *pstk = stacks + threadIdx.x;
-    int64_t k = (int64_t)blockIdx.x * kTB + threadIdx.x;
-    if (k >= nrecv) return;
-    int32_t r = recv[k];
-    float4 ro = pd.origin_area[r];
-    float4 rn = pd.normal_pd[r];
-    int64_t base = LIST ? cand_off[k] : 0;
-    int64_t limit = LIST ? cand_off[k + 1] - base : 0;
-    int64_t cnt = 0;
-    bool over = false;
-    bool ok = for_each_candidate(pd, ro, rn, pd.cluster[r], pd.face[r], pstk, pdepth,
-                                 [&](int32_t sc, float4) {
-                                     if (LIST) {
-                                         if (cnt < limit)
-                                             cand_src[base + cnt] = sc;
-                                         else
-                                             over = true;
-                                     }
-                                     cnt++;
-                                 });
+    const int lane = lane_id();
+    const int64_t k = blockIdx.x;
+    if (k >= nrecv) return;  // whole wave
+    const int32_t r = recv[k];
+    const float4 ro = pd.origin_area[r];
+    const float4 rn = pd.normal_pd[r];
+    const int32_t cl = pd.cluster[r], myface = pd.face[r];
+    const int64_t base = LIST ? cand_off[k] : 0;
+    const int64_t limit = LIST ? cand_off[k + 1] - base : 0;
+    int64_t cnt = 0;  // the receiver's candidates so far (the same in every lane)
+    bool ok = true, over = false;
+    const int32_t q_end = pd.row_faces_ptr[cl + 1];
+    for (int32_t q0 = pd.row_faces_ptr[cl]; q0 < q_end; q0 += 64) {
+        const int32_t q = q0 + lane;
+        int32_t head = -1;
+        if (q < q_end) {
+            const int32_t fc = pd.row_faces[q];
+            if (fc != myface) {
+                head = pd.face_parent[fc];
+                if (head >= 0) {
+                    const float4 fn = pd.normal_pd[head];
+                    if (!((double)dot3(ro.x, ro.y, ro.z, fn.x, fn.y, fn.z) > (double)fn.w + kPlaneTestEps))
+                        head = -1;
+                }
+            }
+        }
+        int32_t n = 0;
+        if (head >= 0) ok &= face_candidates(pd, ro, rn, head, pstk, pdepth, [&](int32_t, float4) { n++; });
+        // exclusive prefix of the lanes' counts
+        int32_t incl = n;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (LIST && head >= 0 && n > 0) {
+            int64_t at = cnt + (incl - n);
+            face_candidates(pd, ro, rn, head, pstk, pdepth, [&](int32_t sc, float4) {
+                if (at < limit)
+                    cand_src[base + at] = sc;
+                else
+                    over = true;
+                at++;
+            });
+        }
+        cnt += __shfl(incl, 63, 64);
+    }
     if (!ok) atomicOr(err, 1);
     if (LIST && (over || cnt != limit)) atomicOr(err, 2);
-    if (!LIST) counts[k] = cnt;
+    if (!LIST && lane == 0) counts[k] = cnt;
 }
 
 constexpr int kRayBatches = 16;  // 64-candidate batches a transfer-ray wave takes
@@ -997,7 +1018,6 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     }
     VH_HIP(hipMemsetAsync(stats.ptr, 0, 16, s));
     VH_HIP(hipMemsetAsync(err.ptr, 0, 16, s));
-    uint32_t grid = (uint32_t)((R + kTB - 1) / kTB);
     // stack rows: the tree's depth bounds a traversal stack; a patch descent
     // holds at most height + 1 entries
     const int32_t tdepth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;
@@ -1005,7 +1025,7 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     KernelTimer timer(c, VH_KERNEL_TRANSFERS, s);
     // 1. candidates per receiver
     if (R > 0)
-        hipLaunchKernelGGL(k_candidates<false>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
+        hipLaunchKernelGGL(k_candidates<false>, dim3((uint32_t)R), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
                            (const int32_t *)B->receivers.ptr, R, (int64_t *)counts.ptr, nullptr, nullptr,
                            (int *)err.ptr, pdepth);
     VH_HIP(hipGetLastError());
@@ -1026,7 +1046,7 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     }
     if (R > 0 && ncand > 0) {
         // 2. their sources, in the reference's order
-        hipLaunchKernelGGL(k_candidates<true>, dim3(grid), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
+        hipLaunchKernelGGL(k_candidates<true>, dim3((uint32_t)R), dim3(kTB), (size_t)pdepth * kTB * 4, s, pd,
                            (const int32_t *)B->receivers.ptr, R, nullptr, (const int64_t *)cand_off.ptr,
                            (int32_t *)cand_src.ptr, (int *)err.ptr, pdepth);
         // 3. every candidate's ray and weight, a lane each
