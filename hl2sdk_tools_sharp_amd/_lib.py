@@ -124,6 +124,7 @@ SIGNATURES = {
     "vh_patches_load": (ctypes.c_int, [P, ctypes.POINTER(VhPatchDesc)]),
     "vh_transfers_build": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]),
     "vh_transfers_read": (ctypes.c_int, [P, P, P, P, I64]),
+    "vh_transfers_export": (ctypes.c_int, [P, P, P, I64]),
     "vh_transfers_load": (ctypes.c_int, [P, P, P, P]),
     "vh_bounce": (ctypes.c_int, [P, ctypes.POINTER(VhBounceOpts), P, P, P, P, P, ctypes.POINTER(I32)]),
     "vh_gather_load": (ctypes.c_int, [P, P]),

@@ -121,6 +121,21 @@ class HipBounceStage:
         HipRayTracer._raise(_lib.lib().vh_transfers_read(self._h, _p(rp), _p(src), _p(w), max(T, 1)))
         return rp, src[:T], w[:T]
 
+    def export_trf1(self, max_chunk_bytes: int = 64 << 20):
+        """The built transfers in the reference's transfer-cache layout
+        (TransferSetCodec.Index and Chunks, Rad/Bounce/TransferSetCodec.cs):
+        (index blob, raw chunks).  The records are written on the device."""
+        from . import trf1
+
+        P = self.patches.npatch
+        counts = np.zeros(max(P, 1), np.int32)
+        HipRayTracer._raise(_lib.lib().vh_transfers_export(self._h, _p(counts), None, 0))
+        counts = counts[:P]
+        T = int(counts.astype(np.int64).sum())
+        records = np.zeros(max(T, 1) * 2, np.uint32)
+        HipRayTracer._raise(_lib.lib().vh_transfers_export(self._h, None, _p(records), max(T, 1)))
+        return trf1.index(counts), trf1.chunks(records[:2 * T].tobytes(), max_chunk_bytes)
+
     def patch_lights(self, lighter, hdr: bool = False, read: bool = True):
         """PatchLighting.BuildPatchLights: the last gather of ``lighter`` (a
         HipFaceLighter over this patch system's faces, with sample areas)

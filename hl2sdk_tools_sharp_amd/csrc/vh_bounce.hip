@@ -417,6 +417,23 @@ __global__ __launch_bounds__(kGB) void k_compact(const int32_t *__restrict__ rec
     if (lane == 0) rscale[r] = scale;
 }
 
+// The transfers as 8-byte records, receiver after receiver: the source patch,
+// then the weight's bits (TransferSetCodec.Chunks' record,
+// Rad/Bounce/TransferSetCodec.cs:85-98; the device is little-endian).
+__global__ __launch_bounds__(256) void k_records(const int32_t *__restrict__ src,
+                                                 const float *__restrict__ w, int64_t n,
+                                                 uint2 *__restrict__ out)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) out[j] = make_uint2((uint32_t)src[j], __float_as_uint(w[j]));
+}
+
+// A device buffer released with its scope.
+struct LocalRecords {
+    DevBuf buf;
+    ~LocalRecords() { buf.release(); }
+};
+
 // Scales weights given as FP32 CSR into FP16 + row scale (for loaded transfers).
 __global__ __launch_bounds__(kGB) void k_half_rows(const int32_t *__restrict__ rows, int32_t nrows,
                                                    const int64_t *__restrict__ row_ptr,
@@ -1133,6 +1150,50 @@ int vh_transfers_read(vh_ctx *c, int64_t *row_ptr, int32_t *src, float *weight, 
         VH_HIP(hipMemcpy(src, B->src.ptr, (size_t)B->ntransfers * 4, hipMemcpyDeviceToHost));
     if (weight && B->ntransfers)
         VH_HIP(hipMemcpy(weight, B->w32.ptr, (size_t)B->ntransfers * 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_transfers_export(vh_ctx *c, int32_t *counts, void *records, int64_t capacity)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_loaded(c, true);
+    if (rc) return rc;
+    BounceData *B = c->bounce;
+    const int64_t T = B->ntransfers;
+    if (records && capacity < T) {
+        set_error(std::to_string(T) + " transfers need as many records, and " + std::to_string(capacity) +
+                  " were given");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (counts) {
+        std::vector<int64_t> rp((size_t)B->npatch + 1);
+        VH_HIP(hipMemcpyAsync(rp.data(), B->row_ptr.ptr, rp.size() * 8, hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        for (int32_t p = 0; p < B->npatch; p++) {
+            const int64_t n = rp[p + 1] - rp[p];
+            if (n > INT32_MAX) {
+                set_error("a patch's transfer list is longer than an int");
+                return VH_EINVAL;
+            }
+            counts[p] = (int32_t)n;
+        }
+    }
+    if (records && T > 0) {
+        LocalRecords out;
+        if ((rc = out.buf.ensure((size_t)T * 8, "transfer records"))) return rc;
+        hipLaunchKernelGGL(k_records, dim3((uint32_t)((T + 255) / 256)), dim3(256), 0, s,
+                           (const int32_t *)B->src.ptr, (const float *)B->w32.ptr, T, (uint2 *)out.buf.ptr);
+        VH_HIP(hipGetLastError());
+        VH_HIP(hipMemcpyAsync(records, out.buf.ptr, (size_t)T * 8, hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+    }
     return VH_OK;
 }
 

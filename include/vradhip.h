@@ -190,6 +190,13 @@ int vh_transfers_read(vh_ctx *ctx, int64_t *row_ptr, int32_t *src, float *weight
 /* Loads transfer lists built elsewhere (a TransferSet, or the TRF1 cache,
  * SourceSharp.MapTools/Rad/Bounce/TransferSetCodec.cs:23-80): row_ptr has
  * npatch + 1 entries. */
+/* The transfers in the reference's transfer-cache layout
+ * (TransferSetCodec.Index / Chunks, Rad/Bounce/TransferSetCodec.cs:23-98):
+ * counts[p] = patch p's list length (npatch entries), and records = every
+ * transfer, patch after patch, as 8 little-endian bytes (int32 source patch,
+ * float32 weight), written on the device.  Either may be NULL; records needs
+ * capacity >= the transfer count. */
+int vh_transfers_export(vh_ctx *ctx, int32_t *counts, void *records, int64_t capacity);
 int vh_transfers_load(vh_ctx *ctx, const int64_t *row_ptr, const int32_t *src,
                       const float *weight);
 

@@ -73,6 +73,29 @@ def test_transfer_build_config_shape():
     np.testing.assert_allclose(gw, w, rtol=1e-6, atol=0)
 
 
+def test_trf1_export_is_the_transfer_set(case):
+    """The device-written records framed as the reference's transfer cache
+    (TransferSetCodec) read back to the built rows, entry for entry, raw and
+    packed."""
+    from hl2sdk_tools_sharp_amd import trf1
+
+    g = case[0]
+    tr, st = open_stage(g)
+    with tr:
+        st.build_transfers()
+        rp, src, w = st.read_transfers()
+        idx, chunks = st.export_trf1(max_chunk_bytes=1 << 16)
+    print(f"[trf1] {len(src)} transfers in {len(chunks)} chunks", flush=True)
+    back = trf1.read(idx, chunks, g.npatch)
+    assert back is not None
+    counts, bsrc, bw, mx = back
+    assert np.array_equal(counts, np.diff(rp)) and mx == int(np.diff(rp).max())
+    assert np.array_equal(bsrc, src) and np.array_equal(bw.view(np.uint32), w.view(np.uint32))
+    assert all(len(c) == 1 << 16 for c in chunks[:-1])
+    unpacked = [trf1.unpack(trf1.pack(c)) for c in chunks]
+    assert unpacked == chunks
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_bounce_fp32_matches_oracle(case, graph):
     g, rp, src, w, _ = case
