@@ -133,6 +133,9 @@ SIGNATURES = {
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
     "vh_patch_lights": (ctypes.c_int, [P, I32, P, P]),
+    "vh_final_load": (ctypes.c_int, [P, P]),
+    "vh_final_layout": (ctypes.c_int, [P, P, ctypes.POINTER(I64)]),
+    "vh_final_light": (ctypes.c_int, [P, P, P, P, I64]),
     "vh_mlp_init": (ctypes.c_int, [P, P, P, ctypes.POINTER(VhMlpOpts)]),
     "vh_mlp_params": (ctypes.c_int, [P, P]),
     "vh_mlp_forward": (ctypes.c_int, [P, P, I64, P]),
@@ -156,6 +159,7 @@ VH_KERNEL_GATHER = 3
 VH_KERNEL_TRANSFERS = 4
 VH_KERNEL_FACELIGHT = 5
 VH_KERNEL_MLP = 6
+VH_KERNEL_FINAL = 7
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1

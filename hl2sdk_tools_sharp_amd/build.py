@@ -29,6 +29,7 @@ SOURCES = [
     "vh_bounce.hip",
     "vh_gather.hip",
     "vh_mlp.hip",
+    "vh_final.hip",
 ]
 
 FLAGS = [

@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "vh_rgbe.h"
 #include "vh_traverse.h"
 
 #pragma clang fp contract(off)
@@ -60,6 +61,9 @@ bool gather_view(vh_ctx *c, GatherView *v)
     v->nfaces = G->nfaces;
     v->nsamples = G->nsamples;
     v->face_ptr = (const int32_t *)G->face_ptr.ptr;
+    v->face_nc = (const int32_t *)G->face_nc.ptr;
+    v->h_face_ptr = &G->h_face_ptr;
+    v->h_normals = &G->h_normals;
     v->pos = (const float *)G->pos.ptr;
     v->ldr = (const float4 *)G->out_ldr.ptr;
     v->hdr = (const float4 *)G->out_hdr.ptr;
@@ -418,31 +422,6 @@ __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first
         if ((int64_t)c >= a.nchunks) break;
         light_chunk(a, (int64_t)c, stk);
     }
-}
-
-// StockLightColor.Encode: the largest channel (ties to the later one) sets
-// the exponent, biased exponent - 134 (0 for a zero maximum); each channel is
-// multiplied by 2^-exponent and truncated to a byte as .NET's saturating
-// float -> int conversion does.
-__device__ __forceinline__ int32_t to_int_sat(float x)
-{
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (int32_t)0x80000000;
-    return (int32_t)x;
-}
-
-__device__ __forceinline__ uint32_t encode_one(float r, float g, float b)
-{
-    float mx = r > g ? (r > b ? r : b) : (g > b ? g : b);
-    int e = 0;
-    if (mx != 0.0f) e = (int)((__float_as_uint(mx) & 0x7F800000u) >> 23) - (7 + 127);
-    float scalar = __uint_as_float((uint32_t)(127 - e) << 23);
-    uint32_t R = (uint8_t)to_int_sat(r * scalar);
-    uint32_t G = (uint8_t)to_int_sat(g * scalar);
-    uint32_t B = (uint8_t)to_int_sat(b * scalar);
-    uint32_t E = (uint8_t)(int8_t)e;
-    return R | (G << 8) | (B << 16) | (E << 24);
 }
 
 __global__ void k_encode(const float *__restrict__ rgb, int64_t n, int32_t stride,

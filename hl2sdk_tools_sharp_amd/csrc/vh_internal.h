@@ -94,6 +94,8 @@ struct vh_ctx {
     struct GatherData *gather = nullptr;
     // the warm-start network: weights, optimiser state, data set (vh_mlp.hip)
     struct MlpData *mlp = nullptr;
+    // the luxel radial filter and the lump (vh_final.hip)
+    struct FinalData *final_ = nullptr;
 };
 
 namespace vh {
@@ -126,6 +128,9 @@ void gather_release(vh_ctx *c);
 struct GatherView {
     int32_t nfaces = 0, nsamples = 0;
     const int32_t *face_ptr = nullptr;  // device, nfaces + 1
+    const int32_t *face_nc = nullptr;   // device, normals per face (1, or 4 when bumped)
+    const std::vector<int32_t> *h_face_ptr = nullptr;  // host copies of the two
+    const std::vector<int32_t> *h_normals = nullptr;
     const float *pos = nullptr;         // device, 3 per sample
     const float4 *ldr = nullptr;        // device, slot 0 = the flat normal
     const float4 *hdr = nullptr;
@@ -155,6 +160,8 @@ void bounce_pred_ready(vh_ctx *c);
 void bounce_pull_up(vh_ctx *c, float4 *v, hipStream_t s);
 // Frees the warm-start network's buffers (vh_mlp.hip).
 void mlp_release(vh_ctx *c);
+// Frees the final-light buffers (vh_final.hip).
+void final_release(vh_ctx *c);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

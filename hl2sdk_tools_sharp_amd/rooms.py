@@ -498,3 +498,102 @@ def room_lights(g: RoomGrid, sun: bool = True, sky_ambient: bool = False, spot_e
         L[k]["intensity_hdr"] = (80.0, 120.0, 180.0)
         pvs[k] = 0xFF
     return L, pvs
+
+
+def face_neighbours(g: RoomGrid):
+    """FaceNeighbours for the grid's faces (Rad/Light/FaceNeighbours.cs): the
+    faces that share a corner with a face and smooth with it.  Every face is
+    an axis-aligned rectangle, so two faces smooth exactly when their normals
+    agree (the default threshold passes 0 degrees and stops 90); each list is
+    in ascending face order.  Returns (ptr (F + 1), nbr) as int32."""
+    F = g.nface
+    lo = g.face_lo.astype(np.float64)
+    hi = g.face_hi.astype(np.float64)
+    n = g.face_normal
+    ax = np.argmax(np.abs(n), axis=1)
+    nid = ax * 2 + (n[np.arange(F), ax] > 0)
+    corners = []
+    for su in (0, 1):
+        for sv in (0, 1):
+            c = lo.copy()
+            u, v = (ax + 1) % 3, (ax + 2) % 3
+            rows = np.arange(F)
+            c[rows, u] = np.where(su, hi[rows, u], lo[rows, u])
+            c[rows, v] = np.where(sv, hi[rows, v], lo[rows, v])
+            corners.append(c)
+    C = np.concatenate(corners)                        # (4F, 3)
+    fid = np.tile(np.arange(F), 4)
+    key = np.concatenate([np.round(C * 64).astype(np.int64), np.tile(nid, 4)[:, None]], axis=1)
+    order = np.lexsort(key.T[::-1])
+    key, fid = key[order], fid[order]
+    same = np.all(key[1:] == key[:-1], axis=1)
+    pairs = []
+    for d in range(1, 5):
+        if len(fid) <= d:
+            break
+        run = np.ones(len(fid) - d, bool)
+        for k in range(d):
+            run &= same[k:len(same) - d + 1 + k]
+        a, b = fid[:-d][run], fid[d:][run]
+        pairs.append(np.stack([a, b], 1))
+        pairs.append(np.stack([b, a], 1))
+    P = np.concatenate(pairs) if pairs else np.zeros((0, 2), np.int64)
+    P = P[P[:, 0] != P[:, 1]]
+    P = np.unique(P, axis=0)
+    ptr = np.zeros(F + 1, np.int64)
+    np.add.at(ptr, P[:, 0] + 1, 1)
+    np.cumsum(ptr, out=ptr)
+    return ptr.astype(np.int32), P[:, 1].astype(np.int32)
+
+
+def room_final_inputs(g: RoomGrid, faces, luxel: float = 16.0):
+    """The final stage's inputs for ``room_faces(g, luxel)``: each face's
+    luxel frame, with s along the face's second in-plane axis and t along its
+    first so that luxel s + t * width is sample s + t * width (samples run
+    along the second axis fastest); the luxels are the samples' points; each
+    sample's bounds its cell, (s -+ 0.5, t -+ 0.5); a face without samples
+    (sky) is not lit.  The patches are the grid's, four-corner windings.
+    Returns (FinalFaces, FinalPatches)."""
+    from .final import FACE_DTYPE, FinalFaces, FinalPatches
+
+    F = g.nface
+    fi = np.arange(F)
+    lo = g.face_lo.astype(np.float64)
+    hi = g.face_hi.astype(np.float64)
+    n = g.face_normal
+    ax = np.argmax(np.abs(n), axis=1)
+    u, v = (ax + 1) % 3, (ax + 2) % 3
+    ext_u = hi[fi, u] - lo[fi, u]
+    ext_v = hi[fi, v] - lo[fi, v]
+    nu = np.maximum(1, np.round(ext_u / luxel).astype(np.int64))
+    nv = np.maximum(1, np.round(ext_v / luxel).astype(np.int64))
+    cu, cv = ext_u / nu, ext_v / nv
+    cnt = np.diff(faces.face_sample_ptr)
+    lit = cnt > 0
+    fa = np.zeros(F, FACE_DTYPE)
+    eu = np.zeros((F, 3))
+    ev = np.zeros((F, 3))
+    eu[fi, u] = 1.0
+    ev[fi, v] = 1.0
+    org = lo.copy()
+    org[fi, u] += 0.5 * cu
+    org[fi, v] += 0.5 * cv
+    fa["origin"] = org
+    fa["w2l_s"] = ev / cv[:, None]
+    fa["w2l_t"] = eu / cu[:, None]
+    fa["l2w_s"] = ev * cv[:, None]
+    fa["l2w_t"] = eu * cu[:, None]
+    fa["width"] = np.where(lit, nv, 0)
+    fa["height"] = np.where(lit, nu, 0)
+    fa["lit"] = lit
+    S = int(faces.face_sample_ptr[-1])
+    fs = np.repeat(fi, cnt)
+    k = np.arange(S) - faces.face_sample_ptr[fs]
+    t_, s_ = k // nv[fs], k % nv[fs]
+    bounds = np.stack([s_ - 0.5, t_ - 0.5, s_ + 0.5, t_ + 0.5], 1).astype(np.float32)
+    ptr, nbr = face_neighbours(g)
+    ff = FinalFaces(fa, np.asarray(faces.sample_pos, np.float32), bounds, ptr, nbr)
+    P = g.npatch
+    fp = FinalPatches(g.origin.astype(np.float32), g.face.astype(np.int32), g.child1.astype(np.int32),
+                      (4 * np.arange(P + 1)).astype(np.int32), g.winding.reshape(-1, 3).astype(np.float32))
+    return ff, fp

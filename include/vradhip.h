@@ -326,6 +326,86 @@ int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacit
  * patch, may be NULL) receives a copy. */
 int vh_patch_lights(vh_ctx *ctx, int32_t which, const float *sample_area, float *direct);
 
+/* ---- The final light: FinalLighting.RunAsync / FinalLightFace.Run
+ * (SourceSharp.MapTools/Rad/Final/FinalLightFace.cs:93-299) ----
+ *
+ * One light style, faces that are not displacements, macro textures the
+ * identity, the Correct sides of SampleRadialEdgeOffByOne and
+ * PatchRadialNeighbourBumpFromSelf. */
+
+/* FaceLightInfo's luxel frame (SourceSharp.MapTools/Rad/Light/FaceLightInfo.cs:264-290)
+ * and the face's final-light inputs. */
+typedef struct vh_final_face {
+    float origin[3];       /* LuxelOrigin */
+    float w2l_s[3], w2l_t[3]; /* WorldToLuxelS / T */
+    float l2w_s[3], l2w_t[3]; /* LuxelToWorldS / T */
+    int32_t min_s, min_t;  /* LuxelMinS / T */
+    int32_t width, height; /* the luxel grid (LightmapTextureSizeInLuxels + 1 each) */
+    float minlight;        /* the face entity's _minlight * 128 */
+    int32_t lit;           /* 0: TEX_SPECIAL or no light style; the face gets no lump bytes */
+} vh_final_face;
+
+/* The faces are the loaded gather's, in its order; a face is bumped when its
+ * gather normal count is 4.  luxel_pos: width * height world points per face
+ * (FaceLight.Luxels, face after face; luxel j of a face is its lump entry j).
+ * sample_bounds: per gather sample, min s, min t, max s, max t in its own
+ * face's luxel space (LightSample.MinS ... MaxT).  Neighbours
+ * (FaceNeighbours.Neighbours) as CSR: nbr_ptr (nfaces + 1) into nbr.  The
+ * patch radial's patches (npatch may be 0): origin (3), face, child1 (-1 for
+ * a leaf) and winding points (winding_ptr, npatch + 1 offsets, into winding,
+ * 3 floats a point); each face walks its leaf patches highest index first,
+ * the order PatchSubdivider.RebuildFaceLists leaves FacePatches in
+ * (Rad/Light/PatchSubdivider.cs:408-423).  Everything is copied. */
+typedef struct vh_final_desc {
+    int32_t nfaces;
+    const vh_final_face *faces;
+    const float *luxel_pos;
+    const float *sample_bounds;
+    const int32_t *nbr_ptr, *nbr;
+    int32_t npatch;
+    const float *patch_origin;
+    const int32_t *patch_face, *patch_child1;
+    const int32_t *winding_ptr;
+    const float *winding;
+} vh_final_desc;
+
+/* Uploads the final-light inputs; needs the gather loaded (vh_gather_load)
+ * with the same faces. */
+int vh_final_load(vh_ctx *ctx, const vh_final_desc *desc);
+
+/* LightmapOffsets.Compute for one style (Rad/Light/LightmapOffsets.cs): each
+ * lit face reserves 4 bytes for its average colour, then 4 bytes per luxel per
+ * normal.  offsets (nfaces, may be NULL) receives each face's light offset,
+ * just past its average word (-1 for a face that is not lit); bytes (may be
+ * NULL) the lump's size. */
+int vh_final_layout(vh_ctx *ctx, int32_t *offsets, int64_t *bytes);
+
+#define VH_FINAL_NO_BOUNCE 0     /* no patch radial (Bounces = 0) */
+#define VH_FINAL_BOUNCE_DEVICE 1 /* patch lights = the last vh_bounce's bounced light
+                                    (flat: a bumped face may own no leaf patch) */
+#define VH_FINAL_BOUNCE_HOST 2   /* patch lights from the patch_light argument */
+
+typedef struct vh_final_opts {
+    int32_t which;      /* 0 = the last gather's LDR light, 1 = HDR */
+    int32_t bounce;     /* VH_FINAL_* */
+    int32_t red_errors; /* -rederrors: a luxel nothing reached is 2550 red, not black */
+} vh_final_opts;
+
+/* FinalLightFace.Run for every lit face: LuxelRadial.AddDirect of the face's
+ * samples, then each lit neighbour's re-projected by its bounds' corners
+ * (BuildLuxelRadial, FinalLightFace.cs:301-389); with a bounce,
+ * LuxelRadial.AddBounced of the face's and its neighbours' leaf patches
+ * (BuildPatchRadial, :404-416, 489-536); per luxel Sample of both grids
+ * (LuxelRadial.cs:314-359) added, a face without samples painted
+ * (255, 0, 0), max(value, minlight), the luxels' ColorRGBExp32 at
+ * offset + (b * numLuxels + j) * 4 and the median of the flat values of the
+ * luxels whose flat sample had weight (element count / 2 of the sorted
+ * values, per channel) at offset - 4.  patch_light: TotalLight per normal,
+ * 4 x 3 floats per patch, for VH_FINAL_BOUNCE_HOST (else ignored).  lump
+ * receives vh_final_layout's bytes (capacity at least that). */
+int vh_final_light(vh_ctx *ctx, const vh_final_opts *opts, const float *patch_light, uint8_t *lump,
+                   int64_t capacity);
+
 /* ---- The warm-start network (no counterpart in the reference) ----
  *
  * A per-patch MLP, 25 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form) on
@@ -404,6 +484,7 @@ int vh_mlp_predict(vh_ctx *ctx, const float *emit0, float *start);
 #define VH_KERNEL_TRANSFERS 4
 #define VH_KERNEL_FACELIGHT 5
 #define VH_KERNEL_MLP 6 /* the warm-start network's GEMMs (inference) or training steps */
+#define VH_KERNEL_FINAL 7 /* vh_final_light's launches, bracketed together */
 #define VH_KERNEL_COUNT 8
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);

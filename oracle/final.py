@@ -1,0 +1,324 @@
+"""CPU restatement of vrad's final-light stage -- TEST INFRASTRUCTURE ONLY.
+
+FinalLightFace.Run (Rad/Final/FinalLightFace.cs:93-299) for faces that are
+not displacements, one light style, with the luxel accumulation grid
+LuxelRadial (Rad/Final/LuxelRadial.cs), on the Correct side of the
+compliance switches the stage reads (SampleRadialEdgeOffByOne: a point one
+past the last column or row is off the grid; PatchRadialNeighbourBumpFromSelf:
+a neighbour's patches are bumped when the neighbour is).  Macro textures are
+the identity (the synthetic maps have none).  Every float operation is a
+numpy float32 scalar operation in the reference's order; the double
+promotions the reference makes (EQUAL_EPSILON, 0.1, RADIALDIST, 1.0 / weight)
+are made in Python floats (IEEE double).  Slow on purpose: a plain statement
+of the arithmetic, for small inputs.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .gather import encode_rgbe
+
+f32 = np.float32
+RADIAL_DIST2 = 2                   # RADIALDIST2, LuxelRadial.cs:42
+RADIAL_DIST = 1.42                 # RADIALDIST (double), :45
+WEIGHT_EPS = f32(0.00001)          # WEIGHT_EPS, :48
+ONE_OVER_SQRT3 = f32(0.57735025882720947)  # OO_SQRT_3, :51
+EQUAL_EPSILON_D = 0.001            # LightConstants.EqualEpsilonDouble
+ERROR_RED = (f32(2550.0), f32(0.0), f32(0.0))
+
+
+class Face:
+    """FaceLightInfo's luxel frame (Rad/Light/FaceLightInfo.cs:264-290) and
+    the face's final-light inputs."""
+
+    def __init__(self, origin, w2l_s, w2l_t, l2w_s, l2w_t, min_s, min_t, width, height,
+                 bumped=False, minlight=0.0, lit=True):
+        self.origin = [f32(x) for x in origin]
+        self.ws = [f32(x) for x in w2l_s]
+        self.wt = [f32(x) for x in w2l_t]
+        self.ls = [f32(x) for x in l2w_s]
+        self.lt = [f32(x) for x in l2w_t]
+        self.min_s, self.min_t = int(min_s), int(min_t)
+        self.w, self.h = int(width), int(height)
+        self.nb = 4 if bumped else 1
+        self.minlight = f32(minlight)
+        self.lit = bool(lit)
+
+    def world_to_luxel(self, p):
+        pos = [f32(f32(p[k]) - self.origin[k]) for k in range(3)]
+        s = f32(f32(f32(pos[0] * self.ws[0]) + f32(pos[1] * self.ws[1])) + f32(pos[2] * self.ws[2]))
+        t = f32(f32(f32(pos[0] * self.wt[0]) + f32(pos[1] * self.wt[1])) + f32(pos[2] * self.wt[2]))
+        return f32(s - f32(self.min_s)), f32(t - f32(self.min_t))
+
+    def luxel_to_world(self, s, t):
+        s = f32(f32(s) + f32(self.min_s))
+        t = f32(f32(t) + f32(self.min_t))
+        pos = [f32(self.origin[k] + f32(self.ls[k] * s)) for k in range(3)]
+        return [f32(pos[k] + f32(self.lt[k] * t)) for k in range(3)]
+
+
+def _trunc(x) -> int:
+    """C#'s (int) cast: toward zero."""
+    return int(math.trunc(float(x)))
+
+
+class Radial:
+    """LuxelRadial over one face's grid (radial_t)."""
+
+    def __init__(self, face: Face):
+        self.face = face
+        n = face.w * face.h
+        self.weight = [f32(0.0)] * n
+        self.light = [[[f32(0.0)] * 3 for _ in range(n)] for _ in range(4)]
+
+    def add_direct(self, point, min_s, min_t, max_s, max_t, light, has_bump, nbr_bump):
+        """AddDirect (LuxelRadial.cs:140-208)."""
+        f = self.face
+        cs, ct = f.world_to_luxel(point)
+        min_s, min_t, max_s, max_t = f32(min_s), f32(min_t), f32(max_s), f32(max_t)
+        s_min, t_min = _trunc(min_s), _trunc(min_t)
+        s_max = _trunc(f32(max_s + f32(0.9999))) + 1
+        t_max = _trunc(f32(max_t + f32(0.9999))) + 1
+        s_min, t_min = max(s_min, 0), max(t_min, 0)
+        s_max, t_max = min(s_max, f.w), min(t_max, f.h)
+        for s in range(s_min, s_max):
+            for t in range(t_min, t_max):
+                a = float(f32(min_s - f32(s)))
+                s0 = f32(a if a > -1.0 else -1.0)
+                a = float(f32(min_t - f32(t)))
+                t0 = f32(a if a > -1.0 else -1.0)
+                a = float(f32(max_s - f32(s)))
+                s1 = f32(a if a < 1.0 else 1.0)
+                a = float(f32(max_t - f32(t)))
+                t1 = f32(a if a < 1.0 else 1.0)
+                area = f32(f32(s1 - s0) * f32(t1 - t0))
+                if float(area) > EQUAL_EPSILON_D:
+                    ds = f32(abs(f32(cs - f32(s))))
+                    dt = f32(abs(f32(ct - f32(t))))
+                    r = ds if ds > dt else dt
+                    r = f32(float(area) / 0.1) if float(r) < 0.1 else f32(area / r)
+                    i = s + t * f.w
+                    if has_bump:
+                        if nbr_bump:
+                            for b in range(4):
+                                self._add(b, i, light[b], r)
+                        else:
+                            self._add(0, i, light[0], r)
+                            bw = f32(r * ONE_OVER_SQRT3)
+                            for b in range(1, 4):
+                                self._add(b, i, light[0], bw)
+                    else:
+                        self._add(0, i, light[0], r)
+                    self.weight[i] = f32(self.weight[i] + r)
+
+    def _add(self, b, i, c, r):
+        """LightingValue.AddWeighted: Lighting += weight * src."""
+        L = self.light[b][i]
+        for k in range(3):
+            L[k] = f32(L[k] + f32(r * f32(c[k])))
+
+    def add_bounced(self, point, min_s, min_t, max_s, max_t, light, has_bump, nbr_bump):
+        """AddBounced (LuxelRadial.cs:230-293)."""
+        f = self.face
+        cs, ct = f.world_to_luxel(point)
+        dist_s = f32(f32(max_s) - f32(min_s))
+        dist_t = f32(f32(max_t) - f32(min_t))
+        dist_s = f32(1.0 if 1.0 > float(dist_s) else float(dist_s))
+        dist_t = f32(1.0 if 1.0 > float(dist_t) else float(dist_t))
+        s_min = _trunc(float(cs) - float(dist_s) * RADIAL_DIST)
+        t_min = _trunc(float(ct) - float(dist_t) * RADIAL_DIST)
+        s_max = _trunc(float(cs) + float(dist_s) * RADIAL_DIST + 1.0)
+        t_max = _trunc(float(ct) + float(dist_t) * RADIAL_DIST + 1.0)
+        s_min, t_min = max(s_min, 0), max(t_min, 0)
+        s_max, t_max = min(s_max, f.w), min(t_max, f.h)
+        for s in range(s_min, s_max):
+            for t in range(t_min, t_max):
+                ds = f32(f32(cs - f32(s)) / dist_s)
+                dt = f32(f32(ct - f32(t)) / dist_t)
+                r = f32(f32(RADIAL_DIST2) - f32(f32(ds * ds) + f32(dt * dt)))
+                i = s + t * f.w
+                if r > 0:
+                    if has_bump:
+                        if nbr_bump:
+                            for b in range(4):
+                                self._addc(b, i, light[b], r)
+                        else:
+                            self._addc(0, i, light[0], r)
+                            bw = f32(r * ONE_OVER_SQRT3)
+                            for b in range(1, 4):
+                                self._addc(b, i, light[0], bw)
+                    else:
+                        self._addc(0, i, light[0], r)
+                    self.weight[i] = f32(self.weight[i] + r)
+
+    def _addc(self, b, i, c, r):
+        """Lighting += r * light (Vec3 times float)."""
+        L = self.light[b][i]
+        for k in range(3):
+            L[k] = f32(L[k] + f32(f32(c[k]) * r))
+
+    def sample(self, point, nb, red_errors):
+        """Sample (LuxelRadial.cs:314-359), the Correct edge test."""
+        f = self.face
+        cs, ct = f.world_to_luxel(point)
+        u = _trunc(f32(cs + f32(0.5)))
+        v = _trunc(f32(ct + f32(0.5)))
+        i = u + v * f.w
+        if u < 0 or u >= f.w or v < 0 or v >= f.h:
+            return [list(ERROR_RED) for _ in range(nb)], False
+        ok = True
+        out = []
+        for b in range(nb):
+            wgt = self.weight[i] if i < f.w * f.h else f32(0.0)
+            if wgt > WEIGHT_EPS:
+                sc = f32(1.0 / float(wgt))
+                out.append([f32(self.light[b][i][k] * sc) for k in range(3)])
+            else:
+                out.append(list(ERROR_RED) if red_errors else [f32(0.0)] * 3)
+                if b == 0:
+                    ok = False
+        return out, ok
+
+
+def patch_range(face: Face, winding):
+    """PatchLightmapCoordRange (FinalLightFace.cs:519-536)."""
+    mn_s, mn_t, mx_s, mx_t = f32(1e30), f32(1e30), f32(-1e30), f32(-1e30)
+    for p in winding:
+        s, t = face.world_to_luxel(p)
+        mn_s = mn_s if mn_s < s else s
+        mx_s = mx_s if mx_s > s else s
+        mn_t = mn_t if mn_t < t else t
+        mx_t = mx_t if mx_t > t else t
+    return mn_s, mn_t, mx_s, mx_t
+
+
+def face_patch_lists(nfaces: int, patch_face, patch_leaf) -> List[List[int]]:
+    """Each face's leaf patches in the order FacePatches walks them: the list
+    RebuildFaceLists makes by prepending every patch in index order, so the
+    highest index first (Rad/Light/PatchSubdivider.cs:408-423), less the
+    patches with children (FinalLightFace.cs:500-504)."""
+    lists: List[List[int]] = [[] for _ in range(nfaces)]
+    for p in range(len(patch_face) - 1, -1, -1):
+        if patch_leaf[p]:
+            lists[int(patch_face[p])].append(p)
+    return lists
+
+
+def final_light(faces: Sequence[Face], luxels: Sequence[np.ndarray], samples: Sequence[dict],
+                neighbours: Sequence[Sequence[int]], patches: Optional[dict] = None,
+                red_errors: bool = False, only=None) -> List[Optional[dict]]:
+    """Run's per-face loop for style 0 (FinalLightFace.cs:93-299).
+
+    samples[f]: {'pos' (n,3), 'bounds' (n,4) minS minT maxS maxT in face f's
+    luxel space, 'light' (nb,n,3)}; luxels[f]: (numLuxels,3) world points;
+    patches: None when there is no bounce, else {'origin' (P,3), 'winding'
+    list of (k,3), 'face' (P,), 'leaf' (P,) bool, 'total' (P,4,3) TotalLight
+    per normal}.  Returns per face {'luxels' (nb, numLuxels, 3) after
+    minlight, 'median' (3,)}, or None for a face that is not lit (or not in
+    ``only``, a set of faces to run when given)."""
+    out: List[Optional[dict]] = []
+    plist = None
+    if patches is not None:
+        plist = face_patch_lists(len(faces), patches["face"], patches["leaf"])
+    for fn, face in enumerate(faces):
+        if not face.lit or (only is not None and fn not in only):
+            out.append(None)
+            continue
+        nb = face.nb
+        bumped = nb == 4
+        rad = Radial(face)
+        sm = samples[fn]
+        # BuildLuxelRadial (FinalLightFace.cs:301-389): own samples, then each
+        # neighbour's, their bounds carried corner by corner into this grid
+        for k in range(len(sm["pos"])):
+            rad.add_direct(sm["pos"][k], *sm["bounds"][k], [sm["light"][b][k] for b in range(nb)],
+                           bumped, bumped)
+        for n in neighbours[fn]:
+            nface, ns = faces[n], samples[n]
+            if not nface.lit or len(ns["pos"]) == 0:
+                continue
+            nbump = nface.nb == 4
+            for k in range(len(ns["pos"])):
+                b0 = ns["bounds"][k]
+                tmp = nface.luxel_to_world(b0[0], b0[1])
+                mn_s, mn_t = face.world_to_luxel(tmp)
+                tmp = nface.luxel_to_world(b0[2], b0[3])
+                mx_s, mx_t = face.world_to_luxel(tmp)
+                light = [ns["light"][b][k] for b in range(nface.nb)]
+                rad.add_direct(ns["pos"][k], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
+        prad = None
+        if plist is not None:
+            # BuildPatchRadial (FinalLightFace.cs:404-416, 489-516): the face's
+            # leaf patches, then each neighbour's; a neighbour's patches are
+            # bumped when that neighbour is
+            prad = Radial(face)
+            for src in [fn] + list(neighbours[fn]):
+                nbump = bumped if src == fn else faces[src].nb == 4
+                for p in plist[src]:
+                    mn_s, mn_t, mx_s, mx_t = patch_range(face, patches["winding"][p])
+                    light = [np.asarray(patches["total"][p][b], np.float32) for b in range(4)]
+                    prad.add_bounced(patches["origin"][p], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
+        lum = np.zeros((nb, len(luxels[fn]), 3), np.float32)
+        red_, green_, blue_ = [], [], []
+        no_samples = len(sm["pos"]) == 0
+        for j in range(len(luxels[fn])):
+            lb, ok = rad.sample(luxels[fn][j], nb, red_errors)
+            if prad is not None:
+                v, _ = prad.sample(luxels[fn][j], nb, red_errors)
+                for b in range(nb):
+                    lb[b] = [f32(lb[b][k] + v[b][k]) for k in range(3)]
+            if no_samples:
+                lb = [[f32(255.0), f32(0.0), f32(0.0)] for _ in range(nb)]
+                ok = False
+            for b in range(nb):
+                c = lb[b]
+                c = [c[k] if c[k] > face.minlight else face.minlight for k in range(3)]
+                if b == 0 and ok:
+                    red_.append(c[0])
+                    green_.append(c[1])
+                    blue_.append(c[2])
+                lum[b, j] = c
+        med = np.zeros(3, np.float32)
+        if red_:
+            med = np.array([median(red_), median(green_), median(blue_)], np.float32)
+        out.append({"luxels": lum, "median": med})
+    return out
+
+
+def median(values) -> np.float32:
+    """Median (FinalLightFace.cs:466-472): element count / 2 of the ascending
+    order."""
+    v = sorted(values)
+    return v[len(v) >> 1]
+
+
+def layout(faces: Sequence[Face]):
+    """LightmapOffsets.Compute for one style (Rad/Light/LightmapOffsets.cs):
+    each lit face reserves its average-colour word, then 4 bytes per luxel
+    per normal; its light offset is just past the average word.  Returns
+    (offsets, -1 for a face not lit; total bytes)."""
+    offs, size = [], 0
+    for face in faces:
+        if not face.lit:
+            offs.append(-1)
+            continue
+        size += 4
+        offs.append(size)
+        size += 4 * face.nb * face.w * face.h
+    return offs, size
+
+
+def lump_bytes(faces: Sequence[Face], result: Sequence[Optional[dict]]) -> bytes:
+    """The stage's lump for one style: for each lit face in order, the
+    average colour word, then its luxels bump map by bump map (the face's
+    light offset is 4 bytes past its start: FinalLightFace.cs:270, 279)."""
+    parts = []
+    for face, r in zip(faces, result):
+        if r is None:
+            continue
+        parts.append(encode_rgbe(r["median"].reshape(1, 3)).tobytes())
+        parts.append(encode_rgbe(r["luxels"].reshape(-1, 3)).tobytes())
+    return b"".join(parts)
