@@ -229,6 +229,33 @@ def batching_leg(a, log):
     return out
 
 
+def decode_lump(lump: np.ndarray) -> np.ndarray:
+    """ColorRGBExp32 words back to linear RGB: byte * 2^exponent."""
+    w = np.asarray(lump, np.uint8).reshape(-1, 4)
+    return w[:, :3].astype(np.float64) * np.exp2(w[:, 3].view(np.int8).astype(np.float64))[:, None]
+
+
+def lump_rel_l1(a: np.ndarray, b: np.ndarray) -> float:
+    """Relative L1 distance of two lumps' decoded HDR values (b the reference)."""
+    x, y = decode_lump(a), decode_lump(b)
+    return float(np.abs(x - y).sum() / max(np.abs(y).sum(), 1e-300))
+
+
+def final_lump_cpu(fin, faces, light, total, threads):
+    """The final stage restated on the CPU (vradcpu_final.c) with bounced
+    light ``total`` (P x 3, flat)."""
+    import oracle.final as of
+
+    ff, fp = fin
+    pl = np.zeros((fp.npatch, 4, 3), np.float32)
+    pl[:, 0] = total
+    pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
+          "winding": fp.winding}
+    return of.final_light_c(ff.faces, ff.luxel_pos, ff.sample_bounds, ff.nbr_ptr, ff.nbr,
+                            faces.face_sample_ptr, faces.face_normal_count, faces.sample_pos, light, pd, pl,
+                            threads=threads)
+
+
 def plain_vrad_leg(a, log):
     """The plain-vrad config: a 3 x 3 grid of 512 x 512 x 256 rooms (8-unit
     walls, 128 x 128 doorways) under a sky ceiling, 8 point lights
@@ -246,7 +273,8 @@ def plain_vrad_leg(a, log):
                         nlights=8, sky_ceiling=True, seed=1)
     faces = rooms.room_faces(g, luxel=16.0)
     lights, pvs = rooms.room_lights(g, sun=True)
-    kw = dict(sun_extent=0.0, sun_samples=1, tol=1e-4)
+    fin = rooms.room_final_inputs(g, faces)
+    kw = dict(sun_extent=0.0, sun_samples=1, tol=1e-4, final_inputs=fin)
     log(f"[bench] plain vrad: {g.npatch} patches, {faces.nsamples} luxels, {len(lights)} lights")
     light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
     runs = [light_pass(g, faces, lights, pvs, read_direct=True, **kw) for _ in range(3)]
@@ -264,22 +292,25 @@ def plain_vrad_leg(a, log):
         direct = ob.patch_lights(g, faces, ldr[0], faces.sample_area)
         rp, src, w, _nc = ob.build_transfers(g, threads=threads)
         tot, _res, _en, it = ob.bounce(g, rp, src, w, direct, tol=1e-4, max_iter=100)
-        parts = [ldr[0, faces.face_sample_ptr[f]:faces.face_sample_ptr[f + 1], :3] for f in range(faces.nfaces)]
-        lump = og.encode_rgbe(np.concatenate(parts)).reshape(-1)
+        lump, _ = final_lump_cpu(fin, faces, ldr, tot, threads)
         dt = time.perf_counter() - t0
+        # the GPU's final stage on the GPU's own bounce, restated on the CPU
+        lump_same_bounce, _ = final_lump_cpu(fin, faces, ldr, best.bounce.total_light, threads)
         b = best.bounce.total_light.astype(np.float64)
         out["cpu_baseline"] = {
             "value": dt, "unit": "s", "cores": threads, "kind": "port",
-            "sample": "the whole pass (gather, patch lights, transfer build, bounce, RGBE lump); "
-            "CPU restatement of the reference's stages, pthreads where the stage is parallel",
+            "sample": "the whole pass (gather, patch lights, transfer build, bounce, final light "
+            "and lump); CPU restatement of the reference's stages, pthreads where the stage is parallel",
         }
         out["speedup_vs_cpu"] = dt / best.wall_s
-        out["lump_equal"] = bool(np.array_equal(lump, best.lump))
+        out["lump_equal"] = bool(np.array_equal(lump_same_bounce, best.lump))
+        out["lump_rel_l1_vs_cpu"] = lump_rel_l1(best.lump, lump)
         out["direct_equal"] = bool(np.array_equal(direct.view(np.uint32), best.direct.view(np.uint32)))
         out["bounce_rel_l1_vs_cpu"] = float(np.abs(b - tot).sum() / max(np.abs(tot).sum(), 1e-30))
         out["cpu_iterations"] = it
-        log(f"[bench] plain vrad CPU {dt:.2f} s on {threads} threads; lump equal {out['lump_equal']}, "
-            f"direct equal {out['direct_equal']}, bounce rel L1 {out['bounce_rel_l1_vs_cpu']:.2e}")
+        log(f"[bench] plain vrad CPU {dt:.2f} s on {threads} threads; lump equal on the same bounce "
+            f"{out['lump_equal']}, direct equal {out['direct_equal']}, bounce rel L1 "
+            f"{out['bounce_rel_l1_vs_cpu']:.2e}, lump rel L1 {out['lump_rel_l1_vs_cpu']:.2e}")
     return out
 
 
@@ -378,7 +409,8 @@ def bake_leg(a, log, model):
     """The 200k-patch bake end to end, both forms on the bounce config's map
     (12 x 12 rooms, 202,752 patches, 288 point lights behind room PVS rows,
     luxel 16): scene, gather (HDR pass), patch lights, transfer build, bounce to
-    residual 1e-4, lump.
+    residual 1e-4, final light (radial filter of the luxels and the bounced
+    light) and lump.
       plain:     one gather launch per face, FP32 weights, bounce from zero;
       optimised: the persistent gather queue, FP16 weights, bounce from the
                  warm-start network's prediction (trained in the warm-start leg).
@@ -392,8 +424,9 @@ def bake_leg(a, log, model):
     g = rooms.bounce_grid(12, 12)
     faces = rooms.room_faces(g, luxel=16.0)
     lights, pvs = rooms.room_lights(g, sun=False)
+    fin = rooms.room_final_inputs(g, faces)
     log(f"[bench] bake: {g.npatch} patches, {faces.nsamples} luxels, {len(lights)} lights")
-    kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True)
+    kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True, final_inputs=fin)
     light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
     plain = light_pass(g, faces, lights, pvs, schedule="per_face", weights="f32", **kw)
     fast = light_pass(g, faces, lights, pvs, schedule="queue", weights="f16", warm_start=model, **kw)
@@ -404,7 +437,7 @@ def bake_leg(a, log, model):
         log(f"[bench] bake {name}: {r.wall_s:.3f} s, {r.bounce.iterations} iterations: "
             + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in r.stage_s.items()))
     out["speedup"] = plain.wall_s / fast.wall_s
-    out["lump_equal"] = bool(np.array_equal(plain.lump, fast.lump))
+    out["lump_rel_l1_optimised_vs_plain"] = lump_rel_l1(fast.lump, plain.lump)
     if not a.no_cpu:
         offer = HipTracerFactory(device=0).try_create(g.casters)
         with offer.tracer as tr:
@@ -419,9 +452,30 @@ def bake_leg(a, log, model):
         for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
             d = np.abs((r.bounce.total_light.astype(np.float64) - ref) * leaf).sum()
             out[name]["rel_l1_vs_fp64"] = float(d / max(np.abs(ref * leaf).sum(), 1e-300))
-        log(f"[bench] bake vs FP64 oracle ({it} iterations, {out['fp64_oracle_s']:.1f} s): plain "
-            f"{out['plain_f32']['rel_l1_vs_fp64']:.2e}, optimised {out['optimised_f16_warm']['rel_l1_vs_fp64']:.2e}")
-    log(f"[bench] bake speed-up {out['speedup']:.1f}x; lumps equal: {out['lump_equal']}")
+        # the lightmaps: each form's lump against the lump the same final
+        # stage makes from the FP64 bounce
+        from hl2sdk_tools_sharp_amd.final import HipFinalLighter
+        from hl2sdk_tools_sharp_amd.light import HipFaceLighter
+
+        offer = HipTracerFactory(device=0).try_create(g.casters)
+        with offer.tracer as tr:
+            fl = HipFaceLighter(tr)
+            fl.load(faces, lights, light_pvs=pvs, sun_samples=1)
+            fl.run()
+            fr = HipFinalLighter(fl)
+            fr.load(*fin)
+            pl = np.zeros((g.npatch, 4, 3), np.float32)
+            pl[:, 0] = ref
+            ref_lump, out["final_kernel_ms"] = fr.timed_run(hdr=True, bounce="host", patch_light=pl)
+        for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+            out[name]["lump_hdr_rel_l1_vs_fp64"] = lump_rel_l1(r.lump, ref_lump)
+        log(f"[bench] bake vs FP64 oracle ({it} iterations, {out['fp64_oracle_s']:.1f} s): bounced light "
+            f"plain {out['plain_f32']['rel_l1_vs_fp64']:.2e}, optimised "
+            f"{out['optimised_f16_warm']['rel_l1_vs_fp64']:.2e}; lightmap HDR plain "
+            f"{out['plain_f32']['lump_hdr_rel_l1_vs_fp64']:.2e}, optimised "
+            f"{out['optimised_f16_warm']['lump_hdr_rel_l1_vs_fp64']:.2e}")
+    log(f"[bench] bake speed-up {out['speedup']:.1f}x; lightmap HDR rel L1 optimised vs plain "
+        f"{out['lump_rel_l1_optimised_vs_plain']:.2e}")
     return out
 
 

@@ -7,9 +7,10 @@ the radiosity patches, Rad/Light/PatchLighting.cs), the transfer build
 IVradBounceStage, Rad/VradStages.cs:112-122) and the lump write
 (Rad/Final/FinalLightFace.cs with StockLightColor.Encode).  Here every stage
 runs in libvradhip.so against one scene upload; the patch lights stay on the
-device and feed the bounce without a host round trip.  The lump holds the
-direct light (the luxel radial filter that folds bounced light back into
-luxels is not part of this pass).
+device and feed the bounce without a host round trip, and the final stage
+(the luxel radial filter, FinalLightFace.Run) reads the gather's samples and
+the bounced light where they lie.  Without final-stage inputs the lump holds
+each sample's direct light, encoded in sample order.
 """
 from __future__ import annotations
 
@@ -44,12 +45,15 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
                sun_extent: float = 0.0, sun_samples: int = 30, sky_dirs=None, ao_count: int = 0,
                ao_radius: float = 40.0, weights: str = "f32", tol: float = 1e-4, max_iter: int = 100,
                device: int = 0, read_direct: bool = False, schedule: str = "queue",
-               warm_start: Optional[dict] = None) -> PassResult:
+               warm_start: Optional[dict] = None, final_inputs=None) -> PassResult:
     """Runs one pass (LDR, or HDR with ``hdr``) over room grid ``g`` whose
     faces, in the patch system's order, are ``faces``.  ``schedule`` is the
     gather's ('queue', or 'per_face' for one launch per face); ``warm_start``
     (the network's ``params``, ``gain`` and ``scale``) starts the bounce from
-    the network's prediction instead of from zero.  Raises if the HIP tracer
+    the network's prediction instead of from zero.  ``final_inputs``
+    ((FinalFaces, FinalPatches), e.g. rooms.room_final_inputs) makes the lump
+    the final stage's: luxels filtered from the samples plus the bounced light
+    of the patches round them, minlight, medians.  Raises if the HIP tracer
     declines (there is no CPU path)."""
     out = PassResult()
     clock = time.perf_counter
@@ -91,7 +95,14 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
         out.stage_s["bounce"] = clock() - t
 
         t = clock()
-        out.lump = fl.lightmap(hdr=hdr)
+        if final_inputs is not None:
+            from .final import HipFinalLighter
+
+            fin = HipFinalLighter(fl)
+            fin.load(*final_inputs)
+            out.lump = fin.run(hdr=hdr, bounce="device")
+        else:
+            out.lump = fl.lightmap(hdr=hdr)
         out.stage_s["lump"] = clock() - t
     finally:
         tr.dispose()

@@ -322,3 +322,78 @@ def lump_bytes(faces: Sequence[Face], result: Sequence[Optional[dict]]) -> bytes
         parts.append(encode_rgbe(r["median"].reshape(1, 3)).tobytes())
         parts.append(encode_rgbe(r["luxels"].reshape(-1, 3)).tobytes())
     return b"".join(parts)
+
+
+# --- the C statement (vradcpu_final.c), for maps of realistic size ---
+
+FACE_DTYPE = np.dtype([
+    ("origin", "<f4", 3), ("w2l_s", "<f4", 3), ("w2l_t", "<f4", 3), ("l2w_s", "<f4", 3),
+    ("l2w_t", "<f4", 3), ("min_s", "<i4"), ("min_t", "<i4"), ("width", "<i4"), ("height", "<i4"),
+    ("minlight", "<f4"), ("lit", "<i4"),
+])
+
+
+def final_light_c(faces: np.ndarray, luxel_pos, sample_bounds, nbr_ptr, nbr, face_sample_ptr,
+                  face_normal_count, sample_pos, light, patches: Optional[dict] = None,
+                  patch_light=None, red_errors: bool = False, threads: int = 1):
+    """vc_final_light over whole maps: faces (FACE_DTYPE), the luxels, the
+    samples' bounds, neighbour CSR, the gather's faces, sample points and
+    light ((4, S, 4)), and for a bounce the patches ({'origin', 'face',
+    'child1', 'winding_ptr', 'winding'}) with their TotalLight per normal
+    (npatch, 4, 3).  Returns (lump bytes as uint8, offsets)."""
+    import ctypes
+
+    from . import lib
+
+    L = lib()
+    P = ctypes.c_void_p
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("nfaces", ctypes.c_int32), ("faces", P), ("luxel_pos", P), ("sample_bounds", P),
+                    ("nbr_ptr", P), ("nbr", P), ("npatch", ctypes.c_int32), ("patch_origin", P),
+                    ("patch_face", P), ("patch_child1", P), ("winding_ptr", P), ("winding", P),
+                    ("face_sample_ptr", P), ("face_normal_count", P), ("sample_pos", P),
+                    ("nsamples", ctypes.c_int32), ("light", P), ("patch_light", P),
+                    ("red_errors", ctypes.c_int32)]
+
+    keep = []
+
+    def c(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data if a.size else None
+
+    fa = np.ascontiguousarray(faces, FACE_DTYPE)
+    keep.append(fa)
+    d = Desc()
+    d.nfaces = len(fa)
+    d.faces = fa.ctypes.data
+    d.luxel_pos = c(luxel_pos, np.float32)
+    d.sample_bounds = c(sample_bounds, np.float32)
+    d.nbr_ptr = c(nbr_ptr, np.int32)
+    d.nbr = c(nbr, np.int32)
+    d.face_sample_ptr = c(face_sample_ptr, np.int32)
+    d.face_normal_count = c(face_normal_count, np.int32)
+    d.sample_pos = c(sample_pos, np.float32)
+    d.nsamples = int(np.asarray(sample_pos).shape[0])
+    d.light = c(light, np.float32)
+    d.red_errors = 1 if red_errors else 0
+    if patches is not None and patch_light is not None:
+        d.npatch = len(patches["origin"])
+        d.patch_origin = c(patches["origin"], np.float32)
+        d.patch_face = c(patches["face"], np.int32)
+        d.patch_child1 = c(patches["child1"], np.int32)
+        d.winding_ptr = c(patches["winding_ptr"], np.int32)
+        d.winding = c(patches["winding"], np.float32)
+        d.patch_light = c(patch_light, np.float32)
+    L.vc_final_layout.restype = ctypes.c_int64
+    L.vc_final_layout.argtypes = [P, P]
+    L.vc_final_light.restype = ctypes.c_int
+    L.vc_final_light.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
+    offs = np.zeros(max(len(fa), 1), np.int32)
+    size = L.vc_final_layout(ctypes.byref(d), offs.ctypes.data)
+    out = np.zeros(max(size, 1), np.uint8)
+    rc = L.vc_final_light(ctypes.byref(d), out.ctypes.data, out.shape[0], int(threads))
+    if rc != 0:
+        raise RuntimeError("vc_final_light failed")
+    return out[:size], offs[:len(fa)]

@@ -155,6 +155,45 @@ void vc_encode_rgbe(const float *rgb, int64_t n, int32_t stride, uint8_t *out);
 void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
                float *out_ao, int32_t threads);
 
+/* The final light (vradcpu_final.c): FinalLightFace.Run with LuxelRadial,
+ * one style.  Faces as FaceLightInfo frames; luxel_pos width * height points
+ * per face; per sample its bounds in its own face's luxel space; neighbour
+ * CSR; patches (origin, face, child1, winding CSR); the gather's samples and
+ * light ([4][nsamples][4] floats); patch_light npatch x 4 x 3 (TotalLight per
+ * normal) or NULL for no bounce.  Pinned to oracle/final.py (itself pinned by
+ * the reference's LuxelRadial and FinalLightFace closed forms) on seeded
+ * random faces, byte for byte. */
+typedef struct vc_final_face {
+    float origin[3], w2l_s[3], w2l_t[3], l2w_s[3], l2w_t[3];
+    int32_t min_s, min_t, width, height;
+    float minlight;
+    int32_t lit;
+} vc_final_face;
+
+typedef struct vc_final_desc {
+    int32_t nfaces;
+    const vc_final_face *faces;
+    const float *luxel_pos;
+    const float *sample_bounds;
+    const int32_t *nbr_ptr, *nbr;
+    int32_t npatch;
+    const float *patch_origin;
+    const int32_t *patch_face, *patch_child1;
+    const int32_t *winding_ptr;
+    const float *winding;
+    const int32_t *face_sample_ptr, *face_normal_count;
+    const float *sample_pos;
+    int32_t nsamples;
+    const float *light;
+    const float *patch_light;
+    int32_t red_errors;
+} vc_final_desc;
+
+/* The lump size; offsets (nfaces, may be NULL) each lit face's light offset, -1 otherwise. */
+int64_t vc_final_layout(const vc_final_desc *d, int32_t *offsets);
+/* The lump (capacity >= vc_final_layout's size, else -1). */
+int vc_final_light(const vc_final_desc *d, uint8_t *lump, int64_t capacity, int32_t threads);
+
 #ifdef __cplusplus
 }
 #endif

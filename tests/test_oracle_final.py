@@ -254,3 +254,25 @@ def test_layout_reserves_the_average_word_before_each_lit_face():
     offs, size = fo.layout(faces)
     assert offs == [4, -1, 4 + 24 + 4]
     assert size == 4 + 24 + 4 + 16
+
+
+@pytest.mark.parametrize("seed,red,bounce", [(1, False, False), (2, True, True), (3, False, True),
+                                             (5, True, False)])
+def test_the_c_statement_equals_the_python_one(seed, red, bounce):
+    """vradcpu_final.c, used for whole maps, against this module's statement
+    on seeded random faces: every lump byte."""
+    from final_cases import oracle_inputs, random_case
+
+    _, _, fset, ff, fp = random_case(seed)
+    rng = np.random.default_rng(seed + 10)
+    light = rng.uniform(0, 8, (4, fset.nsamples, 4)).astype(np.float32)
+    total = rng.uniform(0, 30, (fp.npatch, 4, 3)).astype(np.float32) if bounce else None
+    faces, lux, sm, nb, pt = oracle_inputs(ff, fset, light, fp if bounce else None, total)
+    want = fo.lump_bytes(faces, fo.final_light(faces, lux, sm, nb, pt, red))
+    pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
+          "winding": fp.winding}
+    got, offs = fo.final_light_c(ff.faces, ff.luxel_pos, ff.sample_bounds, ff.nbr_ptr, ff.nbr,
+                                 fset.face_sample_ptr, fset.face_normal_count, fset.sample_pos, light,
+                                 pd if bounce else None, total, red, threads=3)
+    assert list(offs) == fo.layout(faces)[0]
+    assert got.tobytes() == want
