@@ -64,7 +64,7 @@ struct FaceDev {
 
 struct FinalArgs {
     const FaceDev *faces;
-    const float *luxel_pos;     // 3 per luxel
+    const float *luxel_pos;     // 3 per luxel, or null: LuxelToWorld of the grid points
     const float4 *bounds;       // per gather sample
     const int32_t *nbr_ptr, *nbr;
     const int32_t *plist_ptr, *plist;  // leaf patches per face, in FacePatches order
@@ -343,8 +343,15 @@ __device__ void final_face(const FinalArgs &a, int32_t face)
     for (int32_t j = lane; j - lane < n; j += kWave) {
         bool ok = false;
         if (j < n) {
-            const float *p = a.luxel_pos + (size_t)(F.lux_base + j) * 3;
-            float2 cc = world_to_luxel(F, p[0], p[1], p[2]);
+            float3 p;
+            if (a.luxel_pos) {
+                const float *q = a.luxel_pos + (size_t)(F.lux_base + j) * 3;
+                p = make_float3(q[0], q[1], q[2]);
+            } else {
+                // BuildFaceLuxels (Rad/Light/FaceSampleBuilder.cs:242-259)
+                p = luxel_to_world(F, (float)(j % w), (float)(j / w));
+            }
+            float2 cc = world_to_luxel(F, p.x, p.y, p.z);
             int32_t u = (int32_t)(cc.x + 0.5f), v = (int32_t)(cc.y + 0.5f);
             bool off = u < 0 || u >= w || v < 0 || v >= h;
             int32_t i = off ? 0 : u + v * w;
@@ -438,6 +445,7 @@ struct FinalData {
     int64_t nluxels = 0, ncells = 0, lump_bytes = 0, ntiles = 0;
     std::vector<int32_t> h_face_ptr, h_normals, h_offsets;
     bool bumped_owner = false;  // some bumped face owns a leaf patch
+    bool have_luxels = false;   // luxel points given (else made from the frames)
     vh::DevBuf faces, luxel_pos, bounds, nbr_ptr, nbr, plist_ptr, plist, porigin, wptr, wpts;
     vh::DevBuf plight, tiles, lit_faces, cells, pcells, mvals, lump;
 };
@@ -540,10 +548,6 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
             return VH_EINVAL;
         }
     }
-    if (lux > 0 && !d->luxel_pos) {
-        set_error("luxel_pos is required");
-        return VH_EINVAL;
-    }
     if (F > 0 && d->nbr_ptr[0] != 0) {
         set_error("nbr_ptr must start at 0");
         return VH_EINVAL;
@@ -610,6 +614,7 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
     D->h_normals = nc;
     D->h_offsets = offsets;
     D->bumped_owner = bumped_owner;
+    D->have_luxels = d->luxel_pos != nullptr;
     std::vector<float4> porg((size_t)std::max(P, 1));
     for (int32_t p = 0; p < P; p++)
         porg[p] = make_float4(d->patch_origin[3 * p], d->patch_origin[3 * p + 1],
@@ -618,7 +623,7 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
     const int32_t zero[2] = {0, 0};
     int rc;
     if ((rc = put(D->faces, fd.data(), (size_t)std::max(F, 1), "final faces", st)) ||
-        (rc = put(D->luxel_pos, d->luxel_pos, (size_t)lux * 3, "luxel positions", st)) ||
+        (rc = put(D->luxel_pos, d->luxel_pos, d->luxel_pos ? (size_t)lux * 3 : 0, "luxel positions", st)) ||
         (rc = put(D->bounds, d->sample_bounds, (size_t)gv.nsamples * 4, "sample bounds", st)) ||
         (rc = put(D->nbr_ptr, F > 0 ? d->nbr_ptr : zero, (size_t)F + 1, "neighbour offsets", st)) ||
         (rc = put(D->nbr, d->nbr, (size_t)nn, "neighbours", st)) ||
@@ -723,7 +728,7 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
     }
     FinalArgs a;
     a.faces = (const FaceDev *)D->faces.ptr;
-    a.luxel_pos = (const float *)D->luxel_pos.ptr;
+    a.luxel_pos = D->have_luxels ? (const float *)D->luxel_pos.ptr : nullptr;
     a.bounds = (const float4 *)D->bounds.ptr;
     a.nbr_ptr = (const int32_t *)D->nbr_ptr.ptr;
     a.nbr = (const int32_t *)D->nbr.ptr;

@@ -54,11 +54,13 @@ class VhFinalOpts(ctypes.Structure):
 class FinalFaces:
     """Per face its luxel frame (FaceLightInfo) and final-light inputs
     (``faces``, FACE_DTYPE), the luxels' world points (width * height per
-    face, face after face), per gather sample its bounds in its face's luxel
-    space (min s, min t, max s, max t) and the neighbour lists as CSR."""
+    face, face after face; None for BuildFaceLuxels' LuxelToWorld(s, t) of
+    every grid point, made on the device), per gather sample its bounds in
+    its face's luxel space (min s, min t, max s, max t) and the neighbour
+    lists as CSR."""
 
     faces: np.ndarray
-    luxel_pos: np.ndarray
+    luxel_pos: Optional[np.ndarray]
     sample_bounds: np.ndarray
     nbr_ptr: np.ndarray
     nbr: np.ndarray
@@ -97,6 +99,8 @@ class HipFinalLighter:
         keep = {}
 
         def c(name, a, dt):
+            if a is None:
+                return None
             keep[name] = np.ascontiguousarray(a, dtype=dt)
             return keep[name].ctypes.data if keep[name].size else None
 

@@ -550,7 +550,8 @@ def room_final_inputs(g: RoomGrid, faces, luxel: float = 16.0):
     """The final stage's inputs for ``room_faces(g, luxel)``: each face's
     luxel frame, with s along the face's second in-plane axis and t along its
     first so that luxel s + t * width is sample s + t * width (samples run
-    along the second axis fastest); the luxels are the samples' points; each
+    along the second axis fastest); the luxels are BuildFaceLuxels' grid
+    points (luxel_pos None: made on the device from the frames); each
     sample's bounds its cell, (s -+ 0.5, t -+ 0.5); a face without samples
     (sky) is not lit.  The patches are the grid's, four-corner windings.
     Returns (FinalFaces, FinalPatches)."""
@@ -592,7 +593,7 @@ def room_final_inputs(g: RoomGrid, faces, luxel: float = 16.0):
     t_, s_ = k // nv[fs], k % nv[fs]
     bounds = np.stack([s_ - 0.5, t_ - 0.5, s_ + 0.5, t_ + 0.5], 1).astype(np.float32)
     ptr, nbr = face_neighbours(g)
-    ff = FinalFaces(fa, np.asarray(faces.sample_pos, np.float32), bounds, ptr, nbr)
+    ff = FinalFaces(fa, None, bounds, ptr, nbr)
     P = g.npatch
     fp = FinalPatches(g.origin.astype(np.float32), g.face.astype(np.int32), g.child1.astype(np.int32),
                       (4 * np.arange(P + 1)).astype(np.int32), g.winding.reshape(-1, 3).astype(np.float32))

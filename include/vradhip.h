@@ -347,7 +347,9 @@ typedef struct vh_final_face {
 
 /* The faces are the loaded gather's, in its order; a face is bumped when its
  * gather normal count is 4.  luxel_pos: width * height world points per face
- * (FaceLight.Luxels, face after face; luxel j of a face is its lump entry j).
+ * (FaceLight.Luxels, face after face; luxel j of a face is its lump entry j),
+ * or NULL for BuildFaceLuxels' points, LuxelToWorld(s, t) of every grid point
+ * s + t * width (Rad/Light/FaceSampleBuilder.cs:242-259), made on the device.
  * sample_bounds: per gather sample, min s, min t, max s, max t in its own
  * face's luxel space (LightSample.MinS ... MaxT).  Neighbours
  * (FaceNeighbours.Neighbours) as CSR: nbr_ptr (nfaces + 1) into nbr.  The

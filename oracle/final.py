@@ -359,6 +359,8 @@ def final_light_c(faces: np.ndarray, luxel_pos, sample_bounds, nbr_ptr, nbr, fac
     keep = []
 
     def c(a, dt):
+        if a is None:
+            return None
         a = np.ascontiguousarray(a, dt)
         keep.append(a)
         return a.ctypes.data if a.size else None

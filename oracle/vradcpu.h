@@ -160,7 +160,7 @@ void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float
  * per face; per sample its bounds in its own face's luxel space; neighbour
  * CSR; patches (origin, face, child1, winding CSR); the gather's samples and
  * light ([4][nsamples][4] floats); patch_light npatch x 4 x 3 (TotalLight per
- * normal) or NULL for no bounce.  Pinned to oracle/final.py (itself pinned by
+ * normal) or NULL for no bounce.  luxel_pos NULL: BuildFaceLuxels' grid points.  Pinned to oracle/final.py (itself pinned by
  * the reference's LuxelRadial and FinalLightFace closed forms) on seeded
  * random faces, byte for byte. */
 typedef struct vc_final_face {

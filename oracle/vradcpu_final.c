@@ -283,7 +283,12 @@ static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, floa
     uint8_t *out = j->lump + j->light_ofs[fn];
     int64_t cnt = 0;
     for (int64_t jx = 0; jx < n; jx++) {
-        const float *pt = d->luxel_pos + (j->lux_base[fn] + jx) * 3;
+        float grid[3];
+        const float *pt = grid;
+        if (d->luxel_pos)
+            pt = d->luxel_pos + (j->lux_base[fn] + jx) * 3;
+        else /* BuildFaceLuxels (Rad/Light/FaceSampleBuilder.cs:242-259) */
+            l2w(face, (float)(jx % face->width), (float)(jx / face->width), grid);
         int ok = sample(rad, pt, d->red_errors, lb);
         if (bounce) {
             sample(prad, pt, d->red_errors, pv);

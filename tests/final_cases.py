@@ -32,7 +32,11 @@ def oracle_inputs(ff: FinalFaces, fset: FaceSet, light: np.ndarray, fp: FinalPat
     faces = oracle_faces(ff, fset)
     n = ff.faces["width"].astype(np.int64) * ff.faces["height"]
     lb = np.concatenate([[0], np.cumsum(n)])
-    luxels = [ff.luxel_pos[lb[f]:lb[f + 1]] for f in range(len(faces))]
+    if ff.luxel_pos is not None:
+        luxels = [ff.luxel_pos[lb[f]:lb[f + 1]] for f in range(len(faces))]
+    else:  # BuildFaceLuxels: LuxelToWorld(s, t) of every grid point
+        luxels = [np.array([face.luxel_to_world(s, t) for t in range(face.h) for s in range(face.w)],
+                           np.float32).reshape(-1, 3) for face in faces]
     samples = []
     for f, face in enumerate(faces):
         s0, s1 = fset.face_sample_ptr[f], fset.face_sample_ptr[f + 1]

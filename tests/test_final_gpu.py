@@ -39,9 +39,13 @@ def check_lump(lump, size, want):
                                  f"{g[bad[:3]].tolist()} vs {e[bad[:3]].tolist()}")
 
 
-@pytest.mark.parametrize("seed,hdr,red", [(1, False, False), (2, True, True), (3, False, True)])
-def test_random_faces_match_oracle(seed, hdr, red):
+@pytest.mark.parametrize("seed,hdr,red,grid", [(1, False, False, False), (2, True, True, False),
+                                               (3, False, True, True)])
+def test_random_faces_match_oracle(seed, hdr, red, grid):
+    """grid: the luxels are the frames' grid points, made on the device."""
     casters, L, fset, ff, fp = random_case(seed)
+    if grid:
+        ff.luxel_pos = None
     print(f"[final] random seed={seed} hdr={hdr} red={red}: {fset.nfaces} faces, {fset.nsamples} samples",
           flush=True)
     tr, fl, fin = open_all(casters)

@@ -256,14 +256,17 @@ def test_layout_reserves_the_average_word_before_each_lit_face():
     assert size == 4 + 24 + 4 + 16
 
 
-@pytest.mark.parametrize("seed,red,bounce", [(1, False, False), (2, True, True), (3, False, True),
-                                             (5, True, False)])
-def test_the_c_statement_equals_the_python_one(seed, red, bounce):
+@pytest.mark.parametrize("seed,red,bounce,grid", [(1, False, False, False), (2, True, True, False),
+                                                  (3, False, True, True), (5, True, False, True)])
+def test_the_c_statement_equals_the_python_one(seed, red, bounce, grid):
     """vradcpu_final.c, used for whole maps, against this module's statement
-    on seeded random faces: every lump byte."""
+    on seeded random faces: every lump byte (grid: the luxels are the frames'
+    grid points)."""
     from final_cases import oracle_inputs, random_case
 
     _, _, fset, ff, fp = random_case(seed)
+    if grid:
+        ff.luxel_pos = None
     rng = np.random.default_rng(seed + 10)
     light = rng.uniform(0, 8, (4, fset.nsamples, 4)).astype(np.float32)
     total = rng.uniform(0, 30, (fp.npatch, 4, 3)).astype(np.float32) if bounce else None
