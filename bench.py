@@ -405,6 +405,80 @@ def warm_start_leg(a, log):
     return out, model
 
 
+def final_leg(a, log):
+    """The final light at the bake's size: the bounce config's map (12 x 12
+    rooms, 202,752 patches, 3,168,000 luxels on 198,000 lit 4 x 4 faces, 288
+    point lights), the HDR gather on the GPU, then FinalLightFace.Run for
+    every face with the patch radial (each patch's TotalLight a seeded
+    stand-in: the stage's work does not depend on the values), timed over
+    ``--final-reps`` calls by HIP events on the library's stream.  The CPU
+    baseline is vradcpu_final.c on the same samples and patch light over the
+    host's cores, and its lump is compared with the GPU's byte for byte."""
+    import oracle.final as of
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, rooms
+    from hl2sdk_tools_sharp_amd.final import HipFinalLighter
+    from hl2sdk_tools_sharp_amd.light import HipFaceLighter
+
+    g = rooms.bounce_grid(12, 12)
+    faces = rooms.room_faces(g, luxel=16.0)
+    lights, pvs = rooms.room_lights(g, sun=False)
+    ff, fp = rooms.room_final_inputs(g, faces)
+    rng = np.random.default_rng(11)
+    pl = np.zeros((g.npatch, 4, 3), np.float32)
+    pl[:, 0] = rng.uniform(0, 50, (g.npatch, 3))
+    cnt = np.diff(faces.face_sample_ptr).astype(np.int64)
+    lit = ff.faces["lit"] == 1
+    n = (ff.faces["width"].astype(np.int64) * ff.faces["height"])
+    seg = np.repeat(np.arange(len(cnt)), np.diff(ff.nbr_ptr))
+    nbr_items = np.bincount(seg, weights=cnt[ff.nbr], minlength=len(cnt)).astype(np.int64)
+    tiles = (n + 63) // 64
+    steps = int(np.sum(np.where(lit, tiles * (cnt + nbr_items), 0)))
+    log(f"[bench] final light: {int(lit.sum())} lit faces, {int(n[lit].sum())} luxels, {g.npatch} patches; "
+        f"{steps} wave-item steps of the direct radial")
+    offer = HipTracerFactory(device=0).try_create(g.casters)
+    with offer.tracer as tr:
+        fl = HipFaceLighter(tr)
+        fl.load(faces, lights, light_pvs=pvs, sun_samples=1)
+        fl.run()
+        fr = HipFinalLighter(fl)
+        t0 = time.perf_counter()
+        fr.load(ff, fp)
+        load_s = time.perf_counter() - t0
+        lump = fr.run(hdr=True, bounce="host", patch_light=pl)  # first-use costs
+        kms = []
+        for _ in range(a.final_reps):
+            lump_t, ms = fr.timed_run(hdr=True, bounce="host", patch_light=pl)
+            kms.append(ms)
+        t0 = time.perf_counter()
+        fr.run(hdr=True, bounce="host", patch_light=pl)
+        wall_s = time.perf_counter() - t0
+        light = fl.read().hdr
+    kernel_ms = float(np.mean(kms))
+    nlux = int(n[lit].sum())
+    out = {"lit_faces": int(lit.sum()), "luxels": nlux, "patches": g.npatch, "wave_item_steps": steps,
+           "kernel_ms": kernel_ms, "kernel_ms_runs": [round(x, 4) for x in kms], "call_wall_ms": wall_s * 1e3,
+           "load_ms": load_s * 1e3, "luxels_per_s": nlux / (kernel_ms * 1e-3),
+           "lump_bytes": int(lump.shape[0]), "repeat_equal": bool(np.array_equal(lump, lump_t))}
+    log(f"[bench] final light: kernels {kernel_ms:.3f} ms ({out['luxels_per_s'] / 1e9:.2f} G luxels/s), "
+        f"call {wall_s * 1e3:.1f} ms, load {load_s * 1e3:.1f} ms")
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
+              "winding": fp.winding}
+        t0 = time.perf_counter()
+        lump_c, _ = of.final_light_c(ff.faces, ff.luxel_pos, ff.sample_bounds, ff.nbr_ptr, ff.nbr,
+                                     faces.face_sample_ptr, faces.face_normal_count, faces.sample_pos, light,
+                                     pd, pl, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": dt * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+            "sample": "the whole map's final light (radial filter, patch radial, minlight, medians, RGBE); "
+            "CPU restatement of FinalLightFace.Run, faces over pthreads"}
+        out["lump_equal_cpu"] = bool(np.array_equal(lump_c, lump))
+        log(f"[bench] final light CPU {dt * 1e3:.0f} ms on {threads} threads; lump equal: {out['lump_equal_cpu']}")
+    return out
+
+
 def bake_leg(a, log, model):
     """The 200k-patch bake end to end, both forms on the bounce config's map
     (12 x 12 rooms, 202,752 patches, 288 point lights behind room PVS rows,
@@ -496,6 +570,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--no-bake", action="store_true")
+    ap.add_argument("--no-final", action="store_true")
+    ap.add_argument("--final-reps", type=int, default=5)
     ap.add_argument("--mlp-maps", type=int, default=64)
     ap.add_argument("--mlp-test-maps", type=int, default=8)
     ap.add_argument("--mlp-epochs", type=int, default=8)
@@ -634,6 +710,7 @@ def main(argv=None) -> int:
     bounce = None if a.no_bounce else bounce_leg(a, log)
     batching = None if a.no_batching else batching_leg(a, log)
     plain = None if a.no_plain else plain_vrad_leg(a, log)
+    final = None if a.no_final else final_leg(a, log)
     warm, model = (None, None) if a.no_warm_start else warm_start_leg(a, log)
     bake = None if (a.no_bake or model is None) else bake_leg(a, log, model)
 
@@ -664,6 +741,7 @@ def main(argv=None) -> int:
         "bounce": bounce,
         "batching": batching,
         "plain_vrad": plain,
+        "final_light": final,
         "warm_start": warm,
         "bake_200k": bake,
     }

@@ -16,11 +16,12 @@
 //     walks the face's items (samples, or patches) 64 at a time: lane l
 //     projects item l into this face's luxel space (the item's point, its
 //     bounds carried corner by corner for a neighbour's sample, a patch's
-//     winding bounds) and its integer window; then the wave takes the 64
-//     items in order, each broadcast from its lane, and every lane whose cell
-//     is in the item's window adds the item's weighted light.  Items whose
-//     rows miss the wave's rows are skipped wave-uniformly.  The cells go to
-//     HBM: weight and light per normal, float4 each.
+//     winding bounds) and its integer window; a ballot keeps the items whose
+//     window meets the wave's cells (most of a neighbour's samples miss a
+//     small face), and the wave takes those in order, each broadcast from its
+//     lane: every lane whose cell is in the item's window adds the item's
+//     weighted light.  The cells go to HBM: weight and light per normal,
+//     float4 each.
 //   k_final: one wave per lit face.  Each lane samples its luxels from both
 //     grids, adds them, clamps, encodes and writes the lump words, and keeps
 //     the flat value of every luxel whose flat sample had weight; the median
@@ -113,8 +114,26 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi)
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
+__device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v)
+{
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        int32_t u = __shfl_up(v, d, kWave);
+        if ((int)threadIdx.x >= d) v += u;
+    }
+    return v;
+}
+
 // One wave's cells of one face: the direct grid (BOUNCED false) or the patch
 // grid.  NB is the face's normal count.
+//
+// The face's items form segments: its own samples (or leaf patches), then
+// each neighbour's.  Lane q describes segment q (source face, first item,
+// count); a wave prefix sum numbers the items, and the wave then takes them
+// 64 at a time across segment boundaries: lane l finds its item's segment,
+// projects the item and loads its light, and the items whose window meets
+// the wave's cells are applied in order from registers, so no memory access
+// waits inside the per-item loop.
 template <bool BOUNCED, int NB>
 __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
 {
@@ -132,39 +151,57 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
 #pragma unroll
     for (int b = 0; b < NB; b++) L[b][0] = L[b][1] = L[b][2] = 0.0f;
 
-    const int32_t n0 = a.nbr_ptr[face], nn = a.nbr_ptr[face + 1] - n0;
-    for (int32_t q = -1; q < nn; q++) {
-        const int32_t src = q < 0 ? face : a.nbr[n0 + q];
-        const FaceDev S = a.faces[src];
-        int32_t begin, end;
-        bool src_bumped;
-        if (!BOUNCED) {
-            // a neighbour without the style or without samples adds nothing
-            if (q >= 0 && !S.lit) continue;
-            begin = S.s0;
-            end = S.s1;
-            src_bumped = S.nb == 4;
-        } else {
-            begin = a.plist_ptr[src];
-            end = a.plist_ptr[src + 1];
-            src_bumped = q < 0 ? NB == 4 : S.nb == 4;
+    const int32_t n0 = a.nbr_ptr[face], nseg = a.nbr_ptr[face + 1] - n0 + 1;
+    for (int32_t q0 = 0; q0 < nseg; q0 += kWave) {
+        const int32_t q = q0 + lane;
+        int32_t src = face, first = 0, cnt = 0, sbump = 0;
+        if (q < nseg) {
+            src = q == 0 ? face : a.nbr[n0 + q - 1];
+            if (!BOUNCED) {
+                // a neighbour without the style or without samples adds nothing
+                const int32_t lit = a.faces[src].lit, nbs = a.faces[src].nb;
+                first = a.faces[src].s0;
+                cnt = (q == 0 || lit) ? a.faces[src].s1 - first : 0;
+                sbump = nbs == 4;
+            } else {
+                first = a.plist_ptr[src];
+                cnt = a.plist_ptr[src + 1] - first;
+                sbump = q == 0 ? NB == 4 : a.faces[src].nb == 4;
+            }
         }
-        for (int32_t base = begin; base < end; base += kWave) {
-            const int m = min(kWave, end - base);
-            // lane l projects item base + l
+        const int32_t incl = wave_inclusive_sum(cnt);
+        const int32_t excl = incl - cnt;
+        const int32_t total = bcast(incl, kWave - 1);
+        const int32_t nq = min(kWave, nseg - q0);
+        for (int32_t base = 0; base < total; base += kWave) {
+            const int32_t g = base + lane;
+            int32_t seg = 0;
+            for (int k = 0; k < nq; k++) seg += bcast(incl, k) <= g;
+            seg = min(seg, kWave - 1);
+            const int32_t isrc = __shfl(src, seg, kWave);
+            const int32_t ifirst = __shfl(first, seg, kWave);
+            const int32_t iexcl = __shfl(excl, seg, kWave);
+            const int32_t ibump = __shfl(sbump, seg, kWave);
+            const bool mine = g < total;
             float cs = 0, ct = 0, pa = 0, pb = 0, pc = 0, pd = 0;
-            int32_t smin = 0, smax = 0, tmin = 0, tmax = 0, item = 0;
-            if (lane < m) {
+            float lt[NB][3];
+#pragma unroll
+            for (int b = 0; b < NB; b++) lt[b][0] = lt[b][1] = lt[b][2] = 0.0f;
+            int32_t smin = 0, smax = 0, tmin = 0, tmax = 0;
+            if (mine) {
+                const int32_t item0 = ifirst + (g - iexcl);
                 if (!BOUNCED) {
-                    item = base + lane;
+                    const int32_t item = item0;
                     const float *p = a.spos + (size_t)item * 3;
                     float2 cc = world_to_luxel(F, p[0], p[1], p[2]);
                     cs = cc.x;
                     ct = cc.y;
                     float4 bd = a.bounds[item];
-                    if (q < 0) {
+                    if (q0 + seg == 0) {
                         pa = bd.x; pb = bd.y; pc = bd.z; pd = bd.w;
                     } else {
+                        // the neighbour's bounds, corner by corner into this grid
+                        const FaceDev S = a.faces[isrc];
                         float3 w0 = luxel_to_world(S, bd.x, bd.y);
                         float2 mn = world_to_luxel(F, w0.x, w0.y, w0.z);
                         float3 w1 = luxel_to_world(S, bd.z, bd.w);
@@ -176,8 +213,15 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
                     tmin = (int32_t)pb;
                     smax = (int32_t)(pc + 0.9999f) + 1;
                     tmax = (int32_t)(pd + 0.9999f) + 1;
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        if (b == 0 || ibump) {
+                            float4 l = a.lv[(size_t)b * a.nsamples + item];
+                            lt[b][0] = l.x; lt[b][1] = l.y; lt[b][2] = l.z;
+                        }
+                    }
                 } else {
-                    item = a.plist[base + lane];
+                    const int32_t item = a.plist[item0];
                     float4 o = a.porigin[item];
                     float2 cc = world_to_luxel(F, o.x, o.y, o.z);
                     cs = cc.x;
@@ -203,73 +247,83 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
                     tmin = (int32_t)((double)ct - (double)dt * kRadialDist);
                     smax = (int32_t)((double)cs + (double)ds * kRadialDist + 1.0);
                     tmax = (int32_t)((double)ct + (double)dt * kRadialDist + 1.0);
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        if (b == 0 || ibump) {
+                            float4 l = a.plight[(size_t)item * 4 + b];
+                            lt[b][0] = l.x; lt[b][1] = l.y; lt[b][2] = l.z;
+                        }
+                    }
                 }
                 smin = clampi(smin, 0, 1 << 30);
                 tmin = clampi(tmin, 0, 1 << 30);
                 smax = min(smax, F.w);
                 tmax = min(tmax, F.h);
             }
-            for (int j = 0; j < m; j++) {
+            // only the items whose window meets this wave's cells, in order
+            const bool live = mine && smin < smax && tmin < tmax && tmax > t_lo && tmin <= t_hi;
+            for (uint64_t todo = __ballot(live); todo; todo &= todo - 1) {
+                const int j = __builtin_ctzll(todo);
                 const int32_t jtmin = bcast(tmin, j), jtmax = bcast(tmax, j);
                 const int32_t jsmin = bcast(smin, j), jsmax = bcast(smax, j);
-                if (jtmax <= t_lo || jtmin > t_hi || jsmin >= jsmax) continue;
                 const float jcs = bcast(cs, j), jct = bcast(ct, j);
                 const float ja = bcast(pa, j), jb = bcast(pb, j);
-                const float jc = bcast(pc, j), jd = bcast(pd, j);
-                const int32_t jitem = bcast(item, j);
                 const bool in = valid && s >= jsmin && s < jsmax && t >= jtmin && t < jtmax;
-                if (!in) continue;
-                float r;
+                float r = 0.0f;
+                bool add = in;
                 if (!BOUNCED) {
                     // AddDirect (LuxelRadial.cs:163-205): the overlap of the
                     // sample's box with the luxel's two-luxel window, over the
                     // Chebyshev distance floored at 0.1
-                    float x = ja - fs;
-                    float s0 = x > -1.0f ? x : -1.0f;
-                    x = jb - ft;
-                    float t0 = x > -1.0f ? x : -1.0f;
-                    x = jc - fs;
-                    float s1 = x < 1.0f ? x : 1.0f;
-                    x = jd - ft;
-                    float t1 = x < 1.0f ? x : 1.0f;
-                    float area = (s1 - s0) * (t1 - t0);
-                    if (!((double)area > kEqualEpsilon)) continue;
-                    float ds = fabsf(jcs - fs), dt = fabsf(jct - ft);
-                    r = ds > dt ? ds : dt;
-                    r = (double)r < 0.1 ? (float)((double)area / 0.1) : area / r;
-                } else {
+                    const float jc = bcast(pc, j), jd = bcast(pd, j);
+                    if (in) {
+                        float x = ja - fs;
+                        float s0 = x > -1.0f ? x : -1.0f;
+                        x = jb - ft;
+                        float t0 = x > -1.0f ? x : -1.0f;
+                        x = jc - fs;
+                        float s1 = x < 1.0f ? x : 1.0f;
+                        x = jd - ft;
+                        float t1 = x < 1.0f ? x : 1.0f;
+                        float area = (s1 - s0) * (t1 - t0);
+                        add = (double)area > kEqualEpsilon;
+                        if (add) {
+                            float ds = fabsf(jcs - fs), dt = fabsf(jct - ft);
+                            r = ds > dt ? ds : dt;
+                            r = (double)r < 0.1 ? (float)((double)area / 0.1) : area / r;
+                        }
+                    }
+                } else if (in) {
                     // AddBounced (LuxelRadial.cs:255-291): 2 - d^2 in patch units
                     float ds = (jcs - fs) / ja, dt = (jct - ft) / jb;
                     r = 2.0f - (ds * ds + dt * dt);
-                    if (!(r > 0.0f)) continue;
+                    add = r > 0.0f;
                 }
-                const float4 *lsrc = BOUNCED ? a.plight + (size_t)jitem * 4
-                                             : a.lv + jitem;
-                const size_t step = BOUNCED ? 1 : (size_t)a.nsamples;
-                if (NB == 4 && src_bumped) {
+                const bool jbump = bcast((int32_t)ibump, j) != 0;
+                float l0[3] = {bcast(lt[0][0], j), bcast(lt[0][1], j), bcast(lt[0][2], j)};
+                if (NB == 4 && jbump) {
+                    float lb[NB][3];
 #pragma unroll
-                    for (int b = 0; b < NB; b++) {
-                        float4 l = lsrc[b * step];
-                        L[b][0] = L[b][0] + r * l.x;
-                        L[b][1] = L[b][1] + r * l.y;
-                        L[b][2] = L[b][2] + r * l.z;
+                    for (int b = 1; b < NB; b++)
+                        for (int k = 0; k < 3; k++) lb[b][k] = bcast(lt[b][k], j);
+                    if (add) {
+#pragma unroll
+                        for (int k = 0; k < 3; k++) L[0][k] = L[0][k] + r * l0[k];
+#pragma unroll
+                        for (int b = 1; b < NB; b++)
+                            for (int k = 0; k < 3; k++) L[b][k] = L[b][k] + r * lb[b][k];
                     }
-                } else {
-                    float4 l = lsrc[0];
-                    L[0][0] = L[0][0] + r * l.x;
-                    L[0][1] = L[0][1] + r * l.y;
-                    L[0][2] = L[0][2] + r * l.z;
+                } else if (add) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++) L[0][k] = L[0][k] + r * l0[k];
                     if (NB == 4) {
                         float bw = r * kOneOverSqrt3;
 #pragma unroll
-                        for (int b = 1; b < NB; b++) {
-                            L[b][0] = L[b][0] + bw * l.x;
-                            L[b][1] = L[b][1] + bw * l.y;
-                            L[b][2] = L[b][2] + bw * l.z;
-                        }
+                        for (int b = 1; b < NB; b++)
+                            for (int k = 0; k < 3; k++) L[b][k] = L[b][k] + bw * l0[k];
                     }
                 }
-                W = W + r;
+                if (add) W = W + r;
             }
         }
     }
