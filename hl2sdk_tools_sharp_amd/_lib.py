@@ -136,6 +136,9 @@ SIGNATURES = {
     "vh_final_load": (ctypes.c_int, [P, P]),
     "vh_final_layout": (ctypes.c_int, [P, P, ctypes.POINTER(I64)]),
     "vh_final_light": (ctypes.c_int, [P, P, P, P, I64]),
+    "vh_ambient_load": (ctypes.c_int, [P, P]),
+    "vh_ambient_cubes": (ctypes.c_int, [P, P, I64, P]),
+    "vh_ambient_directions": (ctypes.c_int, [P]),
     "vh_mlp_init": (ctypes.c_int, [P, P, P, ctypes.POINTER(VhMlpOpts)]),
     "vh_mlp_params": (ctypes.c_int, [P, P]),
     "vh_mlp_forward": (ctypes.c_int, [P, P, I64, P]),
@@ -160,6 +163,7 @@ VH_KERNEL_TRANSFERS = 4
 VH_KERNEL_FACELIGHT = 5
 VH_KERNEL_MLP = 6
 VH_KERNEL_FINAL = 7
+VH_KERNEL_AMBIENT = 8
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1

@@ -30,6 +30,7 @@ SOURCES = [
     "vh_gather.hip",
     "vh_mlp.hip",
     "vh_final.hip",
+    "vh_ambient.hip",
 ]
 
 FLAGS = [

@@ -210,6 +210,7 @@ void destroy(vh_ctx *c)
     gather_release(c);
     mlp_release(c);
     final_release(c);
+    ambient_release(c);
     for (DevBuf *b : {&c->nodes, &c->tris, &c->rays_buf, &c->out_buf, &c->out2_buf, &c->pts_buf,
                       &c->lights_buf, &c->keys_a, &c->keys_b, &c->perm_a, &c->perm_b,
                       &c->sort_tmp, &c->counters})

@@ -500,6 +500,7 @@ struct FinalData {
     std::vector<int32_t> h_face_ptr, h_normals, h_offsets;
     bool bumped_owner = false;  // some bumped face owns a leaf patch
     bool have_luxels = false;   // luxel points given (else made from the frames)
+    bool have_lump = false;     // a vh_final_light has filled the lump
     vh::DevBuf faces, luxel_pos, bounds, nbr_ptr, nbr, plist_ptr, plist, porigin, wptr, wpts;
     vh::DevBuf plight, tiles, lit_faces, cells, pcells, mvals, lump;
 };
@@ -515,6 +516,15 @@ void final_release(vh_ctx *c)
         b->release();
     delete D;
     c->final_ = nullptr;
+}
+
+bool final_lump(vh_ctx *c, FinalLump *v)
+{
+    FinalData *D = c->final_;
+    if (!D || !D->have_lump) return false;
+    v->lump = D->lump.ptr;
+    v->bytes = D->lump_bytes;
+    return true;
 }
 }  // namespace vh
 
@@ -669,6 +679,7 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
     D->h_offsets = offsets;
     D->bumped_owner = bumped_owner;
     D->have_luxels = d->luxel_pos != nullptr;
+    D->have_lump = false;
     std::vector<float4> porg((size_t)std::max(P, 1));
     for (int32_t p = 0; p < P; p++)
         porg[p] = make_float4(d->patch_origin[3 * p], d->patch_origin[3 * p + 1],
@@ -822,6 +833,7 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
         set_error(std::string("final light failed: ") + hipGetErrorString(e));
         return VH_EDEVICE;
     }
+    D->have_lump = true;
     return VH_OK;
 }
 

@@ -96,6 +96,8 @@ struct vh_ctx {
     struct MlpData *mlp = nullptr;
     // the luxel radial filter and the lump (vh_final.hip)
     struct FinalData *final_ = nullptr;
+    // the leaf-ambient fan (vh_ambient.hip)
+    struct AmbientData *ambient = nullptr;
 };
 
 namespace vh {
@@ -162,6 +164,14 @@ void bounce_pull_up(vh_ctx *c, float4 *v, hipStream_t s);
 void mlp_release(vh_ctx *c);
 // Frees the final-light buffers (vh_final.hip).
 void final_release(vh_ctx *c);
+// The last vh_final_light lump, on the device (vh_final.hip).
+struct FinalLump {
+    const void *lump = nullptr;
+    int64_t bytes = 0;
+};
+bool final_lump(vh_ctx *c, FinalLump *v);
+// Frees the leaf-ambient buffers (vh_ambient.hip).
+void ambient_release(vh_ctx *c);
 }  // namespace vh
 
 #define VH_HIP(call)                                                                     \

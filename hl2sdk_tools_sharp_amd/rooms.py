@@ -598,3 +598,37 @@ def room_final_inputs(g: RoomGrid, faces, luxel: float = 16.0):
     fp = FinalPatches(g.origin.astype(np.float32), g.face.astype(np.int32), g.child1.astype(np.int32),
                       (4 * np.arange(P + 1)).astype(np.int32), g.winding.reshape(-1, 3).astype(np.float32))
     return ff, fp
+
+
+def room_ambient_faces(g: RoomGrid, ff, offsets) -> np.ndarray:
+    """The leaf-ambient stage's face records for the final stage's faces
+    ``ff`` (FinalFaces) and lump ``offsets`` (vh_final_layout): lightmap
+    vectors from each face's luxel frame (s = dot(p, w2l_s) - dot(origin,
+    w2l_s)), mins, grid, light offset, the sky flag and reflectivity."""
+    from .ambient import FACE_DTYPE
+
+    fa = ff.faces
+    out = np.zeros(len(fa), FACE_DTYPE)
+    o = fa["origin"].astype(np.float64)
+    for key, src in (("s", "w2l_s"), ("t", "w2l_t")):
+        v = fa[src].astype(np.float64)
+        out[key][:, :3] = fa[src]
+        out[key][:, 3] = -(o * v).sum(1)
+    out["min_s"] = fa["min_s"]
+    out["min_t"] = fa["min_t"]
+    out["smax"] = fa["width"]
+    out["tmax"] = fa["height"]
+    out["light_ofs"] = np.asarray(offsets, np.int32)
+    out["sky"] = g.face_sky.astype(np.int32)
+    out["reflectivity"] = g.face_refl
+    return out
+
+
+def room_ambient_points(g: RoomGrid, per_room: int = 32, seed: int = 5, margin: float = 8.0) -> np.ndarray:
+    """Leaf-ambient sample points: ``per_room`` seeded points inside each
+    room's interior, ``margin`` units off its walls (room-major)."""
+    rng = np.random.default_rng(seed)
+    lo = g.room_lo + margin
+    hi = g.room_hi - margin
+    u = rng.random((len(lo), per_room, 3))
+    return (lo[:, None, :] + u * (hi - lo)[:, None, :]).reshape(-1, 3).astype(np.float32)

@@ -408,6 +408,49 @@ typedef struct vh_final_opts {
 int vh_final_light(vh_ctx *ctx, const vh_final_opts *opts, const float *patch_light, uint8_t *lump,
                    int64_t capacity);
 
+/* ---- The leaf-ambient ray fan: AmbientSampler.ComputeRayCube
+ * (SourceSharp.MapTools/Rad/Ambient/AmbientSampler.cs), one light style ----
+ *
+ * From each sample point, one ray along each of the 162 vertex normals
+ * (Rad/Ambient/VertexNormals.cs) out to AmbientCube.RayLength; the closest
+ * hit over the loaded scene (the scene's caller ids carry the face index in
+ * their low 24 bits); the ray's light by RayAmbientLighting.Accumulate
+ * (Rad/Ambient/RayAmbientLighting.cs:40-110, 160-300): the hit face's average
+ * colour and the luxel under the hit, blended by the cone's radius at the hit
+ * (RemapValClamped 20..40) and tinted by the face's reflectivity, or the
+ * sky-ambient colour for a sky face; then AmbientCube.Project
+ * (Rad/Ambient/AmbientCube.cs) onto +x, -x, +y, -y, +z, -z.  The reference
+ * walks the BSP for the hit (AmbientRayTracer); this walks the BVH.  The
+ * flagged world lights (AddEmitSurfaceLights) are not added. */
+
+/* One face as AmbientScene shades it (its TraceSurface and FaceShade). */
+typedef struct vh_ambient_face {
+    float s[4], t[4];      /* lightmap vectors: luxel s = dot(p, s[0..2]) + s[3], likewise t */
+    float min_s, min_t;    /* the lightmap's mins in luxels */
+    int32_t smax, tmax;    /* the luxel grid (LightmapTextureSizeInLuxels + 1 each) */
+    int32_t light_ofs;     /* the lump offset of luxel 0 of style 0; -1 for no lightmap */
+    int32_t sky;           /* SURF_SKY */
+    float reflectivity[3];
+} vh_ambient_face;
+
+typedef struct vh_ambient_desc {
+    int32_t nfaces;
+    const vh_ambient_face *faces;
+    int32_t has_sky_ambient; /* a sky-ambient light exists (RayAmbientLighting.FindSkyAmbient) */
+    float sky_ambient[3];    /* its intensity */
+    const uint8_t *lump;     /* the lighting lump; NULL = the last vh_final_light's, on the device */
+    int64_t lump_bytes;
+} vh_ambient_desc;
+
+/* Uploads the faces (copied) and the lump, and decodes each face's
+ * AverageTinted (AmbientScene.BuildShades). */
+int vh_ambient_load(vh_ctx *ctx, const vh_ambient_desc *desc);
+/* The ambient cube (6 x 3 floats, box directions in the order above) of each
+ * of n points (3 floats each), against the loaded scene. */
+int vh_ambient_cubes(vh_ctx *ctx, const float *points, int64_t n, float *cubes);
+/* The fan's 162 directions (486 floats), as the kernels use them. */
+int vh_ambient_directions(float *out);
+
 /* ---- The warm-start network (no counterpart in the reference) ----
  *
  * A per-patch MLP, 25 -> 128 -> 128 -> 128 -> 3 with GELU (tanh form) on
@@ -487,7 +530,8 @@ int vh_mlp_predict(vh_ctx *ctx, const float *emit0, float *start);
 #define VH_KERNEL_FACELIGHT 5
 #define VH_KERNEL_MLP 6 /* the warm-start network's GEMMs (inference) or training steps */
 #define VH_KERNEL_FINAL 7 /* vh_final_light's launches, bracketed together */
-#define VH_KERNEL_COUNT 8
+#define VH_KERNEL_AMBIENT 8 /* the leaf-ambient fan */
+#define VH_KERNEL_COUNT 9
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
 

@@ -1,0 +1,149 @@
+"""The leaf-ambient oracle (oracle/ambient.py) pinned by what the arithmetic
+fixes: the 162-direction table's geometry, the cone constant, the lightmap
+decode, RemapValClamped's ends, the box projection against an independent
+float64 statement, and closed forms in a box room -- a uniform lightmap gives
+a uniform cube of its tinted value, a near wall gives the luxel under the
+hit, a far one the face average, the midpoint of the 20..40 cone radius a
+half blend, a sky face the sky-ambient colour, a miss nothing."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle.ambient as oa
+from hl2sdk_tools_sharp_amd.scenes import box_triangles
+from hl2sdk_tools_sharp_amd.tracer import TRACE_ID_OPAQUE, TRACE_ID_SKY, ShadowCasterSet
+
+f32 = np.float32
+
+
+def test_the_fan_is_162_unit_directions_closed_under_negation():
+    d = oa.VERTEX_NORMALS.astype(np.float64)
+    assert d.shape == (162, 3)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    keys = {tuple(np.round(x, 6)) for x in d}
+    assert len(keys) == 162
+    assert all(tuple(np.round(-x, 6)) in keys for x in d)
+    for axis in np.eye(3):
+        assert tuple(axis) in keys and tuple(-axis) in keys
+    # anorms' first entry, and the icosahedron's vertices among them
+    assert tuple(d[0]) == pytest.approx((-0.525731, 0.0, 0.850651))
+    golden = (1 + math.sqrt(5)) / 2
+    ico = np.array([0.0, 1.0, golden]) / math.sqrt(1 + golden * golden)
+    assert tuple(np.round(ico, 6)) in keys
+
+
+def test_the_cone_constant_is_tan_of_7_275_degrees_rounded_once():
+    a = f32(7.275) * (f32(math.pi) / f32(180.0))
+    t = math.tan(float(a))
+    assert oa.TAN_THETA == f32(t)
+    lo, hi = np.nextafter(oa.TAN_THETA, f32(0)), np.nextafter(oa.TAN_THETA, f32(1))
+    # the double lies well inside the float's rounding interval
+    assert (float(lo) + float(oa.TAN_THETA)) / 2 + 1e-12 < t < (float(hi) + float(oa.TAN_THETA)) / 2 - 1e-12
+    assert abs(float(oa.TAN_THETA) - math.tan(math.radians(7.275))) < 1e-6
+
+
+def test_the_decode_is_mantissa_times_two_to_the_exponent_over_255():
+    w = np.array([[128, 64, 0, 0], [255, 1, 2, 3], [10, 20, 30, 0xFE]], np.uint8)
+    got = oa.tex_light_to_linear(w)
+    want = np.array([[128, 64, 0], [255 * 8, 8, 16], [10 / 4, 20 / 4, 30 / 4]], np.float64) / 255.0
+    np.testing.assert_allclose(got, want, rtol=1e-7)
+    assert got[0, 0] == f32(128) * f32(1 / 255.0)
+
+
+def test_remap_clamped_ends_and_midpoint():
+    got = oa.remap_clamped(np.array([0, 20, 30, 40, 100], np.float32), 20, 40, 0, 1)
+    assert list(got) == [0.0, 0.0, 0.5, 1.0, 1.0]
+
+
+def test_project_matches_a_float64_statement():
+    rng = np.random.default_rng(4)
+    rad = rng.uniform(0, 10, (3, 162, 3)).astype(np.float32)
+    got = oa.project(rad)
+    d = oa.VERTEX_NORMALS.astype(np.float64)
+    for j, b in enumerate(oa.BOX.astype(np.float64)):
+        c = d @ b
+        m = c > 0
+        want = (rad.astype(np.float64)[:, m] * c[m][None, :, None]).sum(1) / c[m].sum()
+        np.testing.assert_allclose(got[:, j], want, rtol=2e-6)
+    const = oa.project(np.full((1, 162, 3), 7.0, np.float32))
+    np.testing.assert_allclose(const, 7.0, rtol=1e-6)
+
+
+def _box_room(L=(100, 50, 25, 0), avg=None, sky_ceiling=False, size=512.0):
+    """A closed box room 0..size on each axis: six faces (+-x, +-y, +-z in
+    box_triangles' order), each lightmapped by a 4 x 4 grid of luxels of
+    size / 4 units, every luxel the word ``L`` and the average word ``avg``."""
+    v9, ids = box_triangles(np.zeros((1, 3)), np.full((1, 3), size))
+    ids = ids & 0x00FFFFFF
+    faces = np.zeros(6, oa.FACE_DTYPE)
+    lump = []
+    sky_face = 5  # +z
+    for f, (ax, side) in enumerate([(0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (2, 1)]):
+        u, v = (ax + 1) % 3, (ax + 2) % 3
+        sc = 4.0 / size
+        faces[f]["s"][u] = sc
+        faces[f]["t"][v] = sc
+        faces[f]["smax"] = faces[f]["tmax"] = 4
+        faces[f]["reflectivity"] = (0.5, 0.25, 1.0)
+        if sky_ceiling and f == sky_face:
+            faces[f]["sky"] = 1
+            faces[f]["light_ofs"] = -1
+            continue
+        words = [avg or L] + [L] * 16
+        faces[f]["light_ofs"] = 4 * len(lump) + 4
+        lump += words
+    fl = np.where(ids == sky_face, TRACE_ID_SKY, TRACE_ID_OPAQUE) if sky_ceiling else TRACE_ID_OPAQUE
+    casters = ShadowCasterSet(v9, ids | fl)
+    return casters, faces, np.array(lump, np.uint8).reshape(-1)
+
+
+def test_a_uniform_room_gives_a_uniform_cube_of_the_tinted_luxel():
+    casters, faces, lump = _box_room()
+    pts = np.array([[256, 256, 256], [100, 400, 50], [30, 30, 480]], np.float32)
+    cube = oa.ambient_cubes(casters, faces, None, lump, pts)
+    want = oa.tex_light_to_linear(np.array([[100, 50, 25, 0]], np.uint8))[0] * np.array([0.5, 0.25, 1.0], np.float32)
+    np.testing.assert_allclose(cube, np.broadcast_to(want, cube.shape), rtol=1e-6)
+
+
+def _ray_color(casters, faces, lump, point, direction_index, sky=None):
+    from oracle import CpuBvh, convert_triangles
+
+    bvh = CpuBvh(casters.vertices, convert_triangles(casters.vertices, casters.ids))
+    pts = np.array([point], np.float32)
+    idx, t = bvh.closest(oa.rays(pts), tmin=0.0, threads=1)
+    ids = np.asarray(casters.ids, np.int64)
+    surf = np.where(idx >= 0, ids[np.maximum(idx, 0)], -1).reshape(1, 162)
+    frac = np.where(idx >= 0, t, np.float32(1)).astype(np.float32).reshape(1, 162)
+    return oa.ray_colors(faces, sky, lump, pts, surf, frac)[0, direction_index]
+
+
+def test_near_far_and_midway_walls():
+    # luxels of the -x wall (4 x 4, 128 units each) hold (200, 0, 0); its
+    # average word (0, 200, 0)
+    casters, faces, lump = _box_room(L=(200, 0, 0, 0), avg=(0, 200, 0, 0))
+    refl = np.array([0.5, 0.25, 1.0], np.float32)
+    lux = oa.tex_light_to_linear(np.array([[200, 0, 0, 0]], np.uint8))[0] * refl
+    avg = oa.tex_light_to_linear(np.array([[0, 200, 0, 0]], np.uint8))[0] * refl
+    mx = int(np.nonzero((oa.VERTEX_NORMALS == np.array([-1, 0, 0], np.float32)).all(1))[0][0])
+    tan = float(oa.TAN_THETA)
+    near = _ray_color(casters, faces, lump, (100, 256, 256), mx)  # cone radius 12.8: the luxel
+    assert np.array_equal(near, lux)
+    far = _ray_color(casters, faces, lump, (400, 256, 256), mx)   # 51: the average
+    assert np.array_equal(far, avg)
+    x = 30.0 / tan                                                 # radius 30: half of each
+    mid = _ray_color(casters, faces, lump, (x, 256, 256), mx)
+    np.testing.assert_allclose(mid, 0.5 * lux + 0.5 * avg, rtol=1e-5)
+
+
+def test_a_sky_face_gives_the_sky_ambient_and_a_miss_nothing():
+    casters, faces, lump = _box_room(sky_ceiling=True)
+    up = int(np.nonzero((oa.VERTEX_NORMALS == np.array([0, 0, 1], np.float32)).all(1))[0][0])
+    amb = (40.0, 60.0, 90.0)
+    got = _ray_color(casters, faces, lump, (256, 256, 256), up, sky=amb)
+    assert list(got) == [40.0, 60.0, 90.0]
+    assert list(_ray_color(casters, faces, lump, (256, 256, 256), up)) == [0.0, 0.0, 0.0]
+    # above the room: the upward rays miss, so +z gets nothing
+    cube = oa.ambient_cubes(casters, faces, amb, lump, np.array([[256, 256, 5000]], np.float32))
+    down = np.array([0.0, 0.0, 0.0])
+    assert np.all(cube[0, :4] >= 0) and np.all(cube[0, 4] == down)
