@@ -413,9 +413,14 @@ def final_leg(a, log):
     stand-in: the stage's work does not depend on the values), timed over
     ``--final-reps`` calls by HIP events on the library's stream.  The CPU
     baseline is vradcpu_final.c on the same samples and patch light over the
-    host's cores, and its lump is compared with the GPU's byte for byte."""
+    host's cores, and its lump is compared with the GPU's byte for byte.
+    Then the leaf-ambient fan: ``--ambient-per-room`` seeded points in each
+    room, 162 closest-hit rays each, shaded from that lump on the device; its
+    CPU baseline is the oracle on the first ``--ambient-cpu-points`` points,
+    whose cubes are compared bit for bit."""
     import oracle.final as of
     from hl2sdk_tools_sharp_amd import HipTracerFactory, rooms
+    from hl2sdk_tools_sharp_amd.ambient import HipLeafAmbient
     from hl2sdk_tools_sharp_amd.final import HipFinalLighter
     from hl2sdk_tools_sharp_amd.light import HipFaceLighter
 
@@ -453,6 +458,17 @@ def final_leg(a, log):
         fr.run(hdr=True, bounce="host", patch_light=pl)
         wall_s = time.perf_counter() - t0
         light = fl.read().hdr
+        # the leaf-ambient fan over the lump the final stage left on the device
+        offs, _ = fr.layout()
+        afaces = rooms.room_ambient_faces(g, ff, offs)
+        pts = rooms.room_ambient_points(g, per_room=a.ambient_per_room, seed=5)
+        amb = HipLeafAmbient(tr)
+        amb.load(afaces)
+        amb.cubes(pts[:64])  # first-use costs
+        ams = []
+        for _ in range(a.final_reps):
+            cubes, ms = amb.timed_cubes(pts)
+            ams.append(ms)
     kernel_ms = float(np.mean(kms))
     nlux = int(n[lit].sum())
     out = {"lit_faces": int(lit.sum()), "luxels": nlux, "patches": g.npatch, "wave_item_steps": steps,
@@ -461,6 +477,13 @@ def final_leg(a, log):
            "lump_bytes": int(lump.shape[0]), "repeat_equal": bool(np.array_equal(lump, lump_t))}
     log(f"[bench] final light: kernels {kernel_ms:.3f} ms ({out['luxels_per_s'] / 1e9:.2f} G luxels/s), "
         f"call {wall_s * 1e3:.1f} ms, load {load_s * 1e3:.1f} ms")
+    amb_ms = float(np.mean(ams))
+    nrays = len(pts) * 162
+    out["ambient"] = {"points": len(pts), "rays": nrays, "kernel_ms": amb_ms,
+                      "kernel_ms_runs": [round(x, 4) for x in ams], "rays_per_s": nrays / (amb_ms * 1e-3),
+                      "points_per_s": len(pts) / (amb_ms * 1e-3)}
+    log(f"[bench] leaf ambient: {len(pts)} points x 162 rays in {amb_ms:.3f} ms "
+        f"({out['ambient']['rays_per_s'] / 1e9:.2f} G rays/s)")
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
@@ -476,6 +499,19 @@ def final_leg(a, log):
             "CPU restatement of FinalLightFace.Run, faces over pthreads"}
         out["lump_equal_cpu"] = bool(np.array_equal(lump_c, lump))
         log(f"[bench] final light CPU {dt * 1e3:.0f} ms on {threads} threads; lump equal: {out['lump_equal_cpu']}")
+        import oracle.ambient as oamb
+
+        k = min(len(pts), a.ambient_cpu_points)
+        t0 = time.perf_counter()
+        want = oamb.ambient_cubes(g.casters, afaces, None, lump, pts[:k], threads=threads)
+        dt = time.perf_counter() - t0
+        out["ambient"]["cpu_baseline"] = {
+            "value": k * 162 / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"the first {k} of {len(pts)} points (x 162 rays); the oracle's BVH closest hit "
+            "(pthreads) and the float32 restatement of Accumulate and Project"}
+        out["ambient"]["cubes_differing_cpu"] = int(np.sum(cubes[:k].view(np.uint32) != want.view(np.uint32)))
+        log(f"[bench] leaf ambient CPU {k * 162 / dt / 1e6:.2f} Mrays/s on {threads} threads; "
+            f"cube components differing: {out['ambient']['cubes_differing_cpu']}")
     return out
 
 
@@ -572,6 +608,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-bake", action="store_true")
     ap.add_argument("--no-final", action="store_true")
     ap.add_argument("--final-reps", type=int, default=5)
+    ap.add_argument("--ambient-per-room", type=int, default=128)
+    ap.add_argument("--ambient-cpu-points", type=int, default=2048)
     ap.add_argument("--mlp-maps", type=int, default=64)
     ap.add_argument("--mlp-test-maps", type=int, default=8)
     ap.add_argument("--mlp-epochs", type=int, default=8)
