@@ -6,8 +6,10 @@ normals, each ray's light from the lightmap lump where it lands
 (RayAmbientLighting.Accumulate, Rad/Ambient/RayAmbientLighting.cs), projected
 onto the six box directions (AmbientCube.Project, Rad/Ambient/AmbientCube.cs).
 The lump is the final stage's, read where it lies on the device, or one the
-caller passes.  The flagged world lights (AddEmitSurfaceLights) are not
-added; leaf sampling and the lump of cubes stay with the caller.
+caller passes.  The world lights LeafAmbientSurfaceLights.Classify flags
+(dim surface lights of style 0) are added where the segment to them is open
+(AmbientCube.AddEmitSurfaceLights); leaf sampling and the lump of cubes stay
+with the caller.
 """
 from __future__ import annotations
 
@@ -26,9 +28,17 @@ FACE_DTYPE = np.dtype([
 ])
 
 
+# vh_ambient_light
+LIGHT_DTYPE = np.dtype([
+    ("origin", "<f4", 3), ("normal", "<f4", 3), ("intensity", "<f4", 3), ("radius", "<f4"),
+    ("type", "<i4"), ("style", "<i4"),
+])
+
+
 class VhAmbientDesc(ctypes.Structure):
     _fields_ = [("nfaces", ctypes.c_int32), ("faces", ctypes.c_void_p), ("has_sky_ambient", ctypes.c_int32),
-                ("sky_ambient", ctypes.c_float * 3), ("lump", ctypes.c_void_p), ("lump_bytes", ctypes.c_int64)]
+                ("sky_ambient", ctypes.c_float * 3), ("lump", ctypes.c_void_p), ("lump_bytes", ctypes.c_int64),
+                ("nlights", ctypes.c_int32), ("lights", ctypes.c_void_p)]
 
 
 def directions() -> np.ndarray:
@@ -45,10 +55,13 @@ class HipLeafAmbient:
         self.tracer = tracer
         self._keep = None
 
-    def load(self, faces: np.ndarray, sky_ambient=None, lump: Optional[np.ndarray] = None) -> None:
+    def load(self, faces: np.ndarray, sky_ambient=None, lump: Optional[np.ndarray] = None,
+             lights: Optional[np.ndarray] = None) -> None:
         """faces: FACE_DTYPE per face (the scene's ids carry the face index);
-        lump None reads the last final-light lump on the device."""
+        lump None reads the last final-light lump on the device; lights: the
+        pass's world lights (LIGHT_DTYPE), classified here."""
         fa = np.ascontiguousarray(faces, FACE_DTYPE)
+        li = None if lights is None else np.ascontiguousarray(lights, LIGHT_DTYPE)
         lp = None if lump is None else np.ascontiguousarray(lump, np.uint8)
         d = VhAmbientDesc()
         d.nfaces = len(fa)
@@ -58,7 +71,9 @@ class HipLeafAmbient:
             d.sky_ambient[:] = [float(x) for x in sky_ambient]
         d.lump = None if lp is None else lp.ctypes.data
         d.lump_bytes = 0 if lp is None else lp.shape[0]
-        self._keep = (fa, lp, d)
+        d.nlights = 0 if li is None else len(li)
+        d.lights = None if li is None or len(li) == 0 else li.ctypes.data
+        self._keep = (fa, lp, li, d)
         HipRayTracer._raise(_lib.lib().vh_ambient_load(self.tracer.handle, ctypes.byref(d)))
 
     def cubes(self, points: np.ndarray) -> np.ndarray:

@@ -11,7 +11,9 @@
 // (RemapValClamped 20..40), each tinted by the face's reflectivity, or the
 // sky-ambient colour for a sky face.  The 162 colours meet in LDS, and six
 // threads project them onto the box directions in ray order
-// (AmbientCube.Project, Rad/Ambient/AmbientCube.cs).
+// (AmbientCube.Project, Rad/Ambient/AmbientCube.cs).  Then the flagged world
+// lights: a thread per light traces the segment to it, and the six threads
+// add the unblocked ones in light order (AmbientCube.AddEmitSurfaceLights).
 //
 // The reference finds the hit by walking the BSP (AmbientRayTracer); here
 // the hit is the closest-hit seam's, over the scene's triangles, whose caller
@@ -167,8 +169,15 @@ __device__ __forceinline__ float3 decode(uint32_t w)
     return make_float3((float)(w & 255u) * p, (float)((w >> 8) & 255u) * p, (float)((w >> 16) & 255u) * p);
 }
 
+// A world light flagged into the cubes.
+struct CubeLight {
+    float o[3], n[3], i[3], radius;
+};
+
 struct AmbientArgs {
     const float4 *nodes, *tris;
+    const CubeLight *lights;
+    int32_t nlights;
     const AmbientFaceDev *faces;
     int32_t nfaces;
     const float3 *avg;        // AverageTinted per face
@@ -185,6 +194,7 @@ __global__ __launch_bounds__(kFanBlock) void k_ambient(AmbientArgs a)
 {
     __shared__ int stack_mem[kMaxDepth * kFanBlock];
     __shared__ float rad[kFan * 3];
+    __shared__ unsigned char vis[kFanBlock];
     const int i = threadIdx.x;
     const int64_t pidx = blockIdx.x;
     const float px = a.points[pidx * 3], py = a.points[pidx * 3 + 1], pz = a.points[pidx * 3 + 2];
@@ -262,12 +272,13 @@ __global__ __launch_bounds__(kFanBlock) void k_ambient(AmbientArgs a)
         rad[i * 3 + 2] = cb;
     }
     __syncthreads();
-    if (i < 6) {
-        // AmbientCube.Project for box direction j, rays in order
-        const int j = i;
-        const float bx = j == 0 ? 1.0f : (j == 1 ? -1.0f : 0.0f);
-        const float by = j == 2 ? 1.0f : (j == 3 ? -1.0f : 0.0f);
-        const float bz = j == 4 ? 1.0f : (j == 5 ? -1.0f : 0.0f);
+    // AmbientCube.Project for box direction j = i, rays in order
+    const int j = i;
+    const float bx = j == 0 ? 1.0f : (j == 1 ? -1.0f : 0.0f);
+    const float by = j == 2 ? 1.0f : (j == 3 ? -1.0f : 0.0f);
+    const float bz = j == 4 ? 1.0f : (j == 5 ? -1.0f : 0.0f);
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    if (j < 6) {
         float t = 0.0f, sr = 0.0f, sg = 0.0f, sb = 0.0f;
         for (int k = 0; k < kFan; k++) {
             float c = kDirs[k * 3] * bx + kDirs[k * 3 + 1] * by + kDirs[k * 3 + 2] * bz;
@@ -279,10 +290,81 @@ __global__ __launch_bounds__(kFanBlock) void k_ambient(AmbientArgs a)
             }
         }
         float inv = 1.0f / t;
+        cr = sr * inv;
+        cg = sg * inv;
+        cb = sb * inv;
+    }
+    // AddSurfaceLights: each flagged light whose segment is open
+    for (int32_t base = 0; base < a.nlights; base += kFanBlock) {
+        __syncthreads();
+        const int32_t l = base + i;
+        if (l < a.nlights) {
+            // Ray.Segment (Tracing/IRayTracer.cs:71-88), traced as TestLine
+            const CubeLight &L = a.lights[l];
+            float dx = L.o[0] - px, dy = L.o[1] - py, dz = L.o[2] - pz;
+            float len = sqrtf(dx * dx + dy * dy + dz * dz);
+            float inv = 1.0f / len;
+            Ray r;
+            r.ox = px;
+            r.oy = py;
+            r.oz = pz;
+            r.dx = dx * inv;
+            r.dy = dy * inv;
+            r.dz = dz * inv;
+            r.reach = len;
+            bool open = true;
+            if (vh::ray_finite(r)) {
+                float t;
+                int32_t id;
+                open = vh::trace_one<vh::kAny, false, kFanBlock>(a.nodes, a.tris, stack_mem + i, r, 0.0f, -1,
+                                                                  t, id) < 0;
+            }
+            vis[i] = open;
+        }
+        __syncthreads();
+        if (j < 6) {
+            const int32_t m = min(kFanBlock, a.nlights - base);
+            for (int32_t k = 0; k < m; k++) {
+                if (!vis[k]) continue;
+                const CubeLight &L = a.lights[base + k];
+                // AmbientCube.UnoccludedScale: DistanceFalloff times WorldLightAngle
+                float dx = L.o[0] - px, dy = L.o[1] - py, dz = L.o[2] - pz;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float dscale;
+                if (L.radius != 0.0f && r2 > L.radius * L.radius)
+                    dscale = 0.0f;
+                else
+                    dscale = 1.0f / (r2 < 1.0f ? 1.0f : r2);
+                float len = sqrtf(r2);
+                float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+                if (len != 0.0f) {
+                    nx = dx / len;
+                    ny = dy / len;
+                    nz = dz / len;
+                }
+                float dot = nx * nx + ny * ny + nz * nz;
+                float angle = 0.0f;
+                if (!(dot < 0.0f)) {
+                    float dot2 = -(nx * L.n[0] + ny * L.n[1] + nz * L.n[2]);
+                    if (!((double)dot2 <= 0.01)) angle = dot * dot2;
+                }
+                float ratio = dscale * angle * 1.0f;
+                if (ratio == 0.0f) continue;
+                float tj = bx * nx + by * ny + bz * nz;
+                if (tj > 0.0f) {
+                    float w = tj * ratio;
+                    cr = cr + L.i[0] * w;
+                    cg = cg + L.i[1] * w;
+                    cb = cb + L.i[2] * w;
+                }
+            }
+        }
+    }
+    if (j < 6) {
         float *o = a.cubes + (pidx * 6 + j) * 3;
-        o[0] = sr * inv;
-        o[1] = sg * inv;
-        o[2] = sb * inv;
+        o[0] = cr;
+        o[1] = cg;
+        o[2] = cb;
     }
 }
 
@@ -311,7 +393,8 @@ struct AmbientData {
     bool own_lump = false;
     int32_t has_sky = 0;
     float sky[3] = {0, 0, 0};
-    vh::DevBuf faces, avg, lump, points, cubes;
+    int32_t nflag = 0;
+    vh::DevBuf faces, avg, lump, points, cubes, lights;
 };
 
 namespace vh {
@@ -319,7 +402,7 @@ void ambient_release(vh_ctx *c)
 {
     AmbientData *D = c->ambient;
     if (!D) return;
-    for (DevBuf *b : {&D->faces, &D->avg, &D->lump, &D->points, &D->cubes}) b->release();
+    for (DevBuf *b : {&D->faces, &D->avg, &D->lump, &D->points, &D->cubes, &D->lights}) b->release();
     delete D;
     c->ambient = nullptr;
 }
@@ -344,7 +427,7 @@ int vh_ambient_load(vh_ctx *c, const vh_ambient_desc *d)
 {
     clear_error();
     if (!c || !d || d->nfaces < 0 || (d->nfaces > 0 && !d->faces) || d->lump_bytes < 0 ||
-        (d->lump && d->lump_bytes % 4 != 0)) {
+        (d->lump && d->lump_bytes % 4 != 0) || d->nlights < 0 || (d->nlights > 0 && !d->lights)) {
         set_error("bad leaf-ambient descriptor");
         return VH_EINVAL;
     }
@@ -380,6 +463,22 @@ int vh_ambient_load(vh_ctx *c, const vh_ambient_desc *d)
         o.sky = x.sky != 0;
         std::memcpy(o.refl, x.reflectivity, sizeof o.refl);
     }
+    // LeafAmbientSurfaceLights.Classify, Correct side: surface lights of
+    // style 0 whose brightest channel times InvRSquared((0, 0, 512)) is under 0.005
+    std::vector<CubeLight> flagged;
+    const float ratio512 = 1.0f / (512.0f * 512.0f);
+    for (int32_t l = 0; l < d->nlights; l++) {
+        const vh_ambient_light &x = d->lights[l];
+        if (x.type != VH_EMIT_SURFACE || x.style != 0) continue;
+        float m = std::fmax(std::fmax(x.intensity[0], x.intensity[1]), x.intensity[2]);
+        if (!(m * ratio512 < 0.005f)) continue;
+        CubeLight o;
+        std::memcpy(o.o, x.origin, sizeof o.o);
+        std::memcpy(o.n, x.normal, sizeof o.n);
+        std::memcpy(o.i, x.intensity, sizeof o.i);
+        o.radius = x.radius;
+        flagged.push_back(o);
+    }
     VH_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     if (!c->ambient) c->ambient = new AmbientData();
@@ -388,13 +487,18 @@ int vh_ambient_load(vh_ctx *c, const vh_ambient_desc *d)
     D->lump_bytes = lump_bytes;
     D->has_sky = d->has_sky_ambient != 0;
     std::memcpy(D->sky, d->sky_ambient, sizeof D->sky);
+    D->nflag = (int32_t)flagged.size();
     int rc;
-    if ((rc = D->faces.ensure(fd.size() * sizeof(AmbientFaceDev), "ambient faces")) ||
+    if ((rc = D->lights.ensure(std::max<size_t>(flagged.size(), 1) * sizeof(CubeLight), "cube lights")) ||
+        (rc = D->faces.ensure(fd.size() * sizeof(AmbientFaceDev), "ambient faces")) ||
         (rc = D->avg.ensure(fd.size() * sizeof(float3), "face averages")) ||
         (rc = D->lump.ensure((size_t)std::max<int64_t>(lump_bytes, 4), "ambient lump")))
         return rc;
     VH_HIP(hipMemcpyAsync(D->faces.ptr, fd.data(), fd.size() * sizeof(AmbientFaceDev),
                           hipMemcpyHostToDevice, st));
+    if (!flagged.empty())
+        VH_HIP(hipMemcpyAsync(D->lights.ptr, flagged.data(), flagged.size() * sizeof(CubeLight),
+                              hipMemcpyHostToDevice, st));
     if (lump_bytes > 0)
         VH_HIP(hipMemcpyAsync(D->lump.ptr, d->lump ? (const void *)d->lump : dev_lump, (size_t)lump_bytes,
                               d->lump ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
@@ -435,6 +539,8 @@ int vh_ambient_cubes(vh_ctx *c, const float *points, int64_t n, float *cubes)
     AmbientArgs a;
     a.nodes = (const float4 *)c->nodes.ptr;
     a.tris = (const float4 *)c->tris.ptr;
+    a.lights = (const CubeLight *)D->lights.ptr;
+    a.nlights = D->nflag;
     a.faces = (const AmbientFaceDev *)D->faces.ptr;
     a.nfaces = D->nfaces;
     a.avg = (const float3 *)D->avg.ptr;

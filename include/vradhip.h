@@ -419,9 +419,9 @@ int vh_final_light(vh_ctx *ctx, const vh_final_opts *opts, const float *patch_li
  * colour and the luxel under the hit, blended by the cone's radius at the hit
  * (RemapValClamped 20..40) and tinted by the face's reflectivity, or the
  * sky-ambient colour for a sky face; then AmbientCube.Project
- * (Rad/Ambient/AmbientCube.cs) onto +x, -x, +y, -y, +z, -z.  The reference
- * walks the BSP for the hit (AmbientRayTracer); this walks the BVH.  The
- * flagged world lights (AddEmitSurfaceLights) are not added. */
+ * (Rad/Ambient/AmbientCube.cs) onto +x, -x, +y, -y, +z, -z, and the flagged
+ * world lights on top (AmbientSampler.ComputeCube).  The reference walks the
+ * BSP for the hit (AmbientRayTracer); this walks the BVH. */
 
 /* One face as AmbientScene shades it (its TraceSurface and FaceShade). */
 typedef struct vh_ambient_face {
@@ -433,6 +433,13 @@ typedef struct vh_ambient_face {
     float reflectivity[3];
 } vh_ambient_face;
 
+/* One world light (DWorldLight) as the cubes read it. */
+typedef struct vh_ambient_light {
+    float origin[3], normal[3], intensity[3];
+    float radius;          /* 0 = unlimited */
+    int32_t type, style;   /* VH_EMIT_*, the light style */
+} vh_ambient_light;
+
 typedef struct vh_ambient_desc {
     int32_t nfaces;
     const vh_ambient_face *faces;
@@ -440,10 +447,17 @@ typedef struct vh_ambient_desc {
     float sky_ambient[3];    /* its intensity */
     const uint8_t *lump;     /* the lighting lump; NULL = the last vh_final_light's, on the device */
     int64_t lump_bytes;
+    int32_t nlights;         /* the pass's world lights (may be 0) */
+    const vh_ambient_light *lights;
 } vh_ambient_desc;
 
 /* Uploads the faces (copied) and the lump, and decodes each face's
- * AverageTinted (AmbientScene.BuildShades). */
+ * AverageTinted (AmbientScene.BuildShades).  The world lights are classified
+ * as LeafAmbientSurfaceLights.Classify does (Rad/Ambient/LeafAmbientSurfaceLights.cs,
+ * Correct side): surface lights of style 0 whose brightest channel times
+ * 1/512^2 is under 0.005 go into the cubes, each added after the fan
+ * (AmbientSampler.AddSurfaceLights, AmbientCube.AddEmitSurfaceLights) when the
+ * segment from the point to the light is not blocked. */
 int vh_ambient_load(vh_ctx *ctx, const vh_ambient_desc *desc);
 /* The ambient cube (6 x 3 floats, box directions in the order above) of each
  * of n points (3 floats each), against the loaded scene. */

@@ -12,8 +12,9 @@ AmbientCube.Project (Rad/Ambient/AmbientCube.cs) onto the six box directions.
 The hit surface and fraction come from the closest-hit seam (the oracle's
 BVH closest hit, pinned to the reference's KD closest hit) rather than from
 AmbientRayTracer's BSP walk, whose splits the synthetic maps do not have.
-The flagged world lights (AddEmitSurfaceLights) are a separate pass and are
-not restated.  Arithmetic is numpy float32, one IEEE operation at a time in
+Then the flagged world lights (AmbientSampler.AddSurfaceLights with
+AmbientCube.AddEmitSurfaceLights, classified by LeafAmbientSurfaceLights) are
+added where the segment to the light is open.  Arithmetic is numpy float32, one IEEE operation at a time in
 the reference's order; sums over the 162 rays run in ray order.
 """
 from __future__ import annotations
@@ -87,6 +88,12 @@ BOX = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, 
 RAY_LENGTH = f32(32768.0) * f32(1.74)                     # AmbientCube.RayLength
 CONE_ANGLE = f32(7.275) * (f32(math.pi) / f32(180.0))     # VertexNormals.ConeInnerAngleRadians
 TAN_THETA = f32(math.tan(float(CONE_ANGLE)))              # DetMathF.Tan, correctly rounded
+
+LIGHT_DTYPE = np.dtype([
+    ("origin", "<f4", 3), ("normal", "<f4", 3), ("intensity", "<f4", 3), ("radius", "<f4"),
+    ("type", "<i4"), ("style", "<i4"),
+])
+EMIT_SURFACE = 0
 
 FACE_DTYPE = np.dtype([
     ("s", "<f4", 4), ("t", "<f4", 4), ("min_s", "<f4"), ("min_t", "<f4"), ("smax", "<i4"),
@@ -211,11 +218,66 @@ def rays(points: np.ndarray) -> np.ndarray:
     return out.reshape(-1, 7)
 
 
+def flagged_lights(lights) -> np.ndarray:
+    """LeafAmbientSurfaceLights.Classify, Correct side (Rad/Ambient/LeafAmbientSurfaceLights.cs):
+    surface lights of style 0 whose brightest channel times InvRSquared((0,
+    0, 512)) is under 0.005, in light order."""
+    if lights is None or len(lights) == 0:
+        return np.zeros(0, LIGHT_DTYPE)
+    L = np.asarray(lights, LIGHT_DTYPE)
+    ratio = f32(1) / f32(512 * 512)
+    m = np.maximum(np.maximum(L["intensity"][:, 0], L["intensity"][:, 1]), L["intensity"][:, 2])
+    keep = (L["type"] == EMIT_SURFACE) & (L["style"] == 0) & (m * ratio < f32(0.005))
+    return L[keep]
+
+
+def segment_rays(points: np.ndarray, ends: np.ndarray) -> np.ndarray:
+    """Ray.Segment (Tracing/IRayTracer.cs:71-88, Correct side) from every
+    point to every end, point-major: origin, d / |d|, reach |d|."""
+    p = np.asarray(points, np.float32)[:, None, :]
+    e = np.asarray(ends, np.float32)[None, :, :]
+    d = (e - p).astype(np.float32)
+    length = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = f32(1) / length
+    out = np.zeros(d.shape[:2] + (7,), np.float32)
+    out[..., :3] = np.broadcast_to(p, d.shape)
+    out[..., 3:6] = d * inv[..., None]
+    out[..., 6] = length
+    return out.reshape(-1, 7)
+
+
+def add_surface_lights(cubes: np.ndarray, points: np.ndarray, flagged: np.ndarray, open_: np.ndarray) -> None:
+    """AmbientCube.AddEmitSurfaceLights (Rad/Ambient/AmbientCube.cs) in
+    place: open_ (n, lights) says which segments are unblocked."""
+    for k, L in enumerate(flagged):
+        delta = (L["origin"][None, :] - points).astype(np.float32)
+        r2 = (delta[:, 0] * delta[:, 0] + delta[:, 1] * delta[:, 1]) + delta[:, 2] * delta[:, 2]
+        dscale = f32(1) / np.where(r2 < 1, f32(1), r2)
+        if L["radius"] != 0:
+            dscale = np.where(r2 > L["radius"] * L["radius"], f32(0), dscale)
+        length = np.sqrt(r2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dn = np.where((length != 0)[:, None], delta / length[:, None], f32(0)).astype(np.float32)
+        dot = (dn[:, 0] * dn[:, 0] + dn[:, 1] * dn[:, 1]) + dn[:, 2] * dn[:, 2]
+        dot2 = -((dn[:, 0] * L["normal"][0] + dn[:, 1] * L["normal"][1]) + dn[:, 2] * L["normal"][2])
+        angle = np.where((dot < 0) | (dot2.astype(np.float64) <= 0.1 / 10), f32(0), dot * dot2)
+        ratio = (dscale * angle) * f32(1)
+        use = open_[:, k] & (ratio != 0)
+        for j in range(6):
+            b = BOX[j]
+            t = (b[0] * dn[:, 0] + b[1] * dn[:, 1]) + b[2] * dn[:, 2]
+            add = use & (t > 0)
+            w = t * ratio
+            cubes[:, j] = np.where(add[:, None], cubes[:, j] + L["intensity"][None, :] * w[:, None], cubes[:, j])
+
+
 def ambient_cubes(casters, faces: np.ndarray, sky_ambient, lump: np.ndarray, points: np.ndarray,
-                  threads: int = 1, bvh=None) -> np.ndarray:
-    """ComputeRayCube for every point: (n, 6, 3).  ``casters`` has
-    ``vertices`` (m, 9) and the caller's ``ids`` (the face index in their low
-    24 bits); ``bvh`` may be a CpuBvh already built over them."""
+                  threads: int = 1, bvh=None, lights=None) -> np.ndarray:
+    """ComputeCube for every point: (n, 6, 3).  ``casters`` has ``vertices``
+    (m, 9) and the caller's ``ids`` (the face index in their low 24 bits);
+    ``bvh`` may be a CpuBvh already built over them; ``lights`` the pass's
+    world lights (LIGHT_DTYPE)."""
     from . import CpuBvh, convert_triangles
 
     if bvh is None:
@@ -227,4 +289,12 @@ def ambient_cubes(casters, faces: np.ndarray, sky_ambient, lump: np.ndarray, poi
     surf = np.where(idx >= 0, ids[np.maximum(idx, 0)], -1).reshape(n, 162)
     frac = np.where(idx >= 0, t, np.float32(1.0)).astype(np.float32).reshape(n, 162)
     rad = ray_colors(faces, sky_ambient, np.asarray(lump, np.uint8), points, surf, frac)
-    return project(rad)
+    cubes = project(rad)
+    fl = flagged_lights(lights)
+    if len(fl):
+        seg = segment_rays(points, fl["origin"])
+        bits = bvh.visibility(seg, tmin=0.0, threads=threads)
+        m = seg.shape[0]
+        blocked = ((bits[np.arange(m) >> 6] >> (np.arange(m) & 63).astype(np.uint64)) & np.uint64(1)).astype(bool)
+        add_surface_lights(cubes, points, fl, ~blocked.reshape(n, len(fl)))
+    return cubes

@@ -30,7 +30,21 @@ def lit_rooms():
     return g, fset, lights, pvs, ff, fp, pts
 
 
-def run_gpu(g, fset, lights, pvs, ff, fp, pts, host_lump=False, sky=None):
+def cube_lights(g):
+    """Surface lights under each room's ceiling, facing down: dim ones the
+    cubes take (one with a radius), a bright one they do not."""
+    rows = []
+    for k, (lo, hi) in enumerate(zip(g.room_lo, g.room_hi)):
+        c = (lo + hi) / 2
+        for dx, inten, radius in ((-60, 300.0, 0.0), (60, 900.0, 150.0), (0, 5000.0, 0.0)):
+            rows.append(((c[0] + dx, c[1], hi[2] - 4), (0, 0, -1), (inten, inten * 0.5, inten * 0.25), radius))
+    L = np.zeros(len(rows), ambient.LIGHT_DTYPE)
+    for k, (o, n, i, r) in enumerate(rows):
+        L[k]["origin"], L[k]["normal"], L[k]["intensity"], L[k]["radius"] = o, n, i, r
+    return L
+
+
+def run_gpu(g, fset, lights, pvs, ff, fp, pts, host_lump=False, sky=None, wl=None):
     offer = HipTracerFactory().try_create(g.casters)
     assert offer.tracer is not None, offer.decline_reason
     with offer.tracer as tr:
@@ -45,16 +59,21 @@ def run_gpu(g, fset, lights, pvs, ff, fp, pts, host_lump=False, sky=None):
         lump = fin.run(bounce="host", patch_light=pl)
         faces = rooms.room_ambient_faces(g, ff, offs)
         amb = ambient.HipLeafAmbient(tr)
-        amb.load(faces, sky_ambient=sky, lump=lump if host_lump else None)
+        amb.load(faces, sky_ambient=sky, lump=lump if host_lump else None, lights=wl)
         cubes = amb.cubes(pts)
     return faces, lump, cubes
 
 
-@pytest.mark.parametrize("host_lump,sky", [(False, None), (True, (40.0, 60.0, 90.0))])
-def test_cubes_match_oracle(lit_rooms, host_lump, sky):
+@pytest.mark.parametrize("host_lump,sky,with_lights", [(False, None, False), (True, (40.0, 60.0, 90.0), False),
+                                                      (False, (40.0, 60.0, 90.0), True)])
+def test_cubes_match_oracle(lit_rooms, host_lump, sky, with_lights):
     g, fset, lights, pvs, ff, fp, pts = lit_rooms
-    faces, lump, cubes = run_gpu(*lit_rooms, host_lump=host_lump, sky=sky)
-    want = oa.ambient_cubes(g.casters, faces, sky, lump, pts, threads=8)
+    wl = cube_lights(g) if with_lights else None
+    faces, lump, cubes = run_gpu(*lit_rooms, host_lump=host_lump, sky=sky, wl=wl)
+    want = oa.ambient_cubes(g.casters, faces, sky, lump, pts, threads=8, lights=wl)
+    if with_lights:
+        base = oa.ambient_cubes(g.casters, faces, sky, lump, pts, threads=8)
+        assert np.any(want != base) and len(oa.flagged_lights(wl)) == 2 * len(g.room_lo)
     assert np.any(want > 0)
     same = cubes.view(np.uint32) == want.view(np.uint32)
     if not same.all():

@@ -147,3 +147,44 @@ def test_a_sky_face_gives_the_sky_ambient_and_a_miss_nothing():
     cube = oa.ambient_cubes(casters, faces, amb, lump, np.array([[256, 256, 5000]], np.float32))
     down = np.array([0.0, 0.0, 0.0])
     assert np.all(cube[0, :4] >= 0) and np.all(cube[0, 4] == down)
+
+
+def _lights(*rows):
+    L = np.zeros(len(rows), oa.LIGHT_DTYPE)
+    for k, (o, n, i, radius, style) in enumerate(rows):
+        L[k]["origin"], L[k]["normal"], L[k]["intensity"] = o, n, i
+        L[k]["radius"], L[k]["style"], L[k]["type"] = radius, style, oa.EMIT_SURFACE
+    return L
+
+
+def test_a_dim_surface_light_overhead_adds_intensity_over_r_squared_to_plus_z():
+    # one wall far off so the scene is not empty; the light at 244 units above
+    v9, ids = box_triangles(np.array([[5000.0, 0, 0]]), np.array([[5001.0, 1, 1]]))
+    casters = ShadowCasterSet(v9, (ids & 0xFFFFFF) | TRACE_ID_OPAQUE)
+    faces = np.zeros(6, oa.FACE_DTYPE)
+    faces["light_ofs"] = -1
+    lump = np.zeros(4, np.uint8)
+    pts = np.array([[256, 256, 256]], np.float32)
+    base = oa.ambient_cubes(casters, faces, None, lump, pts)
+    down = (0.0, 0.0, -1.0)
+    lights = _lights(((256, 256, 500), down, (100, 50, 25), 0.0, 0),      # flagged, faces the point
+                     ((256, 256, 500), down, (2000, 1, 1), 0.0, 0),       # too bright: not flagged
+                     ((256, 256, 500), down, (100, 100, 100), 0.0, 2),    # another style
+                     ((256, 256, 500), down, (100, 100, 100), 100.0, 0),  # out of its radius
+                     ((256, 256, 500), (0, 0, 1), (100, 100, 100), 0.0, 0))  # facing away
+    assert len(oa.flagged_lights(lights)) == 3
+    cube = oa.ambient_cubes(casters, faces, None, lump, pts, lights=lights)
+    add = cube - base
+    r = f32(1) / f32(244 * 244)
+    assert list(add[0, 4]) == [f32(100) * r, f32(50) * r, f32(25) * r]
+    assert np.all(add[0, [0, 1, 2, 3, 5]] == 0)
+
+
+def test_a_blocked_surface_light_adds_nothing():
+    casters, faces, lump = _box_room()
+    pts = np.array([[256, 256, 256]], np.float32)
+    base = oa.ambient_cubes(casters, faces, None, lump, pts)
+    outside = _lights(((256, 256, 900), (0, 0, -1), (100, 100, 100), 0.0, 0))
+    inside = _lights(((256, 256, 500), (0, 0, -1), (100, 100, 100), 0.0, 0))
+    assert np.array_equal(oa.ambient_cubes(casters, faces, None, lump, pts, lights=outside), base)
+    assert not np.array_equal(oa.ambient_cubes(casters, faces, None, lump, pts, lights=inside), base)
