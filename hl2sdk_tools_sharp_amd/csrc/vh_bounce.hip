@@ -262,14 +262,17 @@ __global__ __launch_bounds__(kTB) void k_candidates(PatchDev pd, const int32_t *
     if (!LIST && lane == 0) counts[k] = cnt;
 }
 
-constexpr int kRayBatches = 16;  // 64-candidate batches a transfer-ray wave takes
+constexpr int kRayBatches = 32;  // 64-candidate batches a transfer-ray wave takes
 
 // The candidate rays, one lane per candidate over all receivers' candidates
 // in order (VisMatrix.TransferRay and MakeTransfer).  A wave takes
-// kRayBatches x 64 consecutive candidates, so a batch is mostly one
-// receiver's: 64 rays from one start to neighbouring patches, which walk the
-// tree as a packet.  cand_w gets the transfer's weight, or 0 when the ray is
-// blocked or MakeTransfer drops it.
+// kRayBatches x 64 consecutive candidates, mostly one receiver's rays from one
+// start.  It first sorts them by direction (octant, major axis, the larger
+// minor axis: 48 bins) with LDS counters, so a batch of 64 walks the tree as
+// a packet of like directions, mostly in one octant (the specialised walk);
+// each candidate's answer depends only on its own ray, so the order of the
+// walk changes nothing.  cand_w[c] gets the transfer's weight, or 0 when the
+// ray is blocked or MakeTransfer drops it.
 __global__ __launch_bounds__(64) void k_transfer_rays(PatchDev pd, const int32_t *__restrict__ recv,
                                                       int32_t nrecv, const int64_t *__restrict__ cand_off,
                                                       const int32_t *__restrict__ cand_src, int64_t ncand,
@@ -279,23 +282,70 @@ __global__ __launch_bounds__(64) void k_transfer_rays(PatchDev pd, const int32_t
                                                       unsigned long long *__restrict__ stats)
 {
     extern __shared__ int wstk[];  // the wave's walk stack, 3 ints a level
+    __shared__ uint16_t order[kRayBatches * 64];  // the wave's candidates by bin
+    __shared__ int32_t start[49];                 // per bin: count, then next slot
     const int lane = lane_id();
     const int64_t first = (int64_t)blockIdx.x * (kRayBatches * 64);
-    // the receiver of the wave's first candidate: the last k with cand_off[k] <= first
     int32_t lo = 0, hi = nrecv;
     while (hi - lo > 1) {
         const int32_t mid = (lo + hi) >> 1;
         if (cand_off[mid] <= first) lo = mid; else hi = mid;
     }
-    int32_t kb = uniform(lo);
+    const int32_t kb0 = uniform(lo);
+    if (lane < 49) start[lane] = 0;
+    __syncthreads();
+    // pass 1: each candidate's direction bin (48: no ray), counted
+    int key[kRayBatches];
+    {
+        int32_t kb = kb0;
+#pragma unroll
+        for (int b = 0; b < kRayBatches; b++) {
+            const int64_t c = first + (int64_t)b * 64 + lane;
+            int kk = 48;
+            int32_t k = kb;
+            if (c < ncand) {
+                while (cand_off[k + 1] <= c) k++;
+                const int32_t r = recv[k];
+                const float4 ro = pd.origin_area[r], rn = pd.normal_pd[r];
+                const int32_t sc = cand_src[c];
+                const float4 so = pd.origin_area[sc], sn = pd.normal_pd[sc];
+                const float dx = (so.x + sn.x) - (ro.x + rn.x);
+                const float dy = (so.y + sn.y) - (ro.y + rn.y);
+                const float dz = (so.z + sn.z) - (ro.z + rn.z);
+                const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+                const int major = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+                const float m1 = major == 0 ? ay : ax, m2 = major == 2 ? ay : az;
+                const int sub = major * 2 + (m1 >= m2);
+                kk = ((dx < 0.0f) | ((dy < 0.0f) << 1) | ((dz < 0.0f) << 2)) * 6 + sub;
+            }
+            kb = uniform(__builtin_amdgcn_readlane(k, 63));
+            key[b] = kk;
+            atomicAdd(&start[kk], 1);
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int32_t run = 0;
+        for (int q = 0; q < 49; q++) {
+            const int32_t n = start[q];
+            start[q] = run;
+            run += n;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kRayBatches; b++) {
+        const int32_t pos = atomicAdd(&start[key[b]], 1);
+        order[pos] = (uint16_t)(b * 64 + lane);
+    }
+    __syncthreads();
     uint32_t nblocked = 0;
     for (int b = 0; b < kRayBatches; b++) {
-        const int64_t c = first + (int64_t)b * 64 + lane;
+        const int64_t c = first + order[b * 64 + lane];
         const bool valid = c < ncand;
-        int32_t k = kb;
+        int32_t k = kb0;
         if (valid)
             while (cand_off[k + 1] <= c) k++;
-        kb = uniform(__builtin_amdgcn_readlane(k, 63));
         Ray ray{};
         float4 ro{}, rn{}, so{};
         int32_t sc = 0;
