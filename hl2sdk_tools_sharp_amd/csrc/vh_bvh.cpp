@@ -157,8 +157,43 @@ struct Builder {
     static constexpr float kIntersect = 1.5f;
     static constexpr int kLeafMax = 8;
     static constexpr int32_t kForkMin = 8192;  // smallest subtree worth a thread
+    static constexpr int32_t kSplitPar = 65536; // smallest node whose scans are shared out
 
     explicit Builder(std::vector<Prim> &p) : prims(p) {}
+
+    // One scan's partial result over a range of primitives: the bounds of
+    // the boxes and of the centroids, or the three axes' bins.  Merging is
+    // min/max and integer sums, so any split of the range gives the same
+    // result as one serial scan.
+    struct Scan {
+        Box b, cb;
+        int cnt[3][kBins];
+        Box bb[3][kBins];
+    };
+
+    template <class F>
+    static void shared_scan(int32_t first, int32_t count, std::vector<Scan> &part, F &&f)
+    {
+        const int32_t nt = count >= kSplitPar ? (int32_t)build_threads() : 1;
+        part.assign((size_t)nt, Scan{});
+        std::vector<std::thread> pool;
+        int32_t started = 1;
+        try {
+            for (int32_t t = 1; t < nt; t++) {
+                pool.emplace_back([&, t]() {
+                    f(part[t], first + (int32_t)((int64_t)count * t / nt),
+                      first + (int32_t)((int64_t)count * (t + 1) / nt));
+                });
+                started = t + 1;
+            }
+        } catch (const std::system_error &) {
+        }
+        f(part[0], first, first + (int32_t)((int64_t)count / nt));
+        for (int32_t t = started; t < nt; t++)
+            f(part[t], first + (int32_t)((int64_t)count * t / nt),
+              first + (int32_t)((int64_t)count * (t + 1) / nt));
+        for (auto &th : pool) th.join();
+    }
 
     // fork > 0: this many more levels may hand their right subtree to a thread
     int32_t build(int32_t first, int32_t count, int depth, int fork)
@@ -168,11 +203,28 @@ struct Builder {
         Box b, cb;
         b.reset();
         cb.reset();
-        for (int32_t i = first; i < first + count; i++) {
-            b.grow(prims[i].box);
-            Box c;
-            for (int k = 0; k < 3; k++) c.lo[k] = c.hi[k] = prims[i].c[k];
-            cb.grow(c);
+        auto bounds = [&](Box &bo, Box &cbo, int32_t lo, int32_t hi) {
+            for (int32_t i = lo; i < hi; i++) {
+                bo.grow(prims[i].box);
+                Box c;
+                for (int k = 0; k < 3; k++) c.lo[k] = c.hi[k] = prims[i].c[k];
+                cbo.grow(c);
+            }
+        };
+        const bool par = count >= kSplitPar;
+        std::vector<Scan> part;
+        if (par) {
+            shared_scan(first, count, part, [&](Scan &sc, int32_t lo, int32_t hi) {
+                sc.b.reset();
+                sc.cb.reset();
+                bounds(sc.b, sc.cb, lo, hi);
+            });
+            for (const Scan &sc : part) {
+                b.grow(sc.b);
+                cb.grow(sc.cb);
+            }
+        } else {
+            bounds(b, cb, first, first + count);
         }
         nodes[me].box = b;
         max_depth = std::max(max_depth, depth);
@@ -190,15 +242,35 @@ struct Builder {
                 live[ax] = ext[ax] > 0;
                 for (auto &x : bb[ax]) x.reset();
             }
-            for (int32_t i = first; i < first + count; i++) {
-                const Prim &t = prims[i];
-                for (int ax = 0; ax < 3; ax++) {
-                    if (!live[ax]) continue;
-                    int bin = (int)((t.c[ax] - cb.lo[ax]) / ext[ax] * kBins);
-                    bin = std::min(std::max(bin, 0), kBins - 1);
-                    cnt[ax][bin]++;
-                    bb[ax][bin].grow(t.box);
+            auto bin_range = [&](int (&c)[3][kBins], Box (&bx)[3][kBins], int32_t lo, int32_t hi) {
+                for (int32_t i = lo; i < hi; i++) {
+                    const Prim &t = prims[i];
+                    for (int ax = 0; ax < 3; ax++) {
+                        if (!live[ax]) continue;
+                        int bin = (int)((t.c[ax] - cb.lo[ax]) / ext[ax] * kBins);
+                        bin = std::min(std::max(bin, 0), kBins - 1);
+                        c[ax][bin]++;
+                        bx[ax][bin].grow(t.box);
+                    }
                 }
+            };
+            if (par) {
+                shared_scan(first, count, part, [&](Scan &sc, int32_t lo, int32_t hi) {
+                    for (int ax = 0; ax < 3; ax++)
+                        for (int bin = 0; bin < kBins; bin++) {
+                            sc.cnt[ax][bin] = 0;
+                            sc.bb[ax][bin].reset();
+                        }
+                    bin_range(sc.cnt, sc.bb, lo, hi);
+                });
+                for (const Scan &sc : part)
+                    for (int ax = 0; ax < 3; ax++)
+                        for (int bin = 0; bin < kBins; bin++) {
+                            cnt[ax][bin] += sc.cnt[ax][bin];
+                            bb[ax][bin].grow(sc.bb[ax][bin]);
+                        }
+            } else {
+                bin_range(cnt, bb, first, first + count);
             }
             for (int ax = 0; ax < 3; ax++) {
                 if (!live[ax]) continue;
