@@ -62,7 +62,7 @@ class VhPatchDesc(ctypes.Structure):
         ("winding", ctypes.c_void_p), ("reflectivity", ctypes.c_void_p),
         ("face_sky", ctypes.c_void_p), ("face_parent", ctypes.c_void_p),
         ("row_faces_ptr", ctypes.c_void_p), ("row_faces", ctypes.c_void_p),
-        ("receivers", ctypes.c_void_p),
+        ("receivers", ctypes.c_void_p), ("bumped", ctypes.c_void_p), ("bump_normals", ctypes.c_void_p),
     ]
 
 
@@ -133,6 +133,7 @@ SIGNATURES = {
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
     "vh_patch_lights": (ctypes.c_int, [P, I32, P, P]),
+    "vh_bounce_bump_totals": (ctypes.c_int, [P, P]),
     "vh_final_load": (ctypes.c_int, [P, P]),
     "vh_final_layout": (ctypes.c_int, [P, P, ctypes.POINTER(I64)]),
     "vh_final_light": (ctypes.c_int, [P, P, P, P, I64]),

@@ -40,6 +40,7 @@ class BounceResult:
     residuals: np.ndarray  # (iters,)
     added: np.ndarray  # (iters, 3)
     iterations: int
+    bump_light: Optional[np.ndarray] = None  # (P, 3, 3): TotalLight's bump directions, when bumped
 
 
 class HipBounceStage:
@@ -60,7 +61,10 @@ class HipBounceStage:
     def _h(self):
         return self.tracer.handle
 
-    def load_patches(self) -> None:
+    def load_patches(self, bumped=None, bump_normals=None) -> None:
+        """Uploads the patch system.  ``bumped`` (one flag per patch,
+        Patch.NeedsBumpmap) with ``bump_normals`` ((P, 3, 3), each patch's bump
+        basis) makes those receivers gather per normal."""
         g = self.patches
         k = self._keep
         k["origin"] = np.ascontiguousarray(g.origin, np.float32)
@@ -86,6 +90,12 @@ class HipBounceStage:
                 "face", "cluster", "winding", "reflectivity", "face_sky", "face_parent",
                 "row_faces_ptr", "row_faces", "receivers")]
         )
+        self._bumped = bumped is not None
+        if bumped is not None:
+            k["bumped"] = np.ascontiguousarray(bumped, np.uint8).reshape(g.npatch)
+            k["bump_normals"] = np.ascontiguousarray(bump_normals, np.float32).reshape(g.npatch, 9)
+            d.bumped = k["bumped"].ctypes.data
+            d.bump_normals = k["bump_normals"].ctypes.data
         HipRayTracer._raise(_lib.lib().vh_patches_load(self._h, ctypes.byref(d)))
         self._loaded = True
 
@@ -177,7 +187,11 @@ class HipBounceStage:
                                  ctypes.byref(it))
         )
         n = it.value
-        return BounceResult(total, res[:n], en[:n], n)
+        bump = None
+        if getattr(self, "_bumped", False):
+            bump = np.zeros((P, 3, 3), np.float32)
+            HipRayTracer._raise(_lib.lib().vh_bounce_bump_totals(self._h, bump.ctypes.data_as(ctypes.c_void_p)))
+        return BounceResult(total, res[:n], en[:n], n, bump)
 
     async def bounce_async(self, emit0, **kw) -> BounceResult:
         import asyncio

@@ -42,6 +42,10 @@ struct BounceData {
     vh::DevBuf row_ptr, src, w32, w16, rscale, gather_rows;
     // bounce state
     vh::DevBuf e0, emit, shoot, add, total, partials, result;
+    // bumped patches: flags, the three bump normals (float4 each), and the
+    // gathered light and totals in the three bump directions
+    bool has_bump = false;
+    vh::DevBuf bumped, bnorm, add_b, total_b;
     // the faces' direct light handed to the patches (vh_patch_lights)
     vh::DevBuf direct, sample_area;
     bool have_direct = false;
@@ -510,6 +514,72 @@ __global__ __launch_bounds__(kGB) void k_half_rows(const int32_t *__restrict__ r
 // 64 consecutive transfers: 256 contiguous bytes of sources and of weights,
 // and shooters of about 64 consecutive patches (rows are sorted by source),
 // a few cache lines.  Four such steps are in flight at once.
+// GatherBumpedScalar / GatherBumpedFour (Radiosity.cs:406-512) for one bumped
+// receiver row, one wave: per transfer the direction to the source
+// normalised (exact length, true divides, zero for a zero length), the
+// receiver's cosine divided out, and shoot * (w * scale) added along each of
+// the four normals (the plane normal, then the bump basis) the source is not
+// behind (Accumulate, :523-531).  Partial sums per lane, reduced across the
+// wave.
+template <bool F16>
+__device__ void gather_bumped(int32_t r, int64_t b, int64_t e, const int32_t *__restrict__ src,
+                              const float *__restrict__ w32, const __half *__restrict__ w16,
+                              const float *__restrict__ rscale, const float4 *__restrict__ shoot,
+                              const float4 *__restrict__ origin_area,
+                              const float4 *__restrict__ normal_pd,
+                              const float4 *__restrict__ bnorm, float4 *__restrict__ add,
+                              float4 *__restrict__ add_b)
+{
+    const int lane = threadIdx.x & 63;
+    const float4 ro = origin_area[r];
+    float4 nrm[4];
+    nrm[0] = normal_pd[r];
+    nrm[1] = bnorm[(size_t)r * 3];
+    nrm[2] = bnorm[(size_t)r * 3 + 1];
+    nrm[3] = bnorm[(size_t)r * 3 + 2];
+    float sum[4][3];
+#pragma unroll
+    for (int n = 0; n < 4; n++) sum[n][0] = sum[n][1] = sum[n][2] = 0.0f;
+    for (int64_t j = b + lane; j < e; j += 64) {
+        const int32_t s = src[j];
+        const float w = F16 ? __half2float(w16[j]) : w32[j];
+        const float4 so = origin_area[s];
+        float dx = so.x - ro.x, dy = so.y - ro.y, dz = so.z - ro.z;
+        const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+        if (len != 0.0f) {
+            dx = dx / len;
+            dy = dy / len;
+            dz = dz / len;
+        } else {
+            dx = dy = dz = 0.0f;
+        }
+        const float scale = 1.0f / ((dx * nrm[0].x + dy * nrm[0].y) + dz * nrm[0].z);
+        const float ws = w * scale;
+        const float4 h = shoot[s];
+        const float vx = h.x * ws, vy = h.y * ws, vz = h.z * ws;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const float d = (dx * nrm[n].x + dy * nrm[n].y) + dz * nrm[n].z;
+            if (!(d <= 0.0f)) {
+                sum[n][0] = sum[n][0] + vx * d;
+                sum[n][1] = sum[n][1] + vy * d;
+                sum[n][2] = sum[n][2] + vz * d;
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+        for (int c = 0; c < 3; c++)
+            for (int off = 32; off > 0; off >>= 1) sum[n][c] += __shfl_xor(sum[n][c], off, 64);
+    if (lane == 0) {
+        const float sc = F16 ? rscale[r] : 1.0f;
+        add[r] = make_float4(sum[0][0] * sc, sum[0][1] * sc, sum[0][2] * sc, 0.0f);
+#pragma unroll
+        for (int n = 1; n < 4; n++)
+            add_b[(size_t)r * 3 + n - 1] = make_float4(sum[n][0] * sc, sum[n][1] * sc, sum[n][2] * sc, 0.0f);
+    }
+}
+
 template <bool F16>
 __global__ __launch_bounds__(kGB) void k_gather(const int32_t *__restrict__ rows, int32_t nrows,
                                                 const int64_t *__restrict__ row_ptr,
@@ -518,13 +588,23 @@ __global__ __launch_bounds__(kGB) void k_gather(const int32_t *__restrict__ rows
                                                 const __half *__restrict__ w16,
                                                 const float *__restrict__ rscale,
                                                 const float4 *__restrict__ shoot,
-                                                float4 *__restrict__ add)
+                                                float4 *__restrict__ add,
+                                                const uint8_t *__restrict__ bumped,
+                                                const float4 *__restrict__ origin_area,
+                                                const float4 *__restrict__ normal_pd,
+                                                const float4 *__restrict__ bnorm,
+                                                float4 *__restrict__ add_b)
 {
     const int lane = threadIdx.x & 63;
     const int64_t k = (int64_t)blockIdx.x * (kGB / 64) + (threadIdx.x >> 6);
     if (k >= nrows) return;
     const int32_t r = rows[k];
     const int64_t b = row_ptr[r], e = row_ptr[r + 1];
+    if (bumped && bumped[r]) {
+        gather_bumped<F16>(r, b, e, src, w32, w16, rscale, shoot, origin_area, normal_pd, bnorm, add,
+                           add_b);
+        return;
+    }
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     auto acc = [&](const float4 &h, float w) {
         ax = __builtin_fmaf(h.x, w, ax);
@@ -604,7 +684,10 @@ __global__ __launch_bounds__(kCB) void k_collect(const int32_t *__restrict__ lea
                                                  float4 *__restrict__ shoot,
                                                  const float4 *__restrict__ e0,
                                                  const float4 *__restrict__ refl,
-                                                 double *__restrict__ partials)
+                                                 double *__restrict__ partials,
+                                                 const uint8_t *__restrict__ bumped,
+                                                 const float4 *__restrict__ add_b,
+                                                 float4 *__restrict__ total_b)
 {
     __shared__ double lds[kCB / 64];
     int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
@@ -632,6 +715,15 @@ __global__ __launch_bounds__(kCB) void k_collect(const int32_t *__restrict__ lea
             ey = em.y;
             ez = em.z;
             float4 rf = refl[p];
+            if (bumped && bumped[p]) {
+                // the bump directions collect as the flat slot does
+                for (int n = 0; n < 3; n++) {
+                    const float4 ab = add_b[(size_t)p * 3 + n];
+                    float4 tb = total_b[(size_t)p * 3 + n];
+                    tb = mode == 0 ? make_float4(tb.x + ab.x, tb.y + ab.y, tb.z + ab.z, 0.0f) : ab;
+                    total_b[(size_t)p * 3 + n] = tb;
+                }
+            }
             total[p] = t;
             emit[p] = make_float4(em.x, em.y, em.z, 0.0f);
             shoot[p] = make_float4(em.x * rf.x, em.y * rf.y, em.z * rf.z, 0.0f);
@@ -669,6 +761,35 @@ __global__ __launch_bounds__(kCB) void k_pullup(const int32_t *__restrict__ list
         float4 rf = refl[p];
         shoot[p] = make_float4(r.x * rf.x, r.y * rf.y, r.z * rf.z, 0.0f);
     }
+}
+
+// The bump directions' totals of bumped parents from their children, one tree
+// level (Radiosity.cs:292-311: a parent blends every normal it has).
+__global__ __launch_bounds__(kCB) void k_pullup_bump(const int32_t *__restrict__ list, int32_t n,
+                                                     const int2 *__restrict__ children,
+                                                     const float4 *__restrict__ origin_area,
+                                                     const uint8_t *__restrict__ bumped,
+                                                     float4 *__restrict__ tb)
+{
+    int64_t i = (int64_t)blockIdx.x * kCB + threadIdx.x;
+    if (i >= n) return;
+    int32_t p = list[i];
+    if (!bumped[p]) return;
+    int2 ch = children[p];
+    float a1 = origin_area[ch.x].w, a2 = origin_area[ch.y].w;
+    float s1 = a1 / (a1 + a2);
+    float s2 = a2 / (a1 + a2);
+    for (int k = 0; k < 3; k++) {
+        float4 x = tb[(size_t)ch.x * 3 + k], y = tb[(size_t)ch.y * 3 + k];
+        tb[(size_t)p * 3 + k] = make_float4((x.x * s1) + (y.x * s2), (x.y * s1) + (y.y * s2),
+                                            (x.z * s1) + (y.z * s2), 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // The FP64 partials summed in a fixed order (deterministic).
@@ -848,7 +969,8 @@ void bounce_release(vh_ctx *c)
                       &B->row_faces_ptr, &B->row_faces, &B->receivers, &B->leaves, &B->levels,
                       &B->row_ptr, &B->src, &B->w32, &B->w16, &B->rscale, &B->gather_rows,
                       &B->e0, &B->emit, &B->shoot, &B->add, &B->total, &B->partials, &B->result,
-                      &B->direct, &B->sample_area, &B->pred})
+                      &B->direct, &B->sample_area, &B->pred, &B->bumped, &B->bnorm, &B->add_b,
+                      &B->total_b})
         b->release();
     if (B->h_result) hipHostFree(B->h_result);
     delete B;
@@ -866,6 +988,7 @@ bool bounce_view(vh_ctx *c, BounceView *v)
     v->e0 = B->have_e0 ? (const float4 *)B->e0.ptr : nullptr;
     v->direct = B->have_direct ? (const float4 *)B->direct.ptr : nullptr;
     v->total = B->have_total ? (const float4 *)B->total.ptr : nullptr;
+    v->total_b = B->have_total && B->has_bump ? (const float4 *)B->total_b.ptr : nullptr;
     v->leaves = (const int32_t *)B->leaves.ptr;
     v->sky = (const uint8_t *)B->sky.ptr;
     v->cluster = (const int32_t *)B->cluster.ptr;
@@ -919,7 +1042,7 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
     if (P <= 0 || F <= 0 || C <= 0 || R < 0 || !d->origin || !d->normal || !d->plane_dist ||
         !d->area || !d->child1 || !d->child2 || !d->next_parent || !d->face || !d->cluster ||
         !d->winding || !d->reflectivity || !d->face_sky || !d->face_parent || !d->row_faces_ptr ||
-        !d->row_faces || (R > 0 && !d->receivers)) {
+        !d->row_faces || (R > 0 && !d->receivers) || (d->bumped && !d->bump_normals)) {
         set_error("incomplete patch descriptor");
         return VH_EINVAL;
     }
@@ -1047,6 +1170,23 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
         (rc = B->result.ensure(64, "residuals")))
         return rc;
     if (!B->h_result) VH_HIP(hipHostMalloc((void **)&B->h_result, 64, hipHostMallocDefault));
+    B->has_bump = false;
+    if (d->bumped) {
+        std::vector<uint8_t> bf(d->bumped, d->bumped + P);
+        std::vector<float4> bn((size_t)P * 3);
+        for (int32_t p = 0; p < P; p++) {
+            bf[p] = bf[p] != 0;
+            B->has_bump = B->has_bump || bf[p];
+            for (int k = 0; k < 3; k++)
+                bn[(size_t)p * 3 + k] = make_float4(d->bump_normals[9 * p + 3 * k], d->bump_normals[9 * p + 3 * k + 1],
+                                                    d->bump_normals[9 * p + 3 * k + 2], 0.0f);
+        }
+        if ((rc = upload(B->bumped, bf.data(), bf.size(), "bump flags", s)) ||
+            (rc = upload(B->bnorm, bn.data(), bn.size(), "bump normals", s)) ||
+            (rc = B->add_b.ensure(v4 * 3, "bump gathered light")) ||
+            (rc = B->total_b.ensure(v4 * 3, "bump totals")))
+            return rc;
+    }
     VH_HIP(hipStreamSynchronize(s));
     return VH_OK;
 }
@@ -1374,6 +1514,14 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
         }
     VH_HIP(hipGetLastError());
 
+    const uint8_t *bumped = B->has_bump ? (const uint8_t *)B->bumped.ptr : nullptr;
+    float4 *add_b = (float4 *)B->add_b.ptr, *total_b = (float4 *)B->total_b.ptr;
+    if (bumped) {
+        const int64_t nb = (int64_t)P * 3;
+        hipLaunchKernelGGL(k_zero4, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, add_b, nb);
+        hipLaunchKernelGGL(k_zero4, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, total_b, nb);
+    }
+    const float4 *npd = (const float4 *)B->normal_pd.ptr, *bnorm = (const float4 *)B->bnorm.ptr;
     int32_t nblk = (B->nleaves + kCB - 1) / kCB;
     bool f16 = o->weights == VH_WEIGHTS_F16;
     auto iteration = [&](bool timed) {
@@ -1386,20 +1534,22 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
                                    (const int32_t *)B->gather_rows.ptr, B->nrecv,
                                    (const int64_t *)B->row_ptr.ptr, (const int32_t *)B->src.ptr,
                                    (const float *)B->w32.ptr, (const __half *)B->w16.ptr,
-                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add);
+                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add, bumped,
+                                   oa, npd, bnorm, add_b);
             else
                 hipLaunchKernelGGL(k_gather<false>, dim3(g), dim3(kGB), 0, s,
                                    (const int32_t *)B->gather_rows.ptr, B->nrecv,
                                    (const int64_t *)B->row_ptr.ptr, (const int32_t *)B->src.ptr,
                                    (const float *)B->w32.ptr, (const __half *)B->w16.ptr,
-                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add);
+                                   (const float *)B->rscale.ptr, (const float4 *)shoot, add, bumped,
+                                   oa, npd, bnorm, add_b);
             if (timed) timer.done();
         }
         if (nblk > 0)
             hipLaunchKernelGGL(k_collect, dim3((uint32_t)nblk), dim3(kCB), 0, s,
                                (const int32_t *)B->leaves.ptr, B->nleaves, sky, kmode,
                                (const float4 *)add, total, emit, shoot, e0, refl,
-                               (double *)B->partials.ptr);
+                               (double *)B->partials.ptr, bumped, (const float4 *)add_b, total_b);
         for (int32_t lv = 0; lv < B->height; lv++) {
             int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
             if (n > 0)
@@ -1468,6 +1618,9 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
         if (n > 0)
             hipLaunchKernelGGL(k_pullup<false>, dim3((uint32_t)((n + kCB - 1) / kCB)), dim3(kCB),
                                0, s, levels + a, n, children, oa, total, nullptr, refl);
+        if (n > 0 && bumped)
+            hipLaunchKernelGGL(k_pullup_bump, dim3((uint32_t)((n + kCB - 1) / kCB)), dim3(kCB), 0, s,
+                               levels + a, n, children, oa, bumped, total_b);
     }
     VH_HIP(hipGetLastError());
     if (total_out) {
@@ -1479,6 +1632,37 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     VH_HIP(hipStreamSynchronize(s));
     if (iters) *iters = it;
     B->have_total = true;
+    return VH_OK;
+}
+
+int vh_bounce_bump_totals(vh_ctx *c, float *out)
+{
+    clear_error();
+    if (!c || !out) {
+        set_error("NULL context or output");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    BounceData *B = c->bounce;
+    if (!B || !B->have_total) {
+        set_error("no bounce result: call vh_bounce first");
+        return VH_ESTATE;
+    }
+    const int32_t P = B->npatch;
+    if (!B->has_bump) {
+        std::memset(out, 0, (size_t)P * 9 * sizeof(float));
+        return VH_OK;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    std::vector<float4> buf((size_t)P * 3);
+    VH_HIP(hipMemcpyAsync(buf.data(), B->total_b.ptr, buf.size() * sizeof(float4), hipMemcpyDeviceToHost,
+                          c->stream));
+    VH_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < buf.size(); i++) {
+        out[3 * i] = buf[i].x;
+        out[3 * i + 1] = buf[i].y;
+        out[3 * i + 2] = buf[i].z;
+    }
     return VH_OK;
 }
 

@@ -479,17 +479,17 @@ __global__ __launch_bounds__(kWave) void k_final(FinalArgs a)
         final_face<1>(a, face);
 }
 
-// The flat bounce's light as TotalLight: slot 0 the bounced light, the bump
-// slots zero (read only by bumped faces' own patches, which the flat mode
-// rejects).
-__global__ void k_plight_flat(const float4 *__restrict__ total, int32_t npatch,
-                              float4 *__restrict__ plight)
+// The bounce's light as TotalLight: slot 0 the flat light, slots 1-3 the bump
+// directions' (zero when the bounce had no bumped patches; then only flat
+// patches are read, which the host checks).
+__global__ void k_plight(const float4 *__restrict__ total, const float4 *__restrict__ total_b,
+                         int32_t npatch, float4 *__restrict__ plight)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npatch * 4) return;
     int32_t p = i >> 2, b = i & 3;
-    float4 t = total[p];
-    plight[i] = b == 0 ? make_float4(t.x, t.y, t.z, 0.0f) : make_float4(0, 0, 0, 0);
+    float4 t = b == 0 ? total[p] : (total_b ? total_b[(size_t)p * 3 + b - 1] : make_float4(0, 0, 0, 0));
+    plight[i] = make_float4(t.x, t.y, t.z, 0.0f);
 }
 
 }  // namespace
@@ -770,14 +770,15 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
                       " patches and the final light's " + std::to_string(P));
             return VH_EINVAL;
         }
-        if (D->bumped_owner) {
-            set_error("the bounce is flat and a bumped face owns leaf patches: pass TotalLight "
-                      "per normal with VH_FINAL_BOUNCE_HOST");
+        if (D->bumped_owner && !bv.total_b) {
+            set_error("the bounce had no bumped patches and a bumped face owns leaf patches: load "
+                      "the patches with their bump flags, or pass TotalLight per normal with "
+                      "VH_FINAL_BOUNCE_HOST");
             return VH_ENOTSUP;
         }
         if (P > 0)
-            hipLaunchKernelGGL(k_plight_flat, dim3((uint32_t)((P * 4 + 255) / 256)), dim3(256), 0,
-                               st, bv.total, P, (float4 *)D->plight.ptr);
+            hipLaunchKernelGGL(k_plight, dim3((uint32_t)((P * 4 + 255) / 256)), dim3(256), 0, st,
+                               bv.total, bv.total_b, P, (float4 *)D->plight.ptr);
     } else if (o->bounce == VH_FINAL_BOUNCE_HOST) {
         if (P > 0 && !patch_light) {
             set_error("patch_light is required with VH_FINAL_BOUNCE_HOST");

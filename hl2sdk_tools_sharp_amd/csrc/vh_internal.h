@@ -147,6 +147,7 @@ struct BounceView {
     const float4 *e0 = nullptr;           // the last vh_bounce's direct light, or null
     const float4 *direct = nullptr;       // the last vh_patch_lights result, or null
     const float4 *total = nullptr;        // the last vh_bounce's bounced light, or null
+    const float4 *total_b = nullptr;      // its bump directions (3 per patch), or null
     const int32_t *leaves = nullptr;      // nleaves leaf patches, ascending
     const uint8_t *sky = nullptr;         // per patch
     const int32_t *cluster = nullptr;     // per patch

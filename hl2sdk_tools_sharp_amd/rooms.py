@@ -448,6 +448,17 @@ def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
                    g.face_cluster[fs].astype(np.int32), area.astype(np.float32))
 
 
+def patch_bumps(g: RoomGrid, fset):
+    """Per patch, whether it gathers per bump normal (Patch.NeedsBumpmap: its
+    face carries bump normals) and its three bump normals.  The reference
+    builds each patch's basis from the phong normal at its origin
+    (Radiosity.PrepareNormals, Rad/Bounce/Radiosity.cs:322-356); the grid's
+    faces are flat and unsmoothed, so every patch takes its face's basis."""
+    bumped = (np.asarray(fset.face_normal_count)[g.face] == 4).astype(np.uint8)
+    normals = np.where(bumped[:, None, None] == 1, np.asarray(fset.face_normals, np.float32)[g.face, 1:4], 0)
+    return bumped, np.ascontiguousarray(normals, np.float32)
+
+
 def room_lights(g: RoomGrid, sun: bool = True, sky_ambient: bool = False, spot_every: int = 0,
                 surface_every: int = 0):
     """The grid's point lights (quadratic falloff 1/d^2, white, intensity from

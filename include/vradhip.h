@@ -168,6 +168,12 @@ typedef struct vh_patch_desc {
     const int32_t *face_parent;
     const int32_t *row_faces_ptr, *row_faces;
     const int32_t *receivers;
+    /* Bumped patches (Patch.NeedsBumpmap; may be NULL: none): one flag per
+     * patch and, per patch, the three bump basis normals (9 floats) the
+     * gather re-applies the cosine on (Radiosity.PrepareNormals,
+     * Rad/Bounce/Radiosity.cs:322-356; the flat slot uses the plane normal). */
+    const uint8_t *bumped;
+    const float *bump_normals;
 } vh_patch_desc;
 
 /* Uploads the patch system; replaces any previous one and its transfers. */
@@ -227,6 +233,15 @@ typedef struct vh_bounce_opts {
  * iteration the residual and the summed leaf emission (3 doubles). */
 int vh_bounce(vh_ctx *ctx, const vh_bounce_opts *opts, const float *emit0, const float *start,
               float *total_out, double *residuals, double *energies, int32_t *iters);
+
+/* The last vh_bounce's TotalLight in the three bump directions (9 floats per
+ * patch, zero for a flat patch).  A bumped receiver gathers as
+ * GatherBumpedScalar / GatherBumpedFour do (Rad/Bounce/Radiosity.cs:406-512):
+ * per transfer the direction to the source normalised, the receiver's cosine
+ * divided out, and the light added along each of its four normals that the
+ * source is not behind; its flat slot is vh_bounce's total_out.  Leaves
+ * collect, bumped parents take their children's area-weighted blend. */
+int vh_bounce_bump_totals(vh_ctx *ctx, float *out);
 
 /* ---- The direct-light gather: RadWorld.LightFacesAsync (SourceSharp.MapTools/Rad/Light/RadWorld.Facelights.cs:194) ---- */
 
@@ -383,8 +398,9 @@ int vh_final_load(vh_ctx *ctx, const vh_final_desc *desc);
 int vh_final_layout(vh_ctx *ctx, int32_t *offsets, int64_t *bytes);
 
 #define VH_FINAL_NO_BOUNCE 0     /* no patch radial (Bounces = 0) */
-#define VH_FINAL_BOUNCE_DEVICE 1 /* patch lights = the last vh_bounce's bounced light
-                                    (flat: a bumped face may own no leaf patch) */
+#define VH_FINAL_BOUNCE_DEVICE 1 /* patch lights = the last vh_bounce's TotalLight, flat and
+                                    bump directions (a bumped face may own leaf patches
+                                    only when the patches were loaded with bump flags) */
 #define VH_FINAL_BOUNCE_HOST 2   /* patch lights from the patch_light argument */
 
 typedef struct vh_final_opts {

@@ -41,7 +41,8 @@ def _bind():
     L.vc_make_transfer.argtypes = [ctypes.POINTER(PatchSetC), i32, i32, ctypes.POINTER(f32)]
     L.vc_make_transfer.restype = ctypes.c_int
     L.vc_transfers_build.argtypes = [ctypes.POINTER(PatchSetC), P_, P_, P_, P_, P_, i32]
-    L.vc_bounce.argtypes = [i32, P_, P_, P_, P_, P_, P_, P_, P_, P_, P_, i32, i32, f64, i32, P_, P_, P_, P_]
+    L.vc_bounce.argtypes = [i32, P_, P_, P_, P_, P_, P_, P_, P_, P_, P_, i32, i32, f64, i32, P_, P_, P_, P_,
+                            P_, P_, P_, P_]
     L._bounce_bound = True
     return L
 
@@ -140,8 +141,12 @@ def build_transfers(g, threads=None):
     return row_ptr, src[:total][keep].copy(), w[:total][keep].copy(), int(counts.sum())
 
 
-def bounce(g, row_ptr, src, w, emit0, start=None, mode=0, max_iter=100, tol=0.0, use_double=False):
+def bounce(g, row_ptr, src, w, emit0, start=None, mode=0, max_iter=100, tol=0.0, use_double=False,
+           bumped=None, bump_normals=None, bump_out=None):
     """Radiosity.BounceAsync restated (see vradcpu_bounce.c for the modes).
+    ``bumped`` (a flag per patch) with ``bump_normals`` ((P, 3, 3)) gathers
+    those receivers per normal (GatherBumpedScalar); ``bump_out`` ((P, 3, 3)
+    float64) then receives TotalLight's bump directions.
     Returns (total (P,3) float64, residuals, energies (iters,3), iters)."""
     L = _bind()
     P = g.npatch
@@ -156,12 +161,24 @@ def bounce(g, row_ptr, src, w, emit0, start=None, mode=0, max_iter=100, tol=0.0,
     rp = np.ascontiguousarray(row_ptr, np.int64)
     s = np.ascontiguousarray(src, np.int32)
     ww = np.ascontiguousarray(w, np.float32)
+    bump_args = (None, None, None, None)
+    if bumped is not None:
+        org = np.ascontiguousarray(g.origin, np.float32)
+        n4 = np.zeros((P, 4, 3), np.float32)
+        n4[:, 0] = g.normal
+        n4[:, 1:] = np.asarray(bump_normals, np.float32).reshape(P, 3, 3)
+        bf = np.ascontiguousarray(bumped, np.uint8)
+        tb = np.zeros((P, 3, 3), np.float64)
+        keep = (org, n4, bf, tb)
+        bump_args = (_p(org), _p(n4), _p(bf), _p(tb))
     L.vc_bounce(P, _p(rp), _p(s), _p(ww), _p(refl), _p(e0), _p(sky),
                 _p(np.ascontiguousarray(g.child1, np.int32)), _p(np.ascontiguousarray(g.child2, np.int32)),
                 _p(np.ascontiguousarray(g.area, np.float32)), None if st is None else _p(st),
                 int(mode), int(max_iter), float(tol), 1 if use_double else 0, _p(out), _p(res), _p(en),
-                ctypes.byref(it))
+                ctypes.byref(it), *bump_args)
     n = it.value
+    if bumped is not None and bump_out is not None:
+        bump_out[...] = keep[3]
     return out, res[:n], en[:n], n
 
 

@@ -111,7 +111,11 @@ void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const floa
                const float *refl, const float *emit0, const uint8_t *sky, const int32_t *child1,
                const int32_t *child2, const float *area, const float *start, int32_t mode,
                int32_t max_iter, double tol, int32_t use_double, double *total_out,
-               double *residuals, double *energies, int32_t *iters);
+               double *residuals, double *energies, int32_t *iters, const float *origin,
+               const float *normals4, const uint8_t *bumped, double *total_bump_out);
+/* (vc_bounce's last four: bumped receivers, may all be NULL; origin 3 and
+ * normals4 12 floats per patch -- the plane normal, then the bump basis --
+ * bumped one flag per patch, total_bump_out 9 doubles per patch.) */
 
 /* ---- direct-light gather, sampler, encode (vradcpu_gather.c) ---- */
 

@@ -291,12 +291,63 @@ void vc_transfers_build(const vc_patchset *ps, const vc_bvh *bvh, const int64_t 
  * the summed leaf emission below 1.0 (Radiosity.cs:163).  Either way at most
  * max_iter iterations.  Parents' totals are pulled up from their children at
  * the end, as CollectLight leaves them. */
+/* GatherBumpedScalar / GatherBumpedFour (Rad/Bounce/Radiosity.cs:406-512) for
+ * receiver j: per transfer the direction to the source normalised (exact
+ * length, true divides, the zero vector for a zero length), the receiver's
+ * cosine divided out, and shoot * (w * scale) added along each of the four
+ * normals (normals4: the plane normal, then the three bump normals) unless the
+ * source is behind it (Accumulate, :523-531).  The geometry is float; the sums
+ * are in the shooters' precision (dbl: double).  out: 4 x 3 sums. */
+static void gather_bumped_row(int32_t j, const int64_t *row_ptr, const int32_t *src, const float *w,
+                              const float *origin, const float *normals4, const void *shoot, int dbl,
+                              void *out)
+{
+    const float *n = normals4 + (int64_t)j * 12;
+    double sd[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    float sf[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int64_t k = row_ptr[j]; k < row_ptr[j + 1]; k++) {
+        int32_t s = src[k];
+        float dx = origin[3 * s] - origin[3 * j];
+        float dy = origin[3 * s + 1] - origin[3 * j + 1];
+        float dz = origin[3 * s + 2] - origin[3 * j + 2];
+        float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+        if (len != 0.0f) {
+            dx = dx / len;
+            dy = dy / len;
+            dz = dz / len;
+        } else {
+            dx = dy = dz = 0.0f;
+        }
+        float scale = 1.0f / ((dx * n[0] + dy * n[1]) + dz * n[2]);
+        float ws = w[k] * scale;
+        for (int q = 0; q < 4; q++) {
+            float d = (dx * n[3 * q] + dy * n[3 * q + 1]) + dz * n[3 * q + 2];
+            if (d <= 0.0f) continue;
+            for (int c = 0; c < 3; c++) {
+                if (dbl) {
+                    double v = ((const double *)shoot)[3 * s + c] * (double)ws;
+                    sd[q][c] = sd[q][c] + v * (double)d;
+                } else {
+                    float v = ((const float *)shoot)[3 * s + c] * ws;
+                    sf[q][c] = sf[q][c] + v * d;
+                }
+            }
+        }
+    }
+    if (dbl)
+        memcpy(out, sd, sizeof sd);
+    else
+        memcpy(out, sf, sizeof sf);
+}
+
 #define BOUNCE_BODY(T)                                                                        \
     {                                                                                         \
         T *emit = (T *)calloc((size_t)P * 3, sizeof(T));                                     \
         T *shoot = (T *)calloc((size_t)P * 3, sizeof(T));                                    \
         T *add = (T *)calloc((size_t)P * 3, sizeof(T));                                      \
         T *tot = (T *)calloc((size_t)P * 3, sizeof(T));                                      \
+        T *addb = (T *)calloc((size_t)P * 9, sizeof(T));                                     \
+        T *totb = (T *)calloc((size_t)P * 9, sizeof(T));                                     \
         for (int32_t i = 0; i < P; i++)                                                       \
             for (int c = 0; c < 3; c++) {                                                     \
                 T x0 = start ? (T)start[3 * i + c] : (T)0;                                    \
@@ -321,6 +372,16 @@ void vc_transfers_build(const vc_patchset *ps, const vc_bvh *bvh, const int64_t 
         int32_t it = 0;                                                                       \
         for (;;) {                                                                            \
             for (int32_t j = 0; j < P; j++) {                                                 \
+                if (bumped && bumped[j]) {                                                    \
+                    T sb[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};                \
+                    gather_bumped_row(j, row_ptr, src, w, origin, normals4,                   \
+                                      (const void *)shoot, sizeof(T) == 8, (void *)sb);       \
+                    for (int c = 0; c < 3; c++) {                                             \
+                        add[3 * j + c] = sb[0][c];                                            \
+                        for (int n = 1; n < 4; n++) addb[9 * j + 3 * (n - 1) + c] = sb[n][c]; \
+                    }                                                                         \
+                    continue;                                                                 \
+                }                                                                             \
                 T sx = 0, sy = 0, sz = 0;                                                     \
                 for (int64_t k = row_ptr[j]; k < row_ptr[j + 1]; k++) {                       \
                     int32_t s = src[k];                                                       \
@@ -338,6 +399,10 @@ void vc_transfers_build(const vc_patchset *ps, const vc_bvh *bvh, const int64_t 
                 if (sky[i]) {                                                                 \
                     for (int c = 0; c < 3; c++) emit[3 * i + c] = shoot[3 * i + c] = (T)0;    \
                 } else if (child1[i] == -1) {                                                 \
+                    if (bumped && bumped[i])                                                  \
+                        for (int q = 0; q < 9; q++)                                           \
+                            totb[9 * i + q] = mode == 0 ? totb[9 * i + q] + addb[9 * i + q]   \
+                                                        : addb[9 * i + q];                    \
                     for (int c = 0; c < 3; c++) {                                             \
                         T a = add[3 * i + c];                                                 \
                         if (mode == 0) {                                                      \
@@ -385,19 +450,33 @@ void vc_transfers_build(const vc_patchset *ps, const vc_bvh *bvh, const int64_t 
                     tot[3 * i + c] = (tot[3 * a + c] * s1) + (tot[3 * b + c] * s2);           \
             }                                                                                 \
         }                                                                                     \
+        for (int32_t i = P - 1; bumped && i >= 0; i--) {                                     \
+            if (child1[i] != -1 && !sky[i] && bumped[i]) {                                    \
+                int32_t a = child1[i], b = child2[i];                                         \
+                T s1 = (T)area[a] / ((T)area[a] + (T)area[b]);                                \
+                T s2 = (T)area[b] / ((T)area[a] + (T)area[b]);                                \
+                for (int q = 0; q < 9; q++)                                                   \
+                    totb[9 * i + q] = (totb[9 * a + q] * s1) + (totb[9 * b + q] * s2);        \
+            }                                                                                 \
+        }                                                                                     \
         for (int64_t i = 0; i < (int64_t)P * 3; i++) total_out[i] = (double)tot[i];           \
+        if (total_bump_out)                                                                   \
+            for (int64_t i = 0; i < (int64_t)P * 9; i++) total_bump_out[i] = (double)totb[i];  \
         *iters = it;                                                                          \
         free(emit);                                                                           \
         free(shoot);                                                                          \
         free(add);                                                                            \
         free(tot);                                                                            \
+        free(addb);                                                                           \
+        free(totb);                                                                           \
     }
 
 void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const float *w,
                const float *refl, const float *emit0, const uint8_t *sky, const int32_t *child1,
                const int32_t *child2, const float *area, const float *start, int32_t mode,
                int32_t max_iter, double tol, int32_t use_double, double *total_out,
-               double *residuals, double *energies, int32_t *iters)
+               double *residuals, double *energies, int32_t *iters, const float *origin,
+               const float *normals4, const uint8_t *bumped, double *total_bump_out)
 {
     if (use_double)
         BOUNCE_BODY(double)

@@ -110,6 +110,38 @@ def test_bounce_fp32_matches_oracle(case, graph):
     np.testing.assert_allclose(r.added[: min(n32, r.iterations)], en32[: min(n32, r.iterations)], rtol=1e-4)
 
 
+def test_bumped_receivers_match_oracle(case):
+    """GatherBumpedScalar (Rad/Bounce/Radiosity.cs:406-421): every third face
+    bumped; the flat totals and the three bump directions' totals within 1e-5
+    (relative L1) of the oracle's FP32 form, parents pulled up by area."""
+    g, rp, src, w, _ = case
+    fset = rooms.room_faces(g, 16.0, bumped_every=3)
+    bumped, bn = rooms.patch_bumps(g, fset)
+    assert 0 < bumped.sum() < g.npatch
+    e0 = emission(g, seed=11, lit=60)
+    tr, st = open_stage(g)
+    with tr:
+        st.load_patches(bumped, bn)
+        st.load_transfers(rp, src, w)
+        r = st.bounce(e0, weights="f32", tol=1e-6, max_iter=300)
+        st.load_patches()
+        st.load_transfers(rp, src, w)
+        flat = st.bounce(e0, weights="f32", tol=1e-6, max_iter=300)
+    tb = np.zeros((g.npatch, 3, 3))
+    t32, _, _, n32 = ob.bounce(g, rp, src, w, e0, tol=1e-6, max_iter=300, bumped=bumped, bump_normals=bn,
+                               bump_out=tb)
+    print(f"[bumped] {int(bumped.sum())} of {g.npatch} patches bumped, {r.iterations} iterations", flush=True)
+    assert abs(r.iterations - n32) <= 1
+    assert r.bump_light is not None and flat.bump_light is None
+    assert rel_l1(r.total_light, t32) < 1e-5
+    assert rel_l1(r.bump_light, tb) < 1e-5
+    assert np.all(r.bump_light[bumped == 0] == 0)
+    # the flat slot of a bumped receiver divides the cosine out and back in:
+    # the same light as the flat gather, to rounding
+    assert rel_l1(r.total_light, flat.total_light) < 1e-5
+    assert np.any(r.bump_light[bumped == 1] > 0)
+
+
 @pytest.mark.parametrize("weights", ["f32", "f16"])
 def test_bounce_within_1e3_of_fp64(case, weights):
     g, rp, src, w, _ = case

@@ -104,8 +104,9 @@ def test_room_faces_match_oracle(rooms_case):
 
 
 def test_device_bounce_equals_the_same_light_from_the_host(rooms_case):
-    """The flat bounce's device-resident light gives the bytes the same light
-    handed in from the host gives; a bumped face owning patches is refused."""
+    """The bounce's device-resident light gives the bytes the same light
+    handed in from the host gives; a bumped face owning patches needs the
+    bounce's bump totals, and is refused without them."""
     from hl2sdk_tools_sharp_amd.bounce import HipBounceStage
 
     g, fset, lights, pvs, ff, fp = rooms_case
@@ -127,12 +128,25 @@ def test_device_bounce_equals_the_same_light_from_the_host(rooms_case):
         host = fin.run(bounce="host", patch_light=total)
         assert np.any(b.total_light > 0)
         assert dev.tobytes() == host.tobytes()
-        # bumped faces with patches: the flat light cannot stand in for theirs
+        # bumped faces with patches: the flat light cannot stand in for
+        # theirs, so a bounce loaded without bump normals is refused ...
         fl.load(fset, lights, light_pvs=pvs, sun_samples=1)
         fl.run()
         fin.load(ff, fp)
         with pytest.raises(NotSupportedError):
             fin.run(bounce="device")
+        # ... and one loaded with them hands over all four directions
+        bumped, bn = rooms.patch_bumps(g, fset)
+        st.load_patches(bumped, bn)
+        st.build_transfers()
+        b = st.bounce(g.pull_up(e.astype(np.float32)), tol=1e-3, max_iter=30)
+        assert b.bump_light is not None and np.any(b.bump_light > 0)
+        fin.load(ff, fp)
+        dev = fin.run(bounce="device")
+        total[:, 0] = b.total_light
+        total[:, 1:] = b.bump_light
+        host = fin.run(bounce="host", patch_light=total)
+        assert dev.tobytes() == host.tobytes()
 
 
 def test_stage_errors():

@@ -191,3 +191,59 @@ def test_reference_stop_rule_and_from_start_form(small_grid):
     c, res_c, _, ic = ob.bounce(g, rp, src, w, e0, tol=1e-6, max_iter=300, use_double=True, mode=1,
                                 start=a.astype(np.float32))
     assert ic <= 2
+
+
+def _two_patches(src_origin, src_normal):
+    from types import SimpleNamespace
+
+    g = SimpleNamespace(npatch=2, refl=np.ones((2, 3), np.float32), sky=np.zeros(2, np.uint8),
+                        child1=np.array([-1, -1], np.int32), child2=np.array([-1, -1], np.int32),
+                        area=np.ones(2, np.float32),
+                        origin=np.array([(0, 0, 0), src_origin], np.float32),
+                        normal=np.array([(0, 0, 1), src_normal], np.float32))
+    return g
+
+
+def test_a_bumped_receiver_gathers_along_each_normal_it_faces():
+    """GatherBumpedScalar (Rad/Bounce/Radiosity.cs:406-421): the cosine divided
+    out and re-applied per normal; a source straight above gives the flat
+    gather on the flat slot and cos(bump normal, z) on each bump slot."""
+    from hl2sdk_tools_sharp_amd.light import bump_basis
+
+    g = _two_patches((0, 0, 10), (0, 0, -1))
+    rp = np.array([0, 1, 1], np.int64)
+    src = np.array([1], np.int32)
+    w = np.array([0.5], np.float32)
+    e0 = np.array([(0, 0, 0), (2, 4, 6)], np.float32)
+    bn = np.zeros((2, 3, 3), np.float32)
+    bn[0] = bump_basis(np.array([0, 0, 1.0]))
+    tb = np.zeros((2, 3, 3))
+    flat, *_ = ob.bounce(g, rp, src, w, e0, max_iter=1, tol=1e-9)
+    tot, *_ = ob.bounce(g, rp, src, w, e0, max_iter=1, tol=1e-9, bumped=np.array([1, 0], np.uint8),
+                        bump_normals=bn, bump_out=tb)
+    assert list(tot[0]) == [1.0, 2.0, 3.0] == list(flat[0])
+    for k in range(3):
+        d = np.float32(bn[0, k, 2])
+        assert list(tb[0, k]) == [float(np.float32(v) * d) for v in (1, 2, 3)]
+    assert np.all(tb[1] == 0)
+
+
+def test_a_bump_normal_the_source_is_behind_gets_nothing():
+    from hl2sdk_tools_sharp_amd.light import bump_basis
+
+    bn = np.zeros((2, 3, 3), np.float32)
+    bn[0] = bump_basis(np.array([0, 0, 1.0]))
+    # a source low on the side away from the first bump normal's tilt
+    t = bn[0, 0] - np.array([0, 0, bn[0, 0, 2]], np.float32)
+    t /= np.linalg.norm(t)
+    so = (-10 * t + np.array([0, 0, 1.0])).astype(np.float32)
+    g = _two_patches(tuple(so), (0, 0, -1))
+    rp = np.array([0, 1, 1], np.int64)
+    tb = np.zeros((2, 3, 3))
+    tot, *_ = ob.bounce(g, rp, np.array([1], np.int32), np.array([0.5], np.float32),
+                        np.array([(0, 0, 0), (2, 4, 6)], np.float32), max_iter=1, tol=1e-9,
+                        bumped=np.array([1, 0], np.uint8), bump_normals=bn, bump_out=tb)
+    assert np.all(tb[0, 0] == 0)
+    assert np.all(tb[0, 1] > 0) and np.all(tb[0, 2] > 0)
+    # the flat slot: the cosine removed and put back, the flat gather within rounding
+    np.testing.assert_allclose(tot[0], [1, 2, 3], rtol=1e-6)
