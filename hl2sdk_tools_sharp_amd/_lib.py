@@ -130,6 +130,7 @@ SIGNATURES = {
     "vh_gather_load": (ctypes.c_int, [P, P]),
     "vh_gather_run": (ctypes.c_int, [P, I32, I32]),
     "vh_gather_read": (ctypes.c_int, [P, P, P, P]),
+    "vh_gather_supersample": (ctypes.c_int, [P, P, P]),
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),
     "vh_patch_lights": (ctypes.c_int, [P, I32, P, P]),
@@ -165,6 +166,7 @@ VH_KERNEL_FACELIGHT = 5
 VH_KERNEL_MLP = 6
 VH_KERNEL_FINAL = 7
 VH_KERNEL_AMBIENT = 8
+VH_KERNEL_SUPERSAMPLE = 9
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1

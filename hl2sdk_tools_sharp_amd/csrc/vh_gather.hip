@@ -106,6 +106,8 @@ struct GatherArgs {
     const float4 *nodes;
     const float4 *tris;
     unsigned long long *rays;  // rays traced, summed per wave
+    int32_t light_filter;      // 0 every light; 1 sky ambient only; 2 all but sky ambient
+    int32_t no_skip;           // ResolveResample: every light's record is added
 };
 
 __device__ __forceinline__ float maxps(float a, float b) { return a > b ? a : b; }
@@ -212,6 +214,10 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
 
     for (int32_t li = 0; li < a.nlights; li++) {
         const vh_light &L = a.lights[li];
+        // EmitResample (FaceLightJob.cs:474-482): ambient points see only sky
+        // ambient, direct points every other light
+        if (a.light_filter == 1 && L.type != VH_EMIT_SKYAMBIENT) continue;
+        if (a.light_filter == 2 && L.type == VH_EMIT_SKYAMBIENT) continue;
         bool inmask = real && pvs_check(a, li, cl);
         if (!group_any(inmask)) continue;
         float dot[4] = {0, 0, 0, 0};
@@ -357,7 +363,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
             fx[n] = masked * falloff;
             nz = nz || fx[n] != 0.0f;
         }
-        if (!group_any(nz)) continue;
+        if (!a.no_skip && !group_any(nz)) continue;
         if (real) {
             _Pragma("unroll") for (int n = 0; n < normals; n++) {
                 acc_l[n].x = acc_l[n].x + (fx[n] * L.intensity[0]);
@@ -491,7 +497,208 @@ GatherArgs gather_args(vh_ctx *c)
     a.nodes = (const float4 *)c->nodes.ptr;
     a.tris = (const float4 *)c->tris.ptr;
     a.rays = (unsigned long long *)G->ticket.ptr + 1;
+    a.light_filter = 0;
+    a.no_skip = 0;
     return a;
+}
+
+// ---------------------------------------------------------------------------
+// Supersampling (FaceLightJob's StyleSupersample passes, FaceLightJob.cs:698-1207)
+
+constexpr float kSupersampleGradient = 0.0625f;  // LightConstants.SupersampleGradient
+
+// ComputeLuxelIntensity: R + G + B to a linear perception space, pow in double
+__device__ __forceinline__ float ss_intensity(float4 v)
+{
+    float i = (v.x + v.y) + v.z;
+    return (float)pow((double)i / 256.0, 1.0 / 2.2);
+}
+
+// One wave per face: its luxel -> sample map, each sample's luxel and its
+// intensity per normal.
+__global__ __launch_bounds__(kWave) void k_ss_init(const int32_t *__restrict__ face_ptr,
+                                                   const int32_t *__restrict__ face_nc,
+                                                   const vh_final_face *__restrict__ frames,
+                                                   const int64_t *__restrict__ lux_ptr,
+                                                   const int32_t *__restrict__ st_in,
+                                                   int32_t *__restrict__ luxmap,
+                                                   int32_t *__restrict__ st,
+                                                   const float4 *__restrict__ light,
+                                                   float *__restrict__ inten, int32_t S)
+{
+    const int32_t f = blockIdx.x, lane = threadIdx.x;
+    const int32_t a = face_ptr[f], b = face_ptr[f + 1], nc = face_nc[f];
+    const int32_t w = frames[f].width, h = frames[f].height;
+    const int64_t base = lux_ptr[f];
+    for (int64_t j = lane; j < (int64_t)w * h; j += kWave) luxmap[base + j] = -1;
+    __syncthreads();
+    for (int32_t s = a + lane; s < b; s += kWave) {
+        int32_t ss, tt;
+        if (st_in) {
+            ss = st_in[2 * (size_t)s];
+            tt = st_in[2 * (size_t)s + 1];
+        } else {
+            ss = (s - a) % w;
+            tt = (s - a) / w;
+        }
+        st[2 * (size_t)s] = ss;
+        st[2 * (size_t)s + 1] = tt;
+        luxmap[base + ss + (int64_t)tt * w] = s;
+        for (int n = 0; n < nc; n++) inten[(size_t)n * S + s] = ss_intensity(light[(size_t)n * S + s]);
+    }
+}
+
+// ComputeGradients + SelectForPass, one wave per face: flags the selected
+// samples, marks them processed and counts them.
+__global__ __launch_bounds__(kWave) void k_ss_select(const int32_t *__restrict__ face_ptr,
+                                                     const int32_t *__restrict__ face_nc,
+                                                     const vh_final_face *__restrict__ frames,
+                                                     const int64_t *__restrict__ lux_ptr,
+                                                     const int32_t *__restrict__ luxmap,
+                                                     const int32_t *__restrict__ st,
+                                                     const float *__restrict__ inten, int32_t S,
+                                                     uint8_t *__restrict__ processed,
+                                                     uint8_t *__restrict__ flags,
+                                                     int32_t *__restrict__ counts, int32_t garbage)
+{
+    const int32_t f = blockIdx.x, lane = threadIdx.x;
+    const int32_t a = face_ptr[f], b = face_ptr[f + 1], nc = face_nc[f];
+    const int32_t w = frames[f].width, h = frames[f].height;
+    const int64_t base = lux_ptr[f];
+    // the neighbours in the order Step visits them
+    const int ds[8] = {-1, 0, 1, -1, 0, 1, -1, 1};
+    const int dt[8] = {-1, -1, -1, 1, 1, 1, 0, 0};
+    int32_t cnt = 0;
+    for (int32_t first = a; first < b; first += kWave) {
+        const int32_t s = first + lane;
+        bool sel = false;
+        if (s < b && !processed[s]) {
+            const int32_t ss = st[2 * (size_t)s], tt = st[2 * (size_t)s + 1];
+            float g = 0.0f;
+            for (int n = 0; n < nc; n++) {
+                const float c = inten[(size_t)n * S + s];
+                for (int k = 0; k < 8; k++) {
+                    const int32_t x = ss + ds[k], y = tt + dt[k];
+                    if (x < 0 || x >= w || y < 0 || y >= h) continue;
+                    const int32_t j = luxmap[base + x + (int64_t)y * w];
+                    if (j < 0) {
+                        if (garbage) g = __builtin_inff();
+                        continue;
+                    }
+                    const float v = fabsf(c - inten[(size_t)n * S + j]);
+                    g = g > v ? g : v;
+                }
+            }
+            sel = !(g < kSupersampleGradient);
+            if (sel) processed[s] = 1;
+        }
+        if (s < b) flags[s] = sel ? 1 : 0;
+        cnt += __popcll(__ballot(sel));
+    }
+    if (lane == 0) counts[f] = cnt;
+}
+
+// EmitSupersamplePoint's points, one wave per face: item i (the i-th selected
+// sample in sample order) owns direct points 16 i .. 16 i + 15 (row-major,
+// 4 x 4) and ambient points 4 i .. 4 i + 3.
+__global__ __launch_bounds__(kWave) void k_ss_points(const int32_t *__restrict__ face_ptr,
+                                                     const vh_final_face *__restrict__ frames,
+                                                     const uint8_t *__restrict__ flags,
+                                                     const int32_t *__restrict__ item_off,
+                                                     const float *__restrict__ pos,
+                                                     const int32_t *__restrict__ cluster,
+                                                     float *__restrict__ dpos, int32_t *__restrict__ dcl,
+                                                     float *__restrict__ apos, int32_t *__restrict__ acl,
+                                                     int32_t *__restrict__ sel_sample,
+                                                     int32_t *__restrict__ sel_face)
+{
+    const int32_t f = blockIdx.x, lane = threadIdx.x;
+    int32_t run = item_off[f];
+    if (item_off[f + 1] == run) return;
+    const int32_t a = face_ptr[f], b = face_ptr[f + 1];
+    const vh_final_face &F = frames[f];
+    const float a4[4] = {-0.375f, -0.125f, 0.125f, 0.375f};  // csshift + c * cscale, 4 wide
+    const float a2[2] = {-0.25f, 0.25f};                      // 2 wide
+    for (int32_t first = a; first < b; first += kWave) {
+        const int32_t s = first + lane;
+        const bool sel = s < b && flags[s];
+        const unsigned long long m = __ballot(sel);
+        if (sel) {
+            const int32_t item = run + __popcll(m & ((1ull << lane) - 1ull));
+            sel_sample[item] = s;
+            sel_face[item] = f;
+            const int32_t cl = cluster[s];
+            // WorldToLuxel of the sample
+            const float px = pos[3 * (size_t)s] - F.origin[0], py = pos[3 * (size_t)s + 1] - F.origin[1],
+                        pz = pos[3 * (size_t)s + 2] - F.origin[2];
+            const float os = (((px * F.w2l_s[0]) + (py * F.w2l_s[1])) + (pz * F.w2l_s[2])) - (float)F.min_s;
+            const float ot = (((px * F.w2l_t[0]) + (py * F.w2l_t[1])) + (pz * F.w2l_t[2])) - (float)F.min_t;
+            for (int p = 0; p < 20; p++) {
+                float cs, ct;
+                if (p < 16) {
+                    cs = os + a4[p >> 2];
+                    ct = ot + a4[p & 3];
+                } else {
+                    cs = os + a2[(p - 16) >> 1];
+                    ct = ot + a2[(p - 16) & 1];
+                }
+                // LuxelToWorld: the two multiply-adds in turn
+                cs = cs + (float)F.min_s;
+                ct = ct + (float)F.min_t;
+                float q[3];
+                for (int k = 0; k < 3; k++) {
+                    q[k] = F.origin[k] + (F.l2w_s[k] * cs);
+                    q[k] = q[k] + (F.l2w_t[k] * ct);
+                }
+                float *o = p < 16 ? dpos + 3 * ((size_t)16 * item + p) : apos + 3 * ((size_t)4 * item + (p - 16));
+                o[0] = q[0];
+                o[1] = q[1];
+                o[2] = q[2];
+                if (p < 16)
+                    dcl[(size_t)16 * item + p] = cl;
+                else
+                    acl[(size_t)4 * item + (p - 16)] = cl;
+            }
+        }
+        run += __popcll(m);
+    }
+}
+
+// ResolveSupersampleItem: the points summed in order, direct / 16 + ambient / 4,
+// written over the sample's light; its intensity recomputed.
+__global__ void k_ss_resolve(int32_t total, const int32_t *__restrict__ sel_sample,
+                             const int32_t *__restrict__ sel_face, const int32_t *__restrict__ face_nc,
+                             const float4 *__restrict__ dout, const float4 *__restrict__ aout,
+                             float4 *__restrict__ light, float *__restrict__ inten, int32_t S)
+{
+    const int32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= total) return;
+    const int32_t s = sel_sample[item], nc = face_nc[sel_face[item]];
+    const size_t nd = (size_t)16 * total, na = (size_t)4 * total;
+    for (int n = 0; n < nc; n++) {
+        float4 d = make_float4(0, 0, 0, 0), m = make_float4(0, 0, 0, 0);
+        for (int p = 0; p < 16; p++) {
+            const float4 x = dout[n * nd + (size_t)16 * item + p];
+            d.x = d.x + x.x;
+            d.y = d.y + x.y;
+            d.z = d.z + x.z;
+            d.w = d.w + x.w;
+        }
+        for (int p = 0; p < 4; p++) {
+            const float4 x = aout[n * na + (size_t)4 * item + p];
+            m.x = m.x + x.x;
+            m.y = m.y + x.y;
+            m.z = m.z + x.z;
+            m.w = m.w + x.w;
+        }
+        float4 v;
+        v.x = (0.0f + (0.0625f * d.x)) + (0.25f * m.x);
+        v.y = (0.0f + (0.0625f * d.y)) + (0.25f * m.y);
+        v.z = (0.0f + (0.0625f * d.z)) + (0.25f * m.z);
+        v.w = (0.0f + (0.0625f * d.w)) + (0.25f * m.w);
+        light[(size_t)n * S + s] = v;
+        inten[(size_t)n * S + s] = ss_intensity(v);
+    }
 }
 
 }  // namespace
@@ -659,6 +866,191 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
     c->stats.rays_gather += (int64_t)rays;
     c->stats.calls++;
     G->have_results = true;
+    return VH_OK;
+}
+
+int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass_counts)
+{
+    clear_error();
+    if (!c || !d) {
+        set_error("NULL context or descriptor");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather || !c->gather->have_results) {
+        set_error("no gather results: call vh_gather_run first");
+        return VH_ESTATE;
+    }
+    if (!c->have_scene) {
+        set_error("no scene loaded: the resample traces against vh_scene_load's casters");
+        return VH_ESTATE;
+    }
+    GatherData *G = c->gather;
+    const int32_t F = G->nfaces, S = G->nsamples;
+    if (d->which != 0 && d->which != 1) {
+        set_error("which is 0 (LDR) or 1 (HDR)");
+        return VH_EINVAL;
+    }
+    if (d->extra_passes < 0 || d->extra_passes > VH_SUPERSAMPLE_MAX_PASSES) {
+        set_error("extra_passes runs from 0 to " + std::to_string(VH_SUPERSAMPLE_MAX_PASSES));
+        return VH_EINVAL;
+    }
+    if (F > 0 && !d->faces) {
+        set_error("no face frames");
+        return VH_EINVAL;
+    }
+    if (pass_counts)
+        for (int32_t k = 0; k < d->extra_passes; k++) pass_counts[k] = 0;
+    // every sample on a luxel of its face's grid, one sample a luxel
+    std::vector<int64_t> lux_ptr((size_t)F + 1, 0);
+    for (int32_t f = 0; f < F; f++) {
+        const int32_t w = d->faces[f].width, h = d->faces[f].height;
+        const int32_t n = G->h_face_ptr[f + 1] - G->h_face_ptr[f];
+        if (w < 0 || h < 0 || (n > 0 && (w == 0 || h == 0))) {
+            set_error("face " + std::to_string(f) + " has samples but no luxel grid");
+            return VH_EINVAL;
+        }
+        if (!d->sample_st && (int64_t)n > (int64_t)w * h) {
+            set_error("face " + std::to_string(f) + " has more samples than luxels");
+            return VH_EINVAL;
+        }
+        lux_ptr[f + 1] = lux_ptr[f] + (int64_t)w * h;
+    }
+    if (d->sample_st) {
+        std::vector<uint8_t> used((size_t)lux_ptr[F], 0);
+        for (int32_t f = 0; f < F; f++) {
+            const int32_t w = d->faces[f].width, h = d->faces[f].height;
+            for (int32_t s = G->h_face_ptr[f]; s < G->h_face_ptr[f + 1]; s++) {
+                const int32_t x = d->sample_st[2 * (size_t)s], y = d->sample_st[2 * (size_t)s + 1];
+                if (x < 0 || x >= w || y < 0 || y >= h || used[lux_ptr[f] + x + (int64_t)y * w]) {
+                    set_error("sample " + std::to_string(s) + " is off its face's luxel grid or shares a luxel");
+                    return VH_EINVAL;
+                }
+                used[lux_ptr[f] + x + (int64_t)y * w] = 1;
+            }
+        }
+    }
+    if (S == 0 || d->extra_passes == 0) return VH_OK;
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    struct Bufs {
+        DevBuf frames, lux_ptr, st_in, luxmap, st, inten, processed, flags, counts, off, dptr, aptr, dcf, dcs,
+            acf, acs, dpos, dcl, apos, acl, sel_sample, sel_face, dl, dh, al, ah;
+        ~Bufs()
+        {
+            for (DevBuf *b : {&frames, &lux_ptr, &st_in, &luxmap, &st, &inten, &processed, &flags, &counts, &off,
+                              &dptr, &aptr, &dcf, &dcs, &acf, &acs, &dpos, &dcl, &apos, &acl, &sel_sample,
+                              &sel_face, &dl, &dh, &al, &ah})
+                b->release();
+        }
+    } B;
+    int rc;
+    if ((rc = upload(B.frames, d->faces, (size_t)F, "face frames", st)) ||
+        (rc = upload(B.lux_ptr, lux_ptr.data(), lux_ptr.size(), "luxel offsets", st)) ||
+        (d->sample_st && (rc = upload(B.st_in, d->sample_st, (size_t)S * 2, "sample luxels", st))) ||
+        (rc = B.luxmap.ensure((size_t)lux_ptr[F] * 4 + 16, "luxel map")) ||
+        (rc = B.st.ensure((size_t)S * 8 + 16, "sample luxels")) ||
+        (rc = B.inten.ensure((size_t)S * 16 + 16, "intensities")) ||
+        (rc = B.processed.ensure((size_t)S + 16, "processed flags")) ||
+        (rc = B.flags.ensure((size_t)S + 16, "selection flags")) ||
+        (rc = B.counts.ensure((size_t)F * 4 + 16, "selection counts")))
+        return rc;
+    float4 *light = (float4 *)(d->which ? G->out_hdr.ptr : G->out_ldr.ptr);
+    VH_HIP(hipMemsetAsync(B.processed.ptr, 0, (size_t)S, st));
+    VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 16, st));
+    hipLaunchKernelGGL(k_ss_init, dim3((uint32_t)F), dim3(kWave), 0, st, (const int32_t *)G->face_ptr.ptr,
+                       (const int32_t *)G->face_nc.ptr, (const vh_final_face *)B.frames.ptr,
+                       (const int64_t *)B.lux_ptr.ptr, (const int32_t *)(d->sample_st ? B.st_in.ptr : nullptr),
+                       (int32_t *)B.luxmap.ptr, (int32_t *)B.st.ptr, (const float4 *)light, (float *)B.inten.ptr, S);
+    VH_HIP(hipGetLastError());
+    const GatherArgs base = gather_args(c);
+    std::vector<int32_t> counts((size_t)F), off((size_t)F + 1), dptr((size_t)F + 1), aptr((size_t)F + 1);
+    std::vector<int32_t> dcf, dcs, acf, acs;
+    for (int32_t pass = 0; pass < d->extra_passes; pass++) {
+        {
+            KernelTimer timer(c, VH_KERNEL_SUPERSAMPLE, st);
+            hipLaunchKernelGGL(k_ss_select, dim3((uint32_t)F), dim3(kWave), 0, st,
+                               (const int32_t *)G->face_ptr.ptr, (const int32_t *)G->face_nc.ptr,
+                               (const vh_final_face *)B.frames.ptr, (const int64_t *)B.lux_ptr.ptr,
+                               (const int32_t *)B.luxmap.ptr, (const int32_t *)B.st.ptr, (const float *)B.inten.ptr,
+                               S, (uint8_t *)B.processed.ptr, (uint8_t *)B.flags.ptr, (int32_t *)B.counts.ptr,
+                               d->unset_is_garbage ? 1 : 0);
+            timer.done();
+        }
+        VH_HIP(hipGetLastError());
+        VH_HIP(hipMemcpyAsync(counts.data(), B.counts.ptr, (size_t)F * 4, hipMemcpyDeviceToHost, st));
+        VH_HIP(hipStreamSynchronize(st));
+        off[0] = 0;
+        for (int32_t f = 0; f < F; f++) off[f + 1] = off[f] + counts[f];
+        const int32_t total = off[F];
+        if (total == 0) break;
+        if ((int64_t)total * 16 > INT32_MAX) {
+            set_error("too many samples selected in one pass for 32-bit point indices");
+            return VH_EINVAL;
+        }
+        if (pass_counts) pass_counts[pass] = total;
+        // chunks of 64 points within one face, as the gather's own
+        dcf.clear(), dcs.clear(), acf.clear(), acs.clear();
+        for (int32_t f = 0; f <= F; f++) {
+            dptr[f] = 16 * off[f];
+            aptr[f] = 4 * off[f];
+        }
+        for (int32_t f = 0; f < F; f++) {
+            for (int32_t x = dptr[f]; x < dptr[f + 1]; x += 64) dcf.push_back(f), dcs.push_back(x);
+            for (int32_t x = aptr[f]; x < aptr[f + 1]; x += 64) acf.push_back(f), acs.push_back(x);
+        }
+        const size_t nd = (size_t)16 * total, na = (size_t)4 * total;
+        if ((rc = upload(B.off, off.data(), off.size(), "selection offsets", st)) ||
+            (rc = upload(B.dptr, dptr.data(), dptr.size(), "direct point ranges", st)) ||
+            (rc = upload(B.aptr, aptr.data(), aptr.size(), "ambient point ranges", st)) ||
+            (rc = upload(B.dcf, dcf.data(), dcf.size(), "direct chunks", st)) ||
+            (rc = upload(B.dcs, dcs.data(), dcs.size(), "direct chunks", st)) ||
+            (rc = upload(B.acf, acf.data(), acf.size(), "ambient chunks", st)) ||
+            (rc = upload(B.acs, acs.data(), acs.size(), "ambient chunks", st)) ||
+            (rc = B.dpos.ensure(nd * 12 + 16, "direct points")) || (rc = B.dcl.ensure(nd * 4 + 16, "direct points")) ||
+            (rc = B.apos.ensure(na * 12 + 16, "ambient points")) || (rc = B.acl.ensure(na * 4 + 16, "ambient points")) ||
+            (rc = B.sel_sample.ensure((size_t)total * 4 + 16, "selection")) ||
+            (rc = B.sel_face.ensure((size_t)total * 4 + 16, "selection")) ||
+            (rc = B.dl.ensure(nd * 64 + 16, "direct point light")) || (rc = B.dh.ensure(nd * 64 + 16, "direct point light")) ||
+            (rc = B.al.ensure(na * 64 + 16, "ambient point light")) || (rc = B.ah.ensure(na * 64 + 16, "ambient point light")))
+            return rc;
+        KernelTimer timer(c, VH_KERNEL_SUPERSAMPLE, st);
+        hipLaunchKernelGGL(k_ss_points, dim3((uint32_t)F), dim3(kWave), 0, st, (const int32_t *)G->face_ptr.ptr,
+                           (const vh_final_face *)B.frames.ptr, (const uint8_t *)B.flags.ptr,
+                           (const int32_t *)B.off.ptr, (const float *)G->pos.ptr, (const int32_t *)G->cluster.ptr,
+                           (float *)B.dpos.ptr, (int32_t *)B.dcl.ptr, (float *)B.apos.ptr, (int32_t *)B.acl.ptr,
+                           (int32_t *)B.sel_sample.ptr, (int32_t *)B.sel_face.ptr);
+        for (int kind = 0; kind < 2; kind++) {
+            GatherArgs a = base;
+            const bool direct = kind == 0;
+            a.face_ptr = (const int32_t *)(direct ? B.dptr.ptr : B.aptr.ptr);
+            a.pos = (const float *)(direct ? B.dpos.ptr : B.apos.ptr);
+            a.cluster = (const int32_t *)(direct ? B.dcl.ptr : B.acl.ptr);
+            a.nsamples = (int32_t)(direct ? nd : na);
+            a.chunk_face = (const int32_t *)(direct ? B.dcf.ptr : B.acf.ptr);
+            a.chunk_first = (const int32_t *)(direct ? B.dcs.ptr : B.acs.ptr);
+            a.nchunks = (int64_t)(direct ? dcf.size() : acf.size());
+            a.out_ldr = (float4 *)(direct ? B.dl.ptr : B.al.ptr);
+            a.out_hdr = (float4 *)(direct ? B.dh.ptr : B.ah.ptr);
+            a.out_ao = nullptr;
+            a.ao_count = 0;
+            a.light_filter = direct ? 2 : 1;
+            a.no_skip = 1;
+            if (a.nchunks > 0)
+                hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)a.nchunks), dim3(kWave), 0, st, a,
+                                   (int64_t)0, nullptr);
+        }
+        hipLaunchKernelGGL(k_ss_resolve, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, total,
+                           (const int32_t *)B.sel_sample.ptr, (const int32_t *)B.sel_face.ptr,
+                           (const int32_t *)G->face_nc.ptr, (const float4 *)(d->which ? B.dh.ptr : B.dl.ptr),
+                           (const float4 *)(d->which ? B.ah.ptr : B.al.ptr), light, (float *)B.inten.ptr, S);
+        timer.done();
+        VH_HIP(hipGetLastError());
+    }
+    unsigned long long rays = 0;
+    VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
+    VH_HIP(hipStreamSynchronize(st));
+    c->stats.rays_gather += (int64_t)rays;
     return VH_OK;
 }
 

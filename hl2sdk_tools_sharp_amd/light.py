@@ -217,6 +217,11 @@ class GatherResult:
     ao: np.ndarray  # (S,)
 
 
+class VhSupersampleDesc(ctypes.Structure):
+    _fields_ = [("faces", ctypes.c_void_p), ("sample_st", ctypes.c_void_p), ("which", ctypes.c_int32),
+                ("extra_passes", ctypes.c_int32), ("unset_is_garbage", ctypes.c_int32)]
+
+
 class HipFaceLighter:
     """The per-sample direct-light gather on the GPU, against the tracer's scene."""
 
@@ -246,6 +251,40 @@ class HipFaceLighter:
             self.tracer.handle, ldr.ctypes.data_as(ctypes.c_void_p), hdr.ctypes.data_as(ctypes.c_void_p),
             ao.ctypes.data_as(ctypes.c_void_p)))
         return GatherResult(ldr, hdr, ao)
+
+    def supersample(self, frames: np.ndarray, which: int = 0, extra_passes: int = 4, sample_st=None,
+                    unset_is_garbage: bool = False) -> list:
+        """FaceLightJob's supersampling passes (Rad/Light/FaceLightJob.cs:698-1207)
+        over the last run's LDR (``which`` 0) or HDR (1) light, refined in
+        place on the device.  ``frames``: each face's luxel frame and grid
+        (final.FACE_DTYPE); ``sample_st`` (S, 2) each sample's luxel, or None
+        for sample k of a face on (k % width, k // width).  Returns the number
+        of samples each pass re-lit."""
+        from .final import FACE_DTYPE
+
+        fr = np.ascontiguousarray(frames, FACE_DTYPE)
+        st = None if sample_st is None else np.ascontiguousarray(sample_st, np.int32).reshape(-1, 2)
+        d = VhSupersampleDesc(fr.ctypes.data, None if st is None else st.ctypes.data, int(which),
+                              int(extra_passes), 1 if unset_is_garbage else 0)
+        counts = np.zeros(max(int(extra_passes), 1), np.int32)
+        HipRayTracer._raise(_lib.lib().vh_gather_supersample(self.tracer.handle, ctypes.byref(d),
+                                                             counts.ctypes.data_as(ctypes.c_void_p)))
+        return [int(x) for x in counts[: int(extra_passes)] if x > 0]
+
+    def timed_supersample(self, frames, **kw) -> tuple:
+        """supersample() with its launches timed by HIP events on the
+        library's stream: (counts, kernel ms, rays traced)."""
+        L = _lib.lib()
+        h = self.tracer.handle
+        before = self.tracer.stats()["rays_gather"]
+        _lib.check(L.vh_timing_enable(h, 1))
+        try:
+            counts = self.supersample(frames, **kw)
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_SUPERSAMPLE, ctypes.byref(ms), ctypes.byref(n)))
+        finally:
+            _lib.check(L.vh_timing_enable(h, 0))
+        return counts, ms.value, self.tracer.stats()["rays_gather"] - before
 
     def timed_run(self, schedule: str = "queue", waves_per_cu: int = 0) -> tuple:
         """One run with the gather kernel timed by HIP events on the library's

@@ -310,6 +310,35 @@ int vh_gather_load(vh_ctx *ctx, const vh_gather_desc *desc);
  * waves_per_cu sizes the persistent launch (0 = a default). */
 int vh_gather_run(vh_ctx *ctx, int32_t schedule, int32_t waves_per_cu);
 
+/* Supersampling, FaceLightJob's passes after the gather (StyleSupersample,
+ * EmitSupersamplePoint / ResolveSupersampleItem,
+ * SourceSharp.MapTools/Rad/Light/FaceLightJob.cs:698-1207), over the last
+ * vh_gather_run's LDR (which = 0) or HDR (1) result, refined in place:
+ *   - per sample and normal, intensity = (float)pow((R + G + B) / 256.0, 1/2.2);
+ *   - per pass (1 .. extra_passes, while the last one selected any), every
+ *     sample not yet supersampled whose largest intensity step to its eight
+ *     neighbouring luxels (every normal) is not below 0.0625 is selected;
+ *   - a selected sample is re-lit at 4 x 4 points (every light but sky
+ *     ambient) and 2 x 2 points (sky ambient only) spread over its luxel
+ *     around WorldToLuxel(sample), taking its cluster, every light's record
+ *     added; its light becomes direct / 16 + ambient / 4.
+ * faces: the gather's faces' luxel frames (vh_final_face, only the frame and
+ * width x height are read); sample_st: per sample its luxel (s, t), or NULL
+ * when sample k of a face sits on luxel (k % width, k / width).
+ * unset_is_garbage 1 reads a neighbouring luxel without a sample as infinity
+ * (stock's uninitialised stack); 0 leaves it out (the Correct side).  Samples
+ * carry no partial-luxel winding: every point counts.  pass_counts (NULL or
+ * extra_passes entries) receives the number of samples each pass re-lit. */
+#define VH_SUPERSAMPLE_MAX_PASSES 64
+typedef struct vh_supersample_desc {
+    const struct vh_final_face *faces;
+    const int32_t *sample_st;
+    int32_t which;
+    int32_t extra_passes; /* DirectLightingSettings.ExtraPasses: 4 */
+    int32_t unset_is_garbage;
+} vh_supersample_desc;
+int vh_gather_supersample(vh_ctx *ctx, const vh_supersample_desc *desc, int32_t *pass_counts);
+
 /* Copies the last run's results: ldr/hdr 4 floats (RGB, direct sun amount)
  * per (normal, sample), normal-major over 4 normal slots; ao one float per
  * sample.  Any pointer may be NULL. */
@@ -561,7 +590,8 @@ int vh_mlp_predict(vh_ctx *ctx, const float *emit0, float *start);
 #define VH_KERNEL_MLP 6 /* the warm-start network's GEMMs (inference) or training steps */
 #define VH_KERNEL_FINAL 7 /* vh_final_light's launches, bracketed together */
 #define VH_KERNEL_AMBIENT 8 /* the leaf-ambient fan */
-#define VH_KERNEL_COUNT 9
+#define VH_KERNEL_SUPERSAMPLE 9 /* vh_gather_supersample's launches */
+#define VH_KERNEL_COUNT 10
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
 

@@ -46,11 +46,24 @@ def encode_rgbe(rgb: np.ndarray) -> np.ndarray:
     return out[:n]
 
 
-def gather(casters, desc_and_keep, nsamples, threads=None):
+def _desc_type(base):
+    class Desc(ctypes.Structure):
+        _fields_ = list(base._fields_) + [("light_filter", ctypes.c_int32), ("no_skip", ctypes.c_int32)]
+
+    return Desc
+
+
+def gather(casters, desc_and_keep, nsamples, threads=None, light_filter: int = 0, no_skip: bool = False):
     """Runs the oracle's gather over a descriptor built by
-    hl2sdk_tools_sharp_amd.light.pack_desc (the two structs share a layout).
+    hl2sdk_tools_sharp_amd.light.pack_desc (the oracle's struct is that layout
+    plus the resample's light filter and no-skip flag, set here).
     Returns (ldr (4,S,4), hdr (4,S,4), ao (S,))."""
-    d, _keep = desc_and_keep
+    base, _keep = desc_and_keep
+    d = _desc_type(type(base))()
+    for name, _ in type(base)._fields_:
+        setattr(d, name, getattr(base, name))
+    d.light_filter = int(light_filter)
+    d.no_skip = 1 if no_skip else 0
     tris = convert_triangles(casters.vertices, casters.ids)
     bvh = CpuBvh(casters.vertices, tris)
     ldr = np.zeros((4, nsamples, 4), np.float32)

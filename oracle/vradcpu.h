@@ -151,6 +151,13 @@ typedef struct vc_gather_desc {
     int32_t ao_count;
     float ao_radius;
     const float *ao_dirs; /* 3 x ao_count per face */
+    /* The supersampling resample (FaceLightJob.EmitResample/ResolveResample,
+     * FaceLightJob.cs:465-555): light_filter 1 keeps only the sky-ambient
+     * lights (the 2 x 2 ambient points), 2 every other light (the 4 x 4
+     * direct points), 0 all; no_skip adds every light's record, as
+     * ResolveResample does (no all-zero skip).  Zero for the primary gather. */
+    int32_t light_filter;
+    int32_t no_skip;
 } vc_gather_desc;
 
 void vc_directional_sampler(int32_t n, float *out);

@@ -232,6 +232,8 @@ static void gather_group(gjob *j, int64_t g)
     }
     for (int32_t li = 0; li < d->nlights; li++) {
         const vc_light *l = &d->lights[li];
+        if (d->light_filter == 1 && l->type != VC_EMIT_SKYAMBIENT) continue;
+        if (d->light_filter == 2 && l->type == VC_EMIT_SKYAMBIENT) continue;
         int mask = 0;
         for (int s = 0; s < LANES; s++)
             if (s < count && pvs_check(d, li, cl[s])) mask |= 1 << s;
@@ -379,7 +381,7 @@ static void gather_group(gjob *j, int64_t g)
                 fx[n][lane] = masked * falloff[lane];
                 skip &= fx[n][lane] == 0.0f;
             }
-        if (skip) continue;
+        if (skip && !d->no_skip) continue;
         for (int n = 0; n < normals; n++)
             for (int i = 0; i < count; i++) {
                 size_t o = ((size_t)n * d->nsamples + first + i) * 4;
