@@ -469,6 +469,19 @@ def final_leg(a, log):
         for _ in range(a.final_reps):
             cubes, ms = amb.timed_cubes(pts)
             ams.append(ms)
+        # supersampling (FaceLightJob's gradient passes) of an HDR gather of
+        # the same map with every light 10^4 times as bright: the map's own
+        # lights leave luxels well under 1 and every gradient under 0.0625;
+        # these put them where lightmaps sit (median R + G + B about 16,
+        # 90th percentile about 100)
+        bright = lights.copy()
+        for key in ("intensity", "intensity_hdr"):
+            bright[key] *= np.float32(1e4)
+        fl.load(faces, bright, light_pvs=pvs, sun_samples=1)
+        fl.run()
+        ss_base = fl.read().hdr
+        ss_counts, ss_ms, ss_rays = fl.timed_supersample(ff.faces, which=1)
+        ss_light = fl.read().hdr
     kernel_ms = float(np.mean(kms))
     nlux = int(n[lit].sum())
     out = {"lit_faces": int(lit.sum()), "luxels": nlux, "patches": g.npatch, "wave_item_steps": steps,
@@ -484,6 +497,11 @@ def final_leg(a, log):
                       "points_per_s": len(pts) / (amb_ms * 1e-3)}
     log(f"[bench] leaf ambient: {len(pts)} points x 162 rays in {amb_ms:.3f} ms "
         f"({out['ambient']['rays_per_s'] / 1e9:.2f} G rays/s)")
+    out["supersample"] = {"lights": "the map's 288 lights x 10^4", "samples": faces.nsamples,
+                          "passes": ss_counts, "kernel_ms": ss_ms, "rays": int(ss_rays),
+                          "rays_per_s": ss_rays / (ss_ms * 1e-3) if ss_ms > 0 else None,
+                          "samples_changed": int(np.count_nonzero(np.any(ss_light != ss_base, axis=(0, 2))))}
+    log(f"[bench] supersampling: passes re-lit {ss_counts} samples, {ss_rays} rays in {ss_ms:.3f} ms")
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
@@ -512,6 +530,25 @@ def final_leg(a, log):
         out["ambient"]["cubes_differing_cpu"] = int(np.sum(cubes[:k].view(np.uint32) != want.view(np.uint32)))
         log(f"[bench] leaf ambient CPU {k * 162 / dt / 1e6:.2f} Mrays/s on {threads} threads; "
             f"cube components differing: {out['ambient']['cubes_differing_cpu']}")
+        # supersampling works face by face, so the oracle over the leading
+        # faces gives those faces' answer on the whole map
+        import oracle.supersample as oss
+
+        nf = min(faces.nfaces, a.supersample_cpu_faces)
+        ns = int(faces.face_sample_ptr[nf])
+        sub = faces.subset(np.arange(nf))
+        t0 = time.perf_counter()
+        want, want_counts = oss.supersample(g.casters, sub, bright, pvs, 0.0, 1, None, ff.faces[:nf],
+                                            ss_base[:, :ns], which=1, threads=threads)
+        dt = time.perf_counter() - t0
+        out["supersample"]["cpu_baseline"] = {
+            "value": dt * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"the first {nf} of {faces.nfaces} faces ({ns} samples): the oracle's passes, "
+            "its gather over pthreads"}
+        out["supersample"]["values_differing_cpu"] = int(np.sum(
+            ss_light[:, :ns].view(np.uint32) != want.view(np.uint32)))
+        log(f"[bench] supersampling CPU {dt * 1e3:.0f} ms for {nf} faces (passes {want_counts}); "
+            f"values differing: {out['supersample']['values_differing_cpu']}")
     return out
 
 
@@ -610,6 +647,7 @@ def main(argv=None) -> int:
     ap.add_argument("--final-reps", type=int, default=5)
     ap.add_argument("--ambient-per-room", type=int, default=128)
     ap.add_argument("--ambient-cpu-points", type=int, default=2048)
+    ap.add_argument("--supersample-cpu-faces", type=int, default=20000)
     ap.add_argument("--mlp-maps", type=int, default=64)
     ap.add_argument("--mlp-test-maps", type=int, default=8)
     ap.add_argument("--mlp-epochs", type=int, default=8)
