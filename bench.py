@@ -573,13 +573,14 @@ def bake_leg(a, log, model):
     lights, pvs = rooms.room_lights(g, sun=False)
     fin = rooms.room_final_inputs(g, faces)
     log(f"[bench] bake: {g.npatch} patches, {faces.nsamples} luxels, {len(lights)} lights")
-    kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True, final_inputs=fin)
+    kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True, final_inputs=fin, extra_passes=4)
     light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
     plain = light_pass(g, faces, lights, pvs, schedule="per_face", weights="f32", **kw)
     fast = light_pass(g, faces, lights, pvs, schedule="queue", weights="f16", warm_start=model, **kw)
     out = {"patches": g.npatch, "luxels": faces.nsamples, "lights": len(lights)}
     for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
         out[name] = {"wall_s": r.wall_s, "iterations": r.bounce.iterations,
+                     "supersample_passes": r.supersample_passes,
                      "stage_s": {k: round(v, 5) for k, v in r.stage_s.items()}}
         log(f"[bench] bake {name}: {r.wall_s:.3f} s, {r.bounce.iterations} iterations: "
             + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in r.stage_s.items()))
@@ -609,6 +610,7 @@ def bake_leg(a, log, model):
             fl = HipFaceLighter(tr)
             fl.load(faces, lights, light_pvs=pvs, sun_samples=1)
             fl.run()
+            fl.supersample(fin[0].faces, which=1, extra_passes=kw["extra_passes"])  # as both forms did
             fr = HipFinalLighter(fl)
             fr.load(*fin)
             pl = np.zeros((g.npatch, 4, 3), np.float32)

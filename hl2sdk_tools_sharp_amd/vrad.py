@@ -22,6 +22,7 @@ import numpy as np
 
 from .bounce import BounceResult, HipBounceStage
 from .light import FaceSet, HipFaceLighter
+from .rooms import patch_bumps
 from .tracer import HipTracerFactory
 
 
@@ -35,6 +36,7 @@ class PassResult:
     bounce: Optional[BounceResult] = None
     lump: Optional[np.ndarray] = None
     transfers: int = 0
+    supersample_passes: list = field(default_factory=list)  # samples re-lit per supersampling pass
 
     @property
     def wall_s(self) -> float:
@@ -45,7 +47,7 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
                sun_extent: float = 0.0, sun_samples: int = 30, sky_dirs=None, ao_count: int = 0,
                ao_radius: float = 40.0, weights: str = "f32", tol: float = 1e-4, max_iter: int = 100,
                device: int = 0, read_direct: bool = False, schedule: str = "queue",
-               warm_start: Optional[dict] = None, final_inputs=None) -> PassResult:
+               warm_start: Optional[dict] = None, final_inputs=None, extra_passes: int = 0) -> PassResult:
     """Runs one pass (LDR, or HDR with ``hdr``) over room grid ``g`` whose
     faces, in the patch system's order, are ``faces``.  ``schedule`` is the
     gather's ('queue', or 'per_face' for one launch per face); ``warm_start``
@@ -53,8 +55,12 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
     the network's prediction instead of from zero.  ``final_inputs``
     ((FinalFaces, FinalPatches), e.g. rooms.room_final_inputs) makes the lump
     the final stage's: luxels filtered from the samples plus the bounced light
-    of the patches round them, minlight, medians.  Raises if the HIP tracer
-    declines (there is no CPU path)."""
+    of the patches round them, minlight, medians.  ``extra_passes`` > 0 runs
+    that many supersampling passes after the gather, before the patch
+    hand-off (FaceLightJob.Finish, Rad/Light/FaceLightJob.cs:935-960; the
+    reference's default is 4, ``-noextra`` 0); they need ``final_inputs`` for
+    the faces' luxel frames.  Raises if the HIP tracer declines (there is no
+    CPU path)."""
     out = PassResult()
     clock = time.perf_counter
 
@@ -70,11 +76,20 @@ def light_pass(g, faces: FaceSet, lights: np.ndarray, light_pvs=None, *, hdr: bo
         fl.load(faces, lights, light_pvs=light_pvs, sun_extent=sun_extent, sun_samples=sun_samples,
                 sky_dirs=sky_dirs, ao_count=ao_count, ao_radius=ao_radius)
         fl.run(schedule)
+        if extra_passes > 0:
+            if final_inputs is None:
+                raise ValueError("supersampling needs final_inputs for the faces' luxel frames")
+            out.supersample_passes = fl.supersample(final_inputs[0].faces, which=1 if hdr else 0,
+                                                    extra_passes=extra_passes)
         out.stage_s["facelight"] = clock() - t
 
         t = clock()
         st = HipBounceStage(tr, g)
-        st.load_patches()
+        # bumped receivers gather per bump normal (Patch.NeedsBumpmap)
+        if np.any(np.asarray(faces.face_normal_count) == 4):
+            st.load_patches(*patch_bumps(g, faces))
+        else:
+            st.load_patches()
         out.direct = st.patch_lights(fl, hdr=hdr, read=read_direct)
         out.stage_s["patch_lights"] = clock() - t
 

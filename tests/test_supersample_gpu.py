@@ -14,6 +14,7 @@ from hl2sdk_tools_sharp_amd.final import FACE_DTYPE
 from hl2sdk_tools_sharp_amd.light import FaceSet, HipFaceLighter
 from hl2sdk_tools_sharp_amd.scenes import box_triangles
 from hl2sdk_tools_sharp_amd.tracer import TRACE_ID_OPAQUE, ShadowCasterSet
+from hl2sdk_tools_sharp_amd.vrad import light_pass
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +126,26 @@ def test_supersample_errors():
         with pytest.raises(ValueError, match="luxel"):
             fl.supersample(fr, sample_st=bad)
         assert fl.supersample(fr, extra_passes=0, sample_st=st) == []
+
+
+def test_light_pass_supersamples_before_the_patch_handoff(rooms_case):
+    """FaceLightJob.Finish builds the patch lights from the supersampled
+    light: a pass with extra passes hands the patches what a gather plus
+    supersampling gives, after the oracle's passes."""
+    from hl2sdk_tools_sharp_amd.bounce import HipBounceStage
+
+    g, fset, lights, pvs, frames, sky, ldr, hdr = rooms_case
+    fin = rooms.room_final_inputs(g, fset)
+    r = light_pass(g, fset, lights, pvs, sun_samples=1, sky_dirs=sky, tol=1e-3, read_direct=True,
+                   final_inputs=fin, extra_passes=4)
+    _, counts = oss.supersample(g.casters, fset, lights, pvs, 0.0, 1, sky, frames, ldr, which=0)
+    assert r.supersample_passes == counts and counts
+    tr, fl = open_lighter(g.casters)
+    with tr:
+        fl.load(fset, lights, light_pvs=pvs, sun_samples=1, sky_dirs=sky)
+        fl.run()
+        fl.supersample(frames, which=0)
+        st = HipBounceStage(tr, g)
+        st.load_patches()
+        direct = st.patch_lights(fl, hdr=False, read=True)
+    assert same(r.direct, direct)
