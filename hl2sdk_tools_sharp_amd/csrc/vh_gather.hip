@@ -214,12 +214,13 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
 
     for (int32_t li = 0; li < a.nlights; li++) {
         const vh_light &L = a.lights[li];
-        // EmitResample (FaceLightJob.cs:474-482): ambient points see only sky
-        // ambient, direct points every other light
-        if (a.light_filter == 1 && L.type != VH_EMIT_SKYAMBIENT) continue;
-        if (a.light_filter == 2 && L.type == VH_EMIT_SKYAMBIENT) continue;
         bool inmask = real && pvs_check(a, li, cl);
         if (!group_any(inmask)) continue;
+        // EmitResample (FaceLightJob.cs:474-482): ambient points see only sky
+        // ambient, direct points every other light
+        if (a.light_filter != 0 &&
+            ((a.light_filter == 1) != (L.type == VH_EMIT_SKYAMBIENT)))
+            continue;
         float dot[4] = {0, 0, 0, 0};
         float falloff = 0.0f, sun = 0.0f;
         if (L.type == VH_EMIT_POINT || L.type == VH_EMIT_SURFACE || L.type == VH_EMIT_SPOTLIGHT) {
