@@ -37,6 +37,10 @@ struct GatherData {
     vh::DevBuf face_ptr, face_nc, face_normals, pos, cluster, lights, pvs, jitter, sky, ao_dirs;
     vh::DevBuf chunk_face, chunk_first;  // per chunk of 16 groups within one face
     vh::DevBuf out_ldr, out_hdr, out_ao, ticket;
+    // supersampling's working set, kept between calls (grown, never shrunk)
+    vh::DevBuf ss_frames, ss_lux_ptr, ss_st_in, ss_luxmap, ss_st, ss_inten, ss_processed, ss_flags, ss_counts,
+        ss_off, ss_dptr, ss_aptr, ss_dcf, ss_dcs, ss_acf, ss_acs, ss_dpos, ss_dcl, ss_apos, ss_acl, ss_sel_sample,
+        ss_sel_face, ss_dl, ss_dh, ss_al, ss_ah;
     bool have_results = false;
 };
 
@@ -48,7 +52,11 @@ void gather_release(vh_ctx *c)
     if (!G) return;
     for (DevBuf *b : {&G->face_ptr, &G->face_nc, &G->face_normals, &G->pos, &G->cluster,
                       &G->lights, &G->pvs, &G->jitter, &G->sky, &G->ao_dirs, &G->chunk_face,
-                      &G->chunk_first, &G->out_ldr, &G->out_hdr, &G->out_ao, &G->ticket})
+                      &G->chunk_first, &G->out_ldr, &G->out_hdr, &G->out_ao, &G->ticket, &G->ss_frames,
+                      &G->ss_lux_ptr, &G->ss_st_in, &G->ss_luxmap, &G->ss_st, &G->ss_inten, &G->ss_processed,
+                      &G->ss_flags, &G->ss_counts, &G->ss_off, &G->ss_dptr, &G->ss_aptr, &G->ss_dcf, &G->ss_dcs,
+                      &G->ss_acf, &G->ss_acs, &G->ss_dpos, &G->ss_dcl, &G->ss_apos, &G->ss_acl,
+                      &G->ss_sel_sample, &G->ss_sel_face, &G->ss_dl, &G->ss_dh, &G->ss_al, &G->ss_ah})
         b->release();
     delete G;
     c->gather = nullptr;
@@ -935,16 +943,12 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
     VH_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     struct Bufs {
-        DevBuf frames, lux_ptr, st_in, luxmap, st, inten, processed, flags, counts, off, dptr, aptr, dcf, dcs,
-            acf, acs, dpos, dcl, apos, acl, sel_sample, sel_face, dl, dh, al, ah;
-        ~Bufs()
-        {
-            for (DevBuf *b : {&frames, &lux_ptr, &st_in, &luxmap, &st, &inten, &processed, &flags, &counts, &off,
-                              &dptr, &aptr, &dcf, &dcs, &acf, &acs, &dpos, &dcl, &apos, &acl, &sel_sample,
-                              &sel_face, &dl, &dh, &al, &ah})
-                b->release();
-        }
-    } B;
+        DevBuf &frames, &lux_ptr, &st_in, &luxmap, &st, &inten, &processed, &flags, &counts, &off, &dptr, &aptr,
+            &dcf, &dcs, &acf, &acs, &dpos, &dcl, &apos, &acl, &sel_sample, &sel_face, &dl, &dh, &al, &ah;
+    } B{G->ss_frames, G->ss_lux_ptr, G->ss_st_in, G->ss_luxmap, G->ss_st, G->ss_inten, G->ss_processed,
+        G->ss_flags, G->ss_counts, G->ss_off, G->ss_dptr, G->ss_aptr, G->ss_dcf, G->ss_dcs, G->ss_acf,
+        G->ss_acs, G->ss_dpos, G->ss_dcl, G->ss_apos, G->ss_acl, G->ss_sel_sample, G->ss_sel_face, G->ss_dl,
+        G->ss_dh, G->ss_al, G->ss_ah};
     int rc;
     if ((rc = upload(B.frames, d->faces, (size_t)F, "face frames", st)) ||
         (rc = upload(B.lux_ptr, lux_ptr.data(), lux_ptr.size(), "luxel offsets", st)) ||
