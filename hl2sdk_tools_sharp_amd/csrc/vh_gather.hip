@@ -29,6 +29,7 @@
 
 struct GatherData {
     int32_t nfaces = 0, nsamples = 0, nlights = 0, pvs_bytes = 0;
+    int32_t n_sky_ambient = 0;  // lights of type sky ambient (the rest light the direct points)
     int32_t sun_samples = 1, sky_count = 0, ao_count = 0;
     float sun_extent = 0.0f, ao_radius = 0.0f;
     int64_t ngroups = 0, nchunks = 0;
@@ -675,10 +676,12 @@ __global__ __launch_bounds__(kWave) void k_ss_points(const int32_t *__restrict__
 
 // ResolveSupersampleItem: the points summed in order, direct / 16 + ambient / 4,
 // written over the sample's light; its intensity recomputed.
+// A kind with no light is not launched: its points would gather exactly +0,
+// so its sum is taken as +0 (have_d / have_a 0).
 __global__ void k_ss_resolve(int32_t total, const int32_t *__restrict__ sel_sample,
                              const int32_t *__restrict__ sel_face, const int32_t *__restrict__ face_nc,
-                             const float4 *__restrict__ dout, const float4 *__restrict__ aout,
-                             float4 *__restrict__ light, float *__restrict__ inten, int32_t S)
+                             const float4 *__restrict__ dout, const float4 *__restrict__ aout, int32_t have_d,
+                             int32_t have_a, float4 *__restrict__ light, float *__restrict__ inten, int32_t S)
 {
     const int32_t item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= total) return;
@@ -686,14 +689,14 @@ __global__ void k_ss_resolve(int32_t total, const int32_t *__restrict__ sel_samp
     const size_t nd = (size_t)16 * total, na = (size_t)4 * total;
     for (int n = 0; n < nc; n++) {
         float4 d = make_float4(0, 0, 0, 0), m = make_float4(0, 0, 0, 0);
-        for (int p = 0; p < 16; p++) {
+        for (int p = 0; p < (have_d ? 16 : 0); p++) {
             const float4 x = dout[n * nd + (size_t)16 * item + p];
             d.x = d.x + x.x;
             d.y = d.y + x.y;
             d.z = d.z + x.z;
             d.w = d.w + x.w;
         }
-        for (int p = 0; p < 4; p++) {
+        for (int p = 0; p < (have_a ? 4 : 0); p++) {
             const float4 x = aout[n * na + (size_t)4 * item + p];
             m.x = m.x + x.x;
             m.y = m.y + x.y;
@@ -776,6 +779,7 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
     G->nfaces = F;
     G->nsamples = S;
     G->nlights = NL;
+    for (int32_t l = 0; l < NL; l++) G->n_sky_ambient += d->lights[l].type == VH_EMIT_SKYAMBIENT ? 1 : 0;
     G->pvs_bytes = NL > 0 ? d->pvs_bytes : 0;
     G->sun_extent = d->sun_extent;
     G->sun_samples = ns;
@@ -1041,14 +1045,17 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
             a.ao_count = 0;
             a.light_filter = direct ? 2 : 1;
             a.no_skip = 1;
-            if (a.nchunks > 0)
+            const bool any = direct ? G->nlights > G->n_sky_ambient : G->n_sky_ambient > 0;
+            if (a.nchunks > 0 && any)
                 hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)a.nchunks), dim3(kWave), 0, st, a,
                                    (int64_t)0, nullptr);
         }
         hipLaunchKernelGGL(k_ss_resolve, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, total,
                            (const int32_t *)B.sel_sample.ptr, (const int32_t *)B.sel_face.ptr,
                            (const int32_t *)G->face_nc.ptr, (const float4 *)(d->which ? B.dh.ptr : B.dl.ptr),
-                           (const float4 *)(d->which ? B.ah.ptr : B.al.ptr), light, (float *)B.inten.ptr, S);
+                           (const float4 *)(d->which ? B.ah.ptr : B.al.ptr),
+                           G->nlights > G->n_sky_ambient ? 1 : 0, G->n_sky_ambient > 0 ? 1 : 0, light,
+                           (float *)B.inten.ptr, S);
         timer.done();
         VH_HIP(hipGetLastError());
     }
