@@ -41,7 +41,7 @@ struct GatherData {
     // supersampling's working set, kept between calls (grown, never shrunk)
     vh::DevBuf ss_frames, ss_lux_ptr, ss_st_in, ss_luxmap, ss_st, ss_inten, ss_processed, ss_flags, ss_counts,
         ss_off, ss_dptr, ss_aptr, ss_dcf, ss_dcs, ss_acf, ss_acs, ss_dpos, ss_dcl, ss_apos, ss_acl, ss_sel_sample,
-        ss_sel_face, ss_dl, ss_dh, ss_al, ss_ah;
+        ss_sel_face, ss_dl, ss_dh, ss_al, ss_ah, ss_sface;
     bool have_results = false;
 };
 
@@ -57,7 +57,8 @@ void gather_release(vh_ctx *c)
                       &G->ss_lux_ptr, &G->ss_st_in, &G->ss_luxmap, &G->ss_st, &G->ss_inten, &G->ss_processed,
                       &G->ss_flags, &G->ss_counts, &G->ss_off, &G->ss_dptr, &G->ss_aptr, &G->ss_dcf, &G->ss_dcs,
                       &G->ss_acf, &G->ss_acs, &G->ss_dpos, &G->ss_dcl, &G->ss_apos, &G->ss_acl,
-                      &G->ss_sel_sample, &G->ss_sel_face, &G->ss_dl, &G->ss_dh, &G->ss_al, &G->ss_ah})
+                      &G->ss_sel_sample, &G->ss_sel_face, &G->ss_dl, &G->ss_dh, &G->ss_al, &G->ss_ah,
+                      &G->ss_sface})
         b->release();
     delete G;
     c->gather = nullptr;
@@ -533,6 +534,7 @@ __global__ __launch_bounds__(kWave) void k_ss_init(const int32_t *__restrict__ f
                                                    const int32_t *__restrict__ st_in,
                                                    int32_t *__restrict__ luxmap,
                                                    int32_t *__restrict__ st,
+                                                   int32_t *__restrict__ sface,
                                                    const float4 *__restrict__ light,
                                                    float *__restrict__ inten, int32_t S)
 {
@@ -553,36 +555,32 @@ __global__ __launch_bounds__(kWave) void k_ss_init(const int32_t *__restrict__ f
         }
         st[2 * (size_t)s] = ss;
         st[2 * (size_t)s + 1] = tt;
+        sface[s] = f;
         luxmap[base + ss + (int64_t)tt * w] = s;
         for (int n = 0; n < nc; n++) inten[(size_t)n * S + s] = ss_intensity(light[(size_t)n * S + s]);
     }
 }
 
-// ComputeGradients + SelectForPass, one wave per face: flags the selected
-// samples, marks them processed and counts them.
-__global__ __launch_bounds__(kWave) void k_ss_select(const int32_t *__restrict__ face_ptr,
-                                                     const int32_t *__restrict__ face_nc,
-                                                     const vh_final_face *__restrict__ frames,
-                                                     const int64_t *__restrict__ lux_ptr,
-                                                     const int32_t *__restrict__ luxmap,
-                                                     const int32_t *__restrict__ st,
-                                                     const float *__restrict__ inten, int32_t S,
-                                                     uint8_t *__restrict__ processed,
-                                                     uint8_t *__restrict__ flags,
-                                                     int32_t *__restrict__ counts, int32_t garbage)
+// ComputeGradients + SelectForPass, a thread per sample: flags the selected
+// samples, marks them processed and counts them per face (counts zeroed
+// before the launch; a count does not depend on the order of the adds).
+__global__ void k_ss_select(const int32_t *__restrict__ sface, const int32_t *__restrict__ face_nc,
+                            const vh_final_face *__restrict__ frames, const int64_t *__restrict__ lux_ptr,
+                            const int32_t *__restrict__ luxmap, const int32_t *__restrict__ st,
+                            const float *__restrict__ inten, int32_t S, uint8_t *__restrict__ processed,
+                            uint8_t *__restrict__ flags, int32_t *__restrict__ counts, int32_t garbage)
 {
-    const int32_t f = blockIdx.x, lane = threadIdx.x;
-    const int32_t a = face_ptr[f], b = face_ptr[f + 1], nc = face_nc[f];
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int32_t f = sface[s], nc = face_nc[f];
     const int32_t w = frames[f].width, h = frames[f].height;
     const int64_t base = lux_ptr[f];
     // the neighbours in the order Step visits them
     const int ds[8] = {-1, 0, 1, -1, 0, 1, -1, 1};
     const int dt[8] = {-1, -1, -1, 1, 1, 1, 0, 0};
-    int32_t cnt = 0;
-    for (int32_t first = a; first < b; first += kWave) {
-        const int32_t s = first + lane;
+    {
         bool sel = false;
-        if (s < b && !processed[s]) {
+        if (!processed[s]) {
             const int32_t ss = st[2 * (size_t)s], tt = st[2 * (size_t)s + 1];
             float g = 0.0f;
             for (int n = 0; n < nc; n++) {
@@ -602,10 +600,9 @@ __global__ __launch_bounds__(kWave) void k_ss_select(const int32_t *__restrict__
             sel = !(g < kSupersampleGradient);
             if (sel) processed[s] = 1;
         }
-        if (s < b) flags[s] = sel ? 1 : 0;
-        cnt += __popcll(__ballot(sel));
+        flags[s] = sel ? 1 : 0;
+        if (sel) atomicAdd(&counts[f], 1);
     }
-    if (lane == 0) counts[f] = cnt;
 }
 
 // EmitSupersamplePoint's points, one wave per face: item i (the i-th selected
@@ -962,7 +959,8 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
         (rc = B.inten.ensure((size_t)S * 16 + 16, "intensities")) ||
         (rc = B.processed.ensure((size_t)S + 16, "processed flags")) ||
         (rc = B.flags.ensure((size_t)S + 16, "selection flags")) ||
-        (rc = B.counts.ensure((size_t)F * 4 + 16, "selection counts")))
+        (rc = B.counts.ensure((size_t)F * 4 + 16, "selection counts")) ||
+        (rc = G->ss_sface.ensure((size_t)S * 4 + 16, "sample faces")))
         return rc;
     float4 *light = (float4 *)(d->which ? G->out_hdr.ptr : G->out_ldr.ptr);
     VH_HIP(hipMemsetAsync(B.processed.ptr, 0, (size_t)S, st));
@@ -970,16 +968,18 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
     hipLaunchKernelGGL(k_ss_init, dim3((uint32_t)F), dim3(kWave), 0, st, (const int32_t *)G->face_ptr.ptr,
                        (const int32_t *)G->face_nc.ptr, (const vh_final_face *)B.frames.ptr,
                        (const int64_t *)B.lux_ptr.ptr, (const int32_t *)(d->sample_st ? B.st_in.ptr : nullptr),
-                       (int32_t *)B.luxmap.ptr, (int32_t *)B.st.ptr, (const float4 *)light, (float *)B.inten.ptr, S);
+                       (int32_t *)B.luxmap.ptr, (int32_t *)B.st.ptr, (int32_t *)G->ss_sface.ptr, (const float4 *)light,
+                       (float *)B.inten.ptr, S);
     VH_HIP(hipGetLastError());
     const GatherArgs base = gather_args(c);
     std::vector<int32_t> counts((size_t)F), off((size_t)F + 1), dptr((size_t)F + 1), aptr((size_t)F + 1);
     std::vector<int32_t> dcf, dcs, acf, acs;
     for (int32_t pass = 0; pass < d->extra_passes; pass++) {
         {
+            VH_HIP(hipMemsetAsync(B.counts.ptr, 0, (size_t)F * 4, st));
             KernelTimer timer(c, VH_KERNEL_SUPERSAMPLE, st);
-            hipLaunchKernelGGL(k_ss_select, dim3((uint32_t)F), dim3(kWave), 0, st,
-                               (const int32_t *)G->face_ptr.ptr, (const int32_t *)G->face_nc.ptr,
+            hipLaunchKernelGGL(k_ss_select, dim3((uint32_t)((S + 255) / 256)), dim3(256), 0, st,
+                               (const int32_t *)G->ss_sface.ptr, (const int32_t *)G->face_nc.ptr,
                                (const vh_final_face *)B.frames.ptr, (const int64_t *)B.lux_ptr.ptr,
                                (const int32_t *)B.luxmap.ptr, (const int32_t *)B.st.ptr, (const float *)B.inten.ptr,
                                S, (uint8_t *)B.processed.ptr, (uint8_t *)B.flags.ptr, (int32_t *)B.counts.ptr,
