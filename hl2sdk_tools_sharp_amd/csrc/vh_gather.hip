@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "vh_detmath.h"
 #include "vh_rgbe.h"
 #include "vh_traverse.h"
 
@@ -269,8 +270,9 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                         float mult = (dot2 - L.stopdot2) / (L.stopdot - L.stopdot2);
                         mult = minps(mult, 1.0f);
                         mult = maxps(mult, 0.0f);
+                        // DetMathF.Pow, correctly rounded
                         if (L.exponent != 0.0f && L.exponent != 1.0f)
-                            mult = (float)pow((double)mult, (double)L.exponent);
+                            mult = detmath::pow_f32(mult, L.exponent);
                         mult = fringe ? mult : 1.0f;
                         fo = mult * fo;
                     }
@@ -518,11 +520,12 @@ GatherArgs gather_args(vh_ctx *c)
 
 constexpr float kSupersampleGradient = 0.0625f;  // LightConstants.SupersampleGradient
 
-// ComputeLuxelIntensity: R + G + B to a linear perception space, pow in double
+// ComputeLuxelIntensity: R + G + B to a linear perception space,
+// DetMath.PowToSingle(I / 256.0, 1.0 / 2.2)
 __device__ __forceinline__ float ss_intensity(float4 v)
 {
     float i = (v.x + v.y) + v.z;
-    return (float)pow((double)i / 256.0, 1.0 / 2.2);
+    return detmath::pow_to_single((double)i / 256.0, 1.0 / 2.2);
 }
 
 // One wave per face: its luxel -> sample map, each sample's luxel and its

@@ -5,7 +5,8 @@ FaceLightJob's supersampling passes for one light style
 
 * ``StyleSupersample.ComputeLuxelIntensity`` (:1091-1104): per normal, the
   light's ``Intensity()`` (R + G + B, LightingValue.cs:49) taken to a linear
-  perception space, ``(float)pow(I / 256.0, 1.0 / 2.2)`` in double;
+  perception space, DetMath.PowToSingle(I / 256.0, 1.0 / 2.2): correctly
+  rounded to double, then to float (vradcpu_math.c);
 * ``ComputeGradients`` / ``Step`` (:1131-1207): the largest absolute step to
   the eight neighbouring luxels, every normal, in the reference's order; a
   neighbour without a sample is left out (Correct) or reads as infinity (the
@@ -50,7 +51,10 @@ def intensity(light: np.ndarray, normal_count: np.ndarray) -> np.ndarray:
     sample's normal count zero."""
     L = light.astype(np.float32)
     i = (L[..., 0] + L[..., 1]) + L[..., 2]
-    out = np.power(i.astype(np.float64) / 256.0, 1.0 / 2.2).astype(np.float32)
+    from .gather import pow_to_single
+
+    x = i.astype(np.float64) / 256.0
+    out = pow_to_single(x.reshape(-1), 1.0 / 2.2).reshape(x.shape)
     out[np.arange(4)[:, None] >= np.asarray(normal_count)[None, :]] = 0
     return out
 

@@ -158,13 +158,35 @@ typedef struct vc_gather_desc {
      * ResolveResample does (no all-zero skip).  Zero for the primary gather. */
     int32_t light_filter;
     int32_t no_skip;
+    /* EmitResample's style test (FaceLightJob.cs:486): per face, the light
+     * style the resample lights (-1 or NULL: every style). */
+    const int32_t *face_style_filter;
 } vc_gather_desc;
+
+/* Correctly rounded powers (vradcpu_math.c): DetMathF.Pow and
+ * DetMath.PowToSingle. */
+float vc_pow_f32(float x, float y);
+float vc_pow_to_single(double x, double y);
+int32_t vc_pow_needs_exact(int32_t kind, double x, double y);
+void vc_pow_f32_array(const float *x, const float *y, int64_t n, float *out);
+void vc_pow_to_single_array(const double *x, const double *y, int64_t n, float *out);
+int32_t vc_pow_scan(int32_t kind, double y, uint32_t first_bits, int64_t count, int32_t misrounded,
+                    uint32_t *found, int32_t max_found);
 
 void vc_directional_sampler(int32_t n, float *out);
 float vc_halton(int32_t base, int32_t index);
 void vc_encode_rgbe(const float *rgb, int64_t n, int32_t stride, uint8_t *out);
-void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
-               float *out_ao, int32_t threads);
+int64_t vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+                  float *out_ao, int32_t threads);
+/* The gather with light styles: out_ldr/out_hdr hold 4 style slots x 4
+ * normals x nsamples x 4 floats, face_styles 4 style numbers per face (255
+ * for an empty slot).  Returns the rays traced; *dropped counts the
+ * (face, light) records lost to a fifth style. */
+int64_t vc_gather_styles(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+                         float *out_ao, uint8_t *face_styles, int64_t *dropped, int32_t threads);
+/* Plants one mistake in the gather (test infrastructure for the tests that
+ * show each closed form catches it); 0 for none. */
+void vc_gather_set_mutation(int32_t which);
 
 /* The final light (vradcpu_final.c): FinalLightFace.Run with LuxelRadial,
  * one style.  Faces as FaceLightInfo frames; luxel_pos width * height points

@@ -120,13 +120,35 @@ void vc_encode_rgbe(const float *rgb, int64_t n, int32_t stride, uint8_t *out)
 
 /* ---- the gather ---- */
 
+/* Planted mistakes (vc_gather_set_mutation): each breaks one rule of the
+ * gather, so a test can show that its closed form catches that mistake. */
+enum {
+    MUT_SURFACE_EPS_SIGN = 1, /* the surface light's ray end moved into the emitter */
+    MUT_SPOT_NO_DOT2 = 2,     /* the spot falloff without its cone dot */
+    MUT_FRINGE_STOPDOT = 3,   /* the fringe's numerator with stopdot for stopdot2 */
+    MUT_BUMP_NOT_ZEROED = 4,  /* bump dots kept where the flat dot is zero */
+    MUT_TRACE_ZERO_DOT = 5,   /* a ray even when the dot is zero */
+    MUT_TRACE_PADDING = 6,    /* padding lanes traced too */
+    MUT_NO_POW = 7,           /* the spot exponent ignored */
+    MUT_HARD_END_START = 8,   /* the hard falloff cut at the start distance */
+    MUT_SKY_AMBIENT_OPEN = 9, /* sky ambient ignores occlusion */
+    MUT_NO_DIST_CLAMP = 10,   /* the distance not clamped at one unit */
+    MUT_SUN_FORWARD = 11,     /* the sun's rays cast along its direction, not back */
+    MUT_SUN_HALF_LENGTH = 12, /* the sun's rays half MaxTraceLength long */
+    MUT_STYLE_MERGED = 13,    /* every style's light added to slot 0 */
+};
+static int g_mut = 0;
+
+void vc_gather_set_mutation(int32_t which) { g_mut = which; }
+
 typedef struct {
     const vc_gather_desc *d;
     const vc_bvh *bvh;
     float *out_ldr, *out_hdr, *out_ao;
-    int64_t ngroups;
-    const int32_t *group_face, *group_first;
+    uint8_t *face_styles; /* NULL: one output slot */
+    int slots;
     int64_t next;
+    int64_t rays, dropped;
     pthread_mutex_t mu;
 } gjob;
 
@@ -147,7 +169,8 @@ static inline float inverse_quadratic(const vc_light *l, float dd)
     return 1.0f / f;
 }
 
-/* GatherStandardLane: returns 1 when the lane needs a ray (pos -> src). */
+/* GatherStandardLane (DirectLightGatherer.cs:1169-1306): returns 1 when the
+ * lane needs a ray (pos -> src). */
 static int standard_lane(const vc_light *l, const float *pos, const float *nrm, int normals,
                          int needed, float *dot_out /* [normals] */, float *falloff_out,
                          float *src_out)
@@ -163,8 +186,9 @@ static int standard_lane(const vc_light *l, const float *pos, const float *nrm, 
     float dot = dot3(delta, nrm);
     dot = maxps(0.0f, dot);
     int hard = l->end_fade > l->start_fade;
-    if (hard && !(dist <= l->end_fade)) return 0;
-    dist = maxps(dist, 1.0f);
+    float end = g_mut == MUT_HARD_END_START ? l->start_fade : l->end_fade;
+    if (hard && !(dist <= end)) return 0;
+    if (g_mut != MUT_NO_DIST_CLAMP) dist = maxps(dist, 1.0f);
     float eval = minps(dist, l->cap_dist);
     float falloff;
     if (l->type == VC_EMIT_POINT) {
@@ -174,19 +198,23 @@ static int standard_lane(const vc_light *l, const float *pos, const float *nrm, 
         dot2 = maxps(0.0f, dot2);
         if (dot == 0.0f) return 0;
         falloff = dot2 / dist2;
-        src[0] = src[0] + (l->normal[0] * DIST_EPSILON);
-        src[1] = src[1] + (l->normal[1] * DIST_EPSILON);
-        src[2] = src[2] + (l->normal[2] * DIST_EPSILON);
+        float eps = g_mut == MUT_SURFACE_EPS_SIGN ? -DIST_EPSILON : DIST_EPSILON;
+        src[0] = src[0] + (l->normal[0] * eps);
+        src[1] = src[1] + (l->normal[1] * eps);
+        src[2] = src[2] + (l->normal[2] * eps);
     } else if (l->type == VC_EMIT_SPOTLIGHT) {
         float dot2 = -dot3(delta, l->normal);
         if (!(dot2 > l->stopdot2)) return 0;
-        falloff = inverse_quadratic(l, eval) * dot2;
+        falloff = inverse_quadratic(l, eval);
+        if (g_mut != MUT_SPOT_NO_DOT2) falloff = falloff * dot2;
         int in_fringe = dot2 <= l->stopdot;
-        float mult = (dot2 - l->stopdot2) / (l->stopdot - l->stopdot2);
+        float lowc = g_mut == MUT_FRINGE_STOPDOT ? l->stopdot : l->stopdot2;
+        float mult = (dot2 - lowc) / (l->stopdot - l->stopdot2);
         mult = minps(mult, 1.0f);
         mult = maxps(mult, 0.0f);
-        if (l->exponent != 0.0f && l->exponent != 1.0f)
-            mult = (float)pow((double)mult, (double)l->exponent);
+        /* DetMathF.Pow, correctly rounded (vradcpu_math.c) */
+        if (l->exponent != 0.0f && l->exponent != 1.0f && g_mut != MUT_NO_POW)
+            mult = vc_pow_f32(mult, l->exponent);
         mult = in_fringe ? mult : 1.0f;
         falloff = mult * falloff;
     } else {
@@ -203,7 +231,7 @@ static int standard_lane(const vc_light *l, const float *pos, const float *nrm, 
         mult = t * mult;
         falloff = mult * falloff;
     }
-    int trace = needed && dot != 0.0f && falloff != 0.0f;
+    int trace = needed && (dot != 0.0f || g_mut == MUT_TRACE_ZERO_DOT) && falloff != 0.0f;
     dot_out[0] = dot;
     *falloff_out = falloff;
     for (int n = 1; n < normals; n++) dot_out[n] = maxps(0.0f, dot3(nrm + 3 * n, delta));
@@ -211,15 +239,32 @@ static int standard_lane(const vc_light *l, const float *pos, const float *nrm, 
     return trace;
 }
 
-static void gather_group(gjob *j, int64_t g)
+/* FaceLight.FindOrAllocateStyle (Rad/Light/FaceLight.cs:204-222): the first
+ * slot holding the style or the first empty one; -1 when all four are taken. */
+static int find_or_allocate_style(uint8_t *styles, int style)
+{
+    for (int k = 0; k < 4; k++) {
+        if (styles[k] == style) return k;
+        if (styles[k] == 255) {
+            styles[k] = (uint8_t)style;
+            return k;
+        }
+    }
+    return -1;
+}
+
+/* One SampleGroup: EmitGroup + ResolveGroup (FaceLightJob.cs:329-463) for
+ * the four samples of face f from `first`, adding into the face's style
+ * slots `styles` as lights first touch it. */
+static void gather_group(gjob *j, int32_t f, int32_t first, uint8_t *styles, int64_t *rays,
+                         int64_t *dropped)
 {
     const vc_gather_desc *d = j->d;
-    int32_t f = j->group_face[g];
-    int32_t first = j->group_first[g];
     int32_t fend = d->face_sample_ptr[f + 1];
     int count = fend - first < LANES ? fend - first : LANES;
     int normals = d->face_normal_count[f];
     const float *fn = d->face_normals + (size_t)f * 12;
+    int32_t want_style = d->face_style_filter ? d->face_style_filter[f] : -1;
     float p[LANES][3];
     int32_t cl[LANES];
     for (int i = 0; i < LANES; i++) {
@@ -234,6 +279,7 @@ static void gather_group(gjob *j, int64_t g)
         const vc_light *l = &d->lights[li];
         if (d->light_filter == 1 && l->type != VC_EMIT_SKYAMBIENT) continue;
         if (d->light_filter == 2 && l->type == VC_EMIT_SKYAMBIENT) continue;
+        if (want_style >= 0 && l->style != want_style) continue;
         int mask = 0;
         for (int s = 0; s < LANES; s++)
             if (s < count && pvs_check(d, li, cl[s])) mask |= 1 << s;
@@ -254,9 +300,12 @@ static void gather_group(gjob *j, int64_t g)
                     /* LightRayLog.MakeRay: the whole segment, reach 1 */
                     float dir[3] = {src[0] - p[lane][0], src[1] - p[lane][1], src[2] - p[lane][2]};
                     float vis = vc_bvh_blocked(j->bvh, p[lane], dir, 1.0f, 0.0f) ? 0.0f : 1.0f;
+                    ++*rays;
                     dot[0][lane] = vis * dot[0][lane];
+                    if (g_mut == MUT_TRACE_PADDING && lane == count - 1) *rays += LANES - count;
                 }
             }
+            /* padding lanes copy the last real lane and are not traced */
             for (int lane = count; lane < LANES; lane++) {
                 for (int n = 0; n < normals; n++) dot[n][lane] = dot[n][count - 1];
                 falloff[lane] = falloff[count - 1];
@@ -273,10 +322,11 @@ static void gather_group(gjob *j, int64_t g)
             if (!all_zero) {
                 int nsamples = d->sun_samples;
                 float jitter_scale = (float)(1.732050807569 * (2 * 16384) * (double)d->sun_extent);
+                float len = g_mut == MUT_SUN_FORWARD ? MAX_TRACE_LENGTH : -MAX_TRACE_LENGTH;
+                if (g_mut == MUT_SUN_HALF_LENGTH) len = -MAX_TRACE_LENGTH * 0.5f;
                 float total[LANES] = {0, 0, 0, 0};
                 for (int s = 0; s < nsamples; s++) {
-                    float delta[3] = {l->normal[0] * -MAX_TRACE_LENGTH, l->normal[1] * -MAX_TRACE_LENGTH,
-                                      l->normal[2] * -MAX_TRACE_LENGTH};
+                    float delta[3] = {l->normal[0] * len, l->normal[1] * len, l->normal[2] * len};
                     if (s != 0) {
                         const float *jt = d->sun_jitter + 3 * (size_t)(s - 1);
                         delta[0] = delta[0] + (jt[0] * jitter_scale);
@@ -288,6 +338,7 @@ static void gather_group(gjob *j, int64_t g)
                         float stop[3] = {delta[0] + p[lane][0], delta[1] + p[lane][1], delta[2] + p[lane][2]};
                         float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
                         occ[lane] = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
+                        ++*rays;
                     }
                     for (int lane = count; lane < LANES; lane++) occ[lane] = occ[count - 1];
                     for (int lane = 0; lane < LANES; lane++) {
@@ -343,7 +394,10 @@ static void gather_group(gjob *j, int64_t g)
                                          (an[1] * -MAX_TRACE_LENGTH) + p[lane][1],
                                          (an[2] * -MAX_TRACE_LENGTH) + p[lane][2]};
                         float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
-                        frac[lane] = 1.0f - (vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f);
+                        float occ = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
+                        ++*rays;
+                        if (g_mut == MUT_SKY_AMBIENT_OPEN) occ = 0.0f;
+                        frac[lane] = 1.0f - occ;
                     } else {
                         frac[lane] = 1.0f;
                     }
@@ -362,11 +416,12 @@ static void gather_group(gjob *j, int64_t g)
         } else {
             continue;
         }
-        /* Resolve: a light behind the face lights no bump direction */
+        /* Resolve (DirectLightGatherer.cs:441-479): a light behind the face
+         * lights no bump direction */
         for (int lane = 0; lane < LANES; lane++) {
             float d0 = maxps(dot[0][lane], 0.0f);
             dot[0][lane] = d0;
-            int nz = d0 > 0.0f;
+            int nz = d0 > 0.0f || g_mut == MUT_BUMP_NOT_ZEROED;
             for (int n = 1; n < normals; n++) {
                 float dn = maxps(dot[n][lane], 0.0f);
                 dot[n][lane] = nz ? dn : 0.0f;
@@ -382,9 +437,19 @@ static void gather_group(gjob *j, int64_t g)
                 skip &= fx[n][lane] == 0.0f;
             }
         if (skip && !d->no_skip) continue;
+        /* the light's style slot on this face; one-slot output takes every
+         * light the caller passed (a resample of one style) */
+        int slot = 0;
+        if (j->slots > 1 && g_mut != MUT_STYLE_MERGED) {
+            slot = find_or_allocate_style(styles, l->style);
+            if (slot < 0) {
+                ++*dropped;
+                continue;
+            }
+        }
         for (int n = 0; n < normals; n++)
             for (int i = 0; i < count; i++) {
-                size_t o = ((size_t)n * d->nsamples + first + i) * 4;
+                size_t o = (((size_t)slot * 4 + n) * d->nsamples + first + i) * 4;
                 float amt = fx[n][i];
                 for (int pass = 0; pass < 2; pass++) {
                     float *out = pass ? j->out_hdr : j->out_ldr;
@@ -406,46 +471,40 @@ static void gather_group(gjob *j, int64_t g)
                 const float *dir = d->ao_dirs + ((size_t)f * d->ao_count + a) * 3;
                 float seg[3] = {dir[0] * d->ao_radius, dir[1] * d->ao_radius, dir[2] * d->ao_radius};
                 if (!vc_bvh_blocked(j->bvh, p[i], seg, 1.0f, 0.0f)) unblocked++;
+                ++*rays;
             }
             j->out_ao[first + i] = (float)unblocked / (float)d->ao_count;
         }
     }
 }
 
+/* A face's groups run in order on one thread: style slots fill in the order
+ * lights first touch the face, group by group (FaceLightJob.cs:435). */
 static void *gather_worker(void *arg)
 {
     gjob *j = (gjob *)arg;
+    const vc_gather_desc *d = j->d;
+    int64_t rays = 0, dropped = 0;
     for (;;) {
         pthread_mutex_lock(&j->mu);
-        int64_t g = j->next;
-        j->next += 16;
+        int64_t f = j->next++;
         pthread_mutex_unlock(&j->mu);
-        if (g >= j->ngroups) break;
-        for (int64_t k = g; k < g + 16 && k < j->ngroups; k++) gather_group(j, k);
+        if (f >= d->nfaces) break;
+        uint8_t styles[4] = {0, 255, 255, 255}; /* style 0 always exists (FaceLightJob.cs:185) */
+        for (int32_t s = d->face_sample_ptr[f]; s < d->face_sample_ptr[f + 1]; s += LANES)
+            gather_group(j, (int32_t)f, s, styles, &rays, &dropped);
+        if (j->face_styles) memcpy(j->face_styles + 4 * f, styles, 4);
     }
+    pthread_mutex_lock(&j->mu);
+    j->rays += rays;
+    j->dropped += dropped;
+    pthread_mutex_unlock(&j->mu);
     return NULL;
 }
 
-/* The whole gather: every face's samples in groups of four.  out_ldr/out_hdr
- * hold 4 floats (RGB, direct sun amount) per (normal, sample), normal-major,
- * zeroed by the caller; out_ao one float per sample (may be NULL). */
-void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
-               float *out_ao, int32_t threads)
+static int64_t run_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+                          float *out_ao, uint8_t *face_styles, int64_t *dropped, int32_t threads)
 {
-    int64_t ng = 0;
-    for (int32_t f = 0; f < d->nfaces; f++) {
-        int32_t n = d->face_sample_ptr[f + 1] - d->face_sample_ptr[f];
-        ng += (n + LANES - 1) / LANES;
-    }
-    int32_t *gf = (int32_t *)malloc((size_t)(ng > 0 ? ng : 1) * 4);
-    int32_t *gs = (int32_t *)malloc((size_t)(ng > 0 ? ng : 1) * 4);
-    int64_t k = 0;
-    for (int32_t f = 0; f < d->nfaces; f++)
-        for (int32_t s = d->face_sample_ptr[f]; s < d->face_sample_ptr[f + 1]; s += LANES) {
-            gf[k] = f;
-            gs[k] = s;
-            k++;
-        }
     gjob j;
     memset(&j, 0, sizeof j);
     j.d = d;
@@ -453,9 +512,8 @@ void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float
     j.out_ldr = out_ldr;
     j.out_hdr = out_hdr;
     j.out_ao = out_ao;
-    j.ngroups = ng;
-    j.group_face = gf;
-    j.group_first = gs;
+    j.face_styles = face_styles;
+    j.slots = face_styles ? 4 : 1;
     pthread_mutex_init(&j.mu, NULL);
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
@@ -464,6 +522,24 @@ void vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float
     gather_worker(&j);
     for (int i = 1; i < threads; i++) pthread_join(th[i], NULL);
     pthread_mutex_destroy(&j.mu);
-    free(gf);
-    free(gs);
+    if (dropped) *dropped = j.dropped;
+    return j.rays;
+}
+
+/* The gather into one slot: every face's samples in groups of four.
+ * out_ldr/out_hdr hold 4 floats (RGB, direct sun amount) per (normal,
+ * sample), normal-major, zeroed by the caller; out_ao one float per sample
+ * (may be NULL).  Every light the descriptor passes adds to that one slot, so
+ * the caller passes one style's lights (or a style filter).  Returns the rays
+ * traced. */
+int64_t vc_gather(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+                  float *out_ao, int32_t threads)
+{
+    return run_gather(d, bvh, out_ldr, out_hdr, out_ao, NULL, NULL, threads);
+}
+
+int64_t vc_gather_styles(const vc_gather_desc *d, const vc_bvh *bvh, float *out_ldr, float *out_hdr,
+                         float *out_ao, uint8_t *face_styles, int64_t *dropped, int32_t threads)
+{
+    return run_gather(d, bvh, out_ldr, out_hdr, out_ao, face_styles, dropped, threads);
 }

@@ -102,9 +102,16 @@ def one_face(points, normal=(0, 0, 1), bumped=False):
 
 
 def run(faces, L, **kw):
-    d = light.pack_desc(faces, L, None, kw.get("sun_extent", 0.0), kw.get("sun_samples", 1),
+    d = light.pack_desc(faces, L, kw.get("pvs"), kw.get("sun_extent", 0.0), kw.get("sun_samples", 1),
                         kw.get("sky_dirs"), kw.get("ao_count", 0), kw.get("ao_radius", 40.0))
     return og.gather(kw.get("casters", far_scene()), d, faces.nsamples)
+
+
+def run_full(faces, L, **kw):
+    """The gather with style slots and the ray count."""
+    d = light.pack_desc(faces, L, kw.get("pvs"), kw.get("sun_extent", 0.0), kw.get("sun_samples", 1),
+                        kw.get("sky_dirs"), kw.get("ao_count", 0), kw.get("ao_radius", 40.0))
+    return og.gather_styles(kw.get("casters", far_scene()), d, faces.nsamples)
 
 
 def test_point_light_inverse_square_and_unit_dot():
@@ -186,3 +193,146 @@ def test_ao_open_and_closed():
     dz = light.ao_directions(np.array([[0, 0, 1]], np.float32), 32)[0, :, 2]
     assert 0 < np.count_nonzero(dz <= 0.125) < 32
     assert ao[0] == np.float32(np.count_nonzero(dz <= 0.125) / 32)
+
+
+# --- every GatherTests fact (src/SourceSharp.Tests/MapTools/Rad/Light/GatherTests.cs:101-312),
+# each with the planted mistake it must catch; the facts live in gather_cases
+# so the GPU tests hold the device to the same closed forms ---
+
+import math  # noqa: E402
+
+import gather_cases as gc  # noqa: E402
+
+
+def gc_run(faces, L, **kw):
+    return run_full(faces, L, **kw)
+
+
+FACTS = gc.FACTS
+
+
+@pytest.mark.parametrize("fact", sorted({f for f, _ in FACTS}, key=lambda f: f.__name__),
+                         ids=lambda f: f.__name__)
+def test_gather_fact(fact):
+    og.set_mutation(0)
+    assert fact(gc_run)
+
+
+@pytest.mark.parametrize("fact,mutation", [(f, m) for f, m in FACTS if m is not None],
+                         ids=lambda v: v.__name__ if callable(v) else str(v))
+def test_planted_mistake_breaks_its_fact(fact, mutation):
+    """The oracle with one planted mistake fails the fact that pins it."""
+    og.set_mutation(mutation)
+    try:
+        assert not fact(gc_run)
+    finally:
+        og.set_mutation(0)
+
+
+# --- DetMathF.Pow / DetMath.PowToSingle: the reference's mpmath table and hard arguments ---
+
+def test_pow_matches_the_reference_table_and_hard_rows():
+    rows = gc.pow_rows()
+    f2 = [r for r in rows if r["kind"] == "F2"]
+    s2 = [r for r in rows if r["kind"] == "S2"]
+    x = np.array([gc.f32(r["a"]) for r in f2], np.float32)
+    y = np.array([gc.f32(r["b"]) for r in f2], np.float32)
+    got = og.pow_f32(x, y).view(np.uint32)
+    assert got.tolist() == [r["expected"] for r in f2]
+    xd = np.array([gc.f64(r["a"]) for r in s2])
+    yd = np.array([gc.f64(r["b"]) for r in s2])
+    got = og.pow_to_single(xd, yd).view(np.uint32)
+    assert got.tolist() == [r["expected"] for r in s2]
+
+
+def test_hard_rows_take_the_exact_step_and_double_rounding_would_miss():
+    """The hard rows exercise the double-double step, and at least one of them
+    is one where (float)pow(double) is not the correctly rounded answer."""
+    hard = json_rows("hard")
+    assert hard and all(og.pow_needs_exact(0 if r["kind"] == "F2" else 1,
+                                           float(gc.f32(r["a"])) if r["kind"] == "F2" else gc.f64(r["a"]),
+                                           float(gc.f32(r["b"])) if r["kind"] == "F2" else gc.f64(r["b"]))
+                        for r in hard)
+    naive_wrong = 0
+    for r in hard:
+        if r["kind"] != "F2":
+            continue
+        v = np.float32(math.pow(float(gc.f32(r["a"])), float(gc.f32(r["b"]))))
+        naive_wrong += int(v.view(np.uint32)) != r["expected"]
+    assert naive_wrong >= 1
+
+
+def json_rows(part):
+    import json
+
+    with open(gc.GOLDEN) as fh:
+        return json.load(fh)[part]
+
+
+def test_pow_against_mpmath_on_random_arguments():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1, 300).astype(np.float32)
+    y = rng.uniform(0.1, 9, 300).astype(np.float32)
+    got = og.pow_f32(x, y)
+    for xi, yi, g in zip(x, y, got):
+        exact = mpmath.power(mpmath.mpf(float(xi)), mpmath.mpf(float(yi)))
+        lo = np.nextafter(g, np.float32(0))
+        hi = np.nextafter(g, np.float32(2))
+        # g is the nearest float: the exact value lies within half an ulp of it
+        assert abs(exact - float(g)) <= min(abs(exact - float(lo)), abs(exact - float(hi)))
+
+
+# --- light styles: FaceLight.FindOrAllocateStyle (Rad/Light/FaceLight.cs:204-222) ---
+
+def _styled(kind_origin_style):
+    L = light.make_lights(len(kind_origin_style))
+    for k, (origin, style) in enumerate(kind_origin_style):
+        L[k]["type"] = light.EMIT_POINT
+        L[k]["origin"] = origin
+        L[k]["constant_attn"] = 1.0
+        L[k]["intensity"] = (1 + k, 2 + k, 3 + k)
+        L[k]["intensity_hdr"] = (1 + k, 2 + k, 3 + k)
+        L[k]["style"] = style
+    return L
+
+
+def test_style_zero_is_always_slot_zero_and_other_styles_take_the_next_slots():
+    f = one_face([[0, 0, -1]])
+    L = _styled([((0, 0, 100), 2), ((5, 0, 100), 0)])
+    o = gc_run(f, L)
+    assert o.face_styles[0].tolist() == [0, 2, 255, 255]
+    one = gc_run(f, L[1:])
+    two = gc_run(f, L[:1])
+    assert np.array_equal(o.ldr[0, 0, 0], one.ldr[0, 0, 0])  # the style-0 light in slot 0
+    assert two.face_styles[0].tolist() == [0, 2, 255, 255] and not np.any(two.ldr[0])
+    assert np.array_equal(o.ldr[1, 0, 0], two.ldr[1, 0, 0])  # style 2 in slot 1, alone or not
+
+
+def test_style_slots_follow_the_first_touch_group_by_group():
+    """Light 0 (style 7) reaches only the second group's samples, light 1
+    (style 3) every sample: group 0 meets style 3 first, so slot 1 is style 3
+    and slot 2 style 7, whatever the light order."""
+    pts = [[x, 0, -1] for x in (0, 1, 2, 3)] + [[x, 0, -1] for x in (1000, 1001, 1002, 1003)]
+    f = one_face(pts)
+    L = _styled([((1000, 0, 30), 7), ((0, 0, 100), 3)])
+    L[0]["start_fade"], L[0]["end_fade"] = 0.0, 60.0  # only the far group is within 60 units
+    o = gc_run(f, L)
+    assert o.face_styles[0].tolist() == [0, 3, 7, 255]
+    assert not np.any(o.ldr[2, 0, :4, :3]) and np.all(o.ldr[2, 0, 4:, 0] > 0)
+    og.set_mutation(13)
+    try:
+        assert gc_run(f, L).face_styles[0].tolist() != [0, 3, 7, 255]
+    finally:
+        og.set_mutation(0)
+
+
+def test_a_fifth_style_is_dropped():
+    f = one_face([[0, 0, -1]])
+    L = _styled([((k, 0, 100), s) for k, s in enumerate([1, 2, 3, 4, 0])])
+    o = gc_run(f, L)
+    assert o.face_styles[0].tolist() == [0, 1, 2, 3]
+    assert o.dropped == 1
+    alone = gc_run(f, L[4:])
+    assert np.array_equal(o.ldr[0, 0, 0], alone.ldr[0, 0, 0])
