@@ -368,9 +368,11 @@ def room_grid(nx: int = 3, ny: int = 3, size=(512.0, 512.0, 256.0), wall: float 
 
 def bounce_grid(nx: int = 12, ny: int = 12, seed: int = 3) -> RoomGrid:
     """The bounce config: a 12 x 12 room grid of 64-unit patches, faces cut to
-    the patch size (so every patch is its own source), rooms sized so the map
-    has 200,000 +- 2 % patches, two point lights per room."""
-    return room_grid(nx, ny, size=BOUNCE_ROOM, patch=64.0, minchop=64.0, tile=64.0,
+    the patch size (so every patch is its own source), 1344 x 1344 x 384
+    rooms joined by 256 x 256 doorways, so the map has 200,000 +- 2 % patches
+    (198,528 at 12 x 12) and its receivers see about 1,500 sources each; two
+    point lights per room."""
+    return room_grid(nx, ny, size=BOUNCE_ROOM, door=BOUNCE_DOOR, patch=64.0, minchop=64.0, tile=64.0,
                      nlights=2 * nx * ny, seed=seed)
 
 
@@ -402,6 +404,7 @@ def direct_light(g: RoomGrid, blocked: np.ndarray) -> np.ndarray:
 
 
 BOUNCE_ROOM = (1344.0, 1344.0, 384.0)
+BOUNCE_DOOR = 256.0
 
 
 def room_faces(g: RoomGrid, luxel: float = 16.0, bumped_every: int = 0):
