@@ -119,6 +119,7 @@ SIGNATURES = {
     "vh_trace_visibility_dev": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, I32, P, P]),
     "vh_trace_closest_dev": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, P, P, P]),
     "vh_trace_point_lights_dev": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
+    "vh_trace_point_lights_noalloc": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
     "vh_sync": (ctypes.c_int, [P, P]),
     "vh_get_stats": (ctypes.c_int, [P, ctypes.POINTER(VhStats)]),
     "vh_patches_load": (ctypes.c_int, [P, ctypes.POINTER(VhPatchDesc)]),

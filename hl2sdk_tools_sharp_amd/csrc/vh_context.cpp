@@ -295,6 +295,9 @@ int vh_open(const vh_open_opts *opts, vh_ctx **out)
         return VH_DECLINED;
     }
     c->cu_count = prop.multiProcessorCount;
+    // the no-allocation entry point's working memory on this device; a
+    // failure here only disables that entry point (it then reports it)
+    if (noalloc_init(dev) != VH_OK) clear_error();
     c->identity = std::string("hip-bvh2-exact-1:") + prop.gcnArchName + ":" +
                   std::to_string(prop.multiProcessorCount) + "cu";
     if (o.fail_stage == 3) {
@@ -527,6 +530,36 @@ int vh_trace_point_lights_dev(vh_ctx *c, const float *d_points, int64_t npoints,
     VH_HIP(hipSetDevice(c->device));
     rc = launch_point_lights(c, d_points, npoints, d_lights, nlights, tmin, d_bits,
                              stream_of(c, stream));
+    if (!rc) {
+        c->stats.rays_visibility += npoints * (int64_t)nlights;
+        c->stats.calls++;
+    }
+    return rc;
+}
+
+int vh_trace_point_lights_noalloc(vh_ctx *c, const float *d_points, int64_t npoints,
+                                  const float *d_lights, int32_t nlights, float tmin,
+                                  uint64_t *d_bits, void *stream)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    Guard g(c);
+    int rc = check_rays_args(c, 7, npoints, tmin);
+    if (rc) return rc;
+    if (nlights < 0) {
+        set_error("negative light count");
+        return VH_EINVAL;
+    }
+    if (npoints > 0 && nlights > 0 && (!d_points || !d_lights || !d_bits)) {
+        set_error("NULL points, lights or bits");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    rc = launch_point_lights_noalloc(c, d_points, npoints, d_lights, nlights, tmin, d_bits,
+                                     stream_of(c, stream));
     if (!rc) {
         c->stats.rays_visibility += npoints * (int64_t)nlights;
         c->stats.calls++;

@@ -122,6 +122,13 @@ int launch_closest(vh_ctx *c, const float *d_rays, int32_t rw, float shared_reac
                    hipStream_t s);
 int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const float *d_lights,
                         int32_t nlights, float tmin, uint64_t *d_bits, hipStream_t s);
+// The no-allocation point-light path (vh_trace.hip): noalloc_init finds this
+// device's copy of the fixed working memory (once per device, with the device
+// current); the launch uses it and allocates nothing.
+int noalloc_init(int device);
+int launch_point_lights_noalloc(vh_ctx *c, const float *d_points, int64_t npoints,
+                                const float *d_lights, int32_t nlights, float tmin,
+                                uint64_t *d_bits, hipStream_t s);
 // Frees the patch/transfer/bounce buffers (vh_bounce.hip).
 void bounce_release(vh_ctx *c);
 // Frees the gather's buffers (vh_gather.hip).

@@ -3,14 +3,34 @@
 // The traversal itself (exact triangle test, LDS stack) is in vh_traverse.h.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "vh_traverse.h"
 
 #pragma clang fp contract(off)
 
 namespace vh {
+// The no-allocation point-light entry point's working memory: a counting sort
+// of up to kNoAllocMaxPoints points by the top 15 bits of their Morton key
+// (each point's bucket, the sorted point order, the bucket offsets).
+constexpr int64_t kNoAllocMaxPoints = VH_NOALLOC_MAX_POINTS;  // 2^21
+constexpr int kBucketBits = 15;
+constexpr int kBuckets = 1 << kBucketBits;
+constexpr size_t kScratchBytes = (size_t)kNoAllocMaxPoints * 8 + (size_t)kBuckets * 4;
+}  // namespace vh
+
+// One copy per GPU, held until the process ends.  Host code never names it
+// except in hipGetSymbolAddress: its device address (per device) is kept in
+// g_scratch_ptr and passed to the kernels.
+static __device__ __align__(256) unsigned char scratch[vh::kScratchBytes];
+
+namespace vh {
 namespace {
 
 constexpr int kBlock = 64;
+constexpr int kMaxScratchDevices = 64;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
 constexpr int kOccluders = 3;  // occluders a point-light lane remembers
 
 __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays, int rw, float shared_reach,
@@ -114,6 +134,79 @@ __global__ void k_morton(const float *__restrict__ pts, int64_t n, float lx, flo
     uint32_t qz = (uint32_t)fminf(fmaxf(z, 0.0f), 1023.0f);
     keys[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
     vals[i] = (int32_t)i;
+}
+
+// The no-allocation path's counting sort: each point's bucket (the top
+// kBucketBits of its Morton key) and the bucket sizes; an exclusive scan of
+// the sizes (one block); each point placed at its bucket's next slot.  The
+// order inside a bucket depends on atomic timing, which changes only which
+// lanes walk together: every point's hit bits are a function of that point.
+__global__ void k_fill_u32(uint32_t *__restrict__ p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ void k_morton_bucket(const float *__restrict__ pts, int64_t n, float lx, float ly, float lz,
+                                float sx, float sy, float sz, uint32_t *__restrict__ bucket,
+                                uint32_t *__restrict__ count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = (pts[i * 3 + 0] - lx) * sx, y = (pts[i * 3 + 1] - ly) * sy,
+          z = (pts[i * 3 + 2] - lz) * sz;
+    uint32_t qx = (uint32_t)fminf(fmaxf(x, 0.0f), 1023.0f);
+    uint32_t qy = (uint32_t)fminf(fmaxf(y, 0.0f), 1023.0f);
+    uint32_t qz = (uint32_t)fminf(fmaxf(z, 0.0f), 1023.0f);
+    uint32_t key = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    uint32_t b = key >> (30 - kBucketBits);
+    bucket[i] = b;
+    atomicAdd(&count[b], 1u);
+}
+
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_bucket_scan(uint32_t *__restrict__ count)
+{
+    __shared__ uint32_t part[kScanThreads];
+    constexpr int per = kBuckets / kScanThreads;
+    const int t = threadIdx.x;
+    uint32_t local[per];
+    uint32_t sum = 0;
+    for (int k = 0; k < per; k++) {
+        local[k] = count[t * per + k];
+        sum += local[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t base = part[t] - sum;
+    for (int k = 0; k < per; k++) {
+        count[t * per + k] = base;
+        base += local[k];
+    }
+}
+
+__global__ void k_bucket_scatter(const uint32_t *__restrict__ bucket, int64_t n,
+                                 uint32_t *__restrict__ next, int32_t *__restrict__ order)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pos = atomicAdd(&next[bucket[i]], 1u);
+    order[pos] = (int32_t)i;
 }
 
 // Adds a per-lane count into counters[slot] once per wave.
@@ -303,6 +396,70 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
                            (const int32_t *)c->perm_b.ptr, npoints, d_lights, nlights, tmin,
                            d_bits, nullptr, depth);
     timer.done();
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+int noalloc_init(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) {
+        set_error("device index " + std::to_string(device) + " is past the no-allocation table");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return VH_OK;
+    void *p = nullptr;
+    VH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+    return VH_OK;
+}
+
+int launch_point_lights_noalloc(vh_ctx *c, const float *d_points, int64_t npoints,
+                                const float *d_lights, int32_t nlights, float tmin,
+                                uint64_t *d_bits, hipStream_t s)
+{
+    if (npoints > kNoAllocMaxPoints) {
+        set_error("vh_trace_point_lights_noalloc takes at most " + std::to_string(kNoAllocMaxPoints) +
+                  " points per call (its working memory is fixed); " + std::to_string(npoints) +
+                  " were given: split the batch or call vh_trace_point_lights_dev");
+        return VH_EINVAL;
+    }
+    if (npoints <= 0 || nlights <= 0) return VH_OK;
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxScratchDevices) base = g_scratch_ptr[c->device];
+    }
+    if (!base) {
+        set_error("the no-allocation working memory of this device was not set up at vh_open");
+        return VH_ESTATE;
+    }
+    uint32_t *bucket = (uint32_t *)base;
+    int32_t *order = (int32_t *)((char *)base + (size_t)kNoAllocMaxPoints * 4);
+    uint32_t *next = (uint32_t *)((char *)base + (size_t)kNoAllocMaxPoints * 8);
+    float sc[3];
+    for (int k = 0; k < 3; k++) {
+        float ext = c->hi[k] - c->lo[k];
+        sc[k] = ext > 0 ? 1024.0f / ext : 0.0f;
+    }
+    const uint32_t pblocks = (uint32_t)((npoints + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(kBuckets / 256), dim3(256), 0, s, next, (int64_t)kBuckets, 0u);
+    hipLaunchKernelGGL(k_morton_bucket, dim3(pblocks), dim3(256), 0, s, d_points, npoints, c->lo[0],
+                       c->lo[1], c->lo[2], sc[0], sc[1], sc[2], bucket, next);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kScanThreads), 0, s, next);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(pblocks), dim3(256), 0, s, bucket, npoints, next, order);
+    VH_HIP(hipGetLastError());
+    if ((nlights & 63) != 0) {
+        int64_t words = (npoints * (int64_t)nlights + 63) / 64;
+        uint32_t wb = (uint32_t)std::min<int64_t>((words + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_u64, dim3(wb), dim3(256), 0, s, (unsigned long long *)d_bits, words);
+    }
+    int32_t depth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;
+    size_t lds = (size_t)(kBlock / 64) * 3 * depth * sizeof(int);
+    hipLaunchKernelGGL(k_point_lights<false>, dim3((uint32_t)((npoints + kBlock - 1) / kBlock)),
+                       dim3(kBlock), lds, s, (const float4 *)c->nodes.ptr, (const float4 *)c->tris.ptr,
+                       d_points, (const int32_t *)order, npoints, d_lights, nlights, tmin, d_bits,
+                       nullptr, depth);
     VH_HIP(hipGetLastError());
     return VH_OK;
 }

@@ -133,6 +133,21 @@ int vh_trace_point_lights_dev(vh_ctx *ctx, const float *d_points, int64_t npoint
                               const float *d_lights, int32_t nlights, float min_distance,
                               uint64_t *d_bits, void *stream);
 
+/* vh_trace_point_lights_dev without any allocation.  No counterpart in the
+ * reference: this is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same rays and the same hit bits as
+ * vh_trace_point_lights_dev; its working memory (the points' spatial order)
+ * is a fixed array in the library, 16.1 MiB on each GPU the library runs on,
+ * held until the process ends.  At most VH_NOALLOC_MAX_POINTS points per
+ * call (VH_EINVAL above that).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race (use one stream, or wait between them).
+ * Clears what it needs itself with fill kernels (no memset). */
+#define VH_NOALLOC_MAX_POINTS 2097152
+int vh_trace_point_lights_noalloc(vh_ctx *ctx, const float *d_points, int64_t npoints,
+                                  const float *d_lights, int32_t nlights, float min_distance,
+                                  uint64_t *d_bits, void *stream);
+
 /* Waits for the work enqueued on `stream` (NULL = the context's stream). */
 int vh_sync(vh_ctx *ctx, void *stream);
 
