@@ -131,6 +131,8 @@ SIGNATURES = {
     "vh_gather_load": (ctypes.c_int, [P, P]),
     "vh_gather_run": (ctypes.c_int, [P, I32, I32]),
     "vh_gather_read": (ctypes.c_int, [P, P, P, P]),
+    "vh_gather_styles": (ctypes.c_int, [P, P, ctypes.POINTER(I32), ctypes.POINTER(I64)]),
+    "vh_gather_read_slot": (ctypes.c_int, [P, I32, P, P]),
     "vh_gather_supersample": (ctypes.c_int, [P, P, P]),
     "vh_encode_rgbe": (ctypes.c_int, [P, P, I64, I32, P]),
     "vh_lightmap_encode": (ctypes.c_int, [P, I32, P, I64]),

@@ -58,9 +58,16 @@ struct FaceDev {
     int32_t lit, nb;   // nb: 1, or 4 for a bumped face
     int32_t s0, s1;    // the face's gather samples
     int32_t pad;
-    int64_t lux_base;  // first luxel in luxel_pos / mvals
-    int64_t cell_base; // first cell of the face's grids (nb * w * h float4)
-    int64_t light_ofs; // the lump offset of luxel 0 (-1 when not lit)
+    int64_t lux_base;  // first luxel in luxel_pos
+};
+
+// One light style of one lit face (FinalLightFace.Run's loop over styles).
+struct UnitDev {
+    int32_t face, slot, style, pad;
+    int64_t cell_base;   // first cell of the unit's grids (nb * w * h float4)
+    int64_t mval_base;   // first of its w * h median candidates
+    int64_t lump_ofs;    // lump offset of its luxel 0
+    int64_t median_ofs;  // lump offset of its median word
 };
 
 struct FinalArgs {
@@ -74,10 +81,12 @@ struct FinalArgs {
     const float *wpts;
     const float4 *plight;       // 4 per patch
     const float *spos;          // gather sample positions, 3 each
-    const float4 *lv;           // gather light, [normal][sample]
+    const float4 *lv;           // gather light, [style slot][normal][sample]
+    const uint8_t *fstyles;     // the gather's 4 style numbers per face
     int32_t nsamples;
-    const int2 *tiles;          // (face, first cell)
-    const int32_t *lit_faces;
+    const UnitDev *units;
+    const int2 *tiles;          // (unit, first cell)
+    const int32_t *lit_units;
     float4 *cells, *pcells;
     float4 *mvals;
     uint32_t *lump;
@@ -134,9 +143,20 @@ __device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v)
 // projects the item and loads its light, and the items whose window meets
 // the wave's cells are applied in order from registers, so no memory access
 // waits inside the per-item loop.
-template <bool BOUNCED, int NB>
-__device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
+// The slot of face f that holds light style `style`, or -1 (a neighbour's
+// samples are matched by the style NUMBER, FinalLightFace.cs:340-360).
+__device__ __forceinline__ int32_t slot_of_style(const FinalArgs &a, int32_t f, int32_t style)
 {
+    for (int k = 0; k < 4; k++)
+        if ((int32_t)a.fstyles[(size_t)f * 4 + k] == style) return k;
+    return -1;
+}
+
+template <bool BOUNCED, int NB>
+__device__ void radial_tile(const FinalArgs &a, int32_t unit, int32_t c0)
+{
+    const UnitDev U = a.units[unit];
+    const int32_t face = U.face;
     const FaceDev F = a.faces[face];
     const int lane = threadIdx.x;
     const int32_t w = F.w, ncell = F.w * F.h;
@@ -154,14 +174,15 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
     const int32_t n0 = a.nbr_ptr[face], nseg = a.nbr_ptr[face + 1] - n0 + 1;
     for (int32_t q0 = 0; q0 < nseg; q0 += kWave) {
         const int32_t q = q0 + lane;
-        int32_t src = face, first = 0, cnt = 0, sbump = 0;
+        int32_t src = face, first = 0, cnt = 0, sbump = 0, sslot = U.slot;
         if (q < nseg) {
             src = q == 0 ? face : a.nbr[n0 + q - 1];
             if (!BOUNCED) {
                 // a neighbour without the style or without samples adds nothing
                 const int32_t lit = a.faces[src].lit, nbs = a.faces[src].nb;
+                if (q != 0) sslot = slot_of_style(a, src, U.style);
                 first = a.faces[src].s0;
-                cnt = (q == 0 || lit) ? a.faces[src].s1 - first : 0;
+                cnt = (q == 0 || (lit && sslot >= 0)) ? a.faces[src].s1 - first : 0;
                 sbump = nbs == 4;
             } else {
                 first = a.plist_ptr[src];
@@ -182,6 +203,7 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
             const int32_t ifirst = __shfl(first, seg, kWave);
             const int32_t iexcl = __shfl(excl, seg, kWave);
             const int32_t ibump = __shfl(sbump, seg, kWave);
+            const int32_t islot = __shfl(sslot, seg, kWave);
             const bool mine = g < total;
             float cs = 0, ct = 0, pa = 0, pb = 0, pc = 0, pd = 0;
             float lt[NB][3];
@@ -216,7 +238,7 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
                         if (b == 0 || ibump) {
-                            float4 l = a.lv[(size_t)b * a.nsamples + item];
+                            float4 l = a.lv[((size_t)islot * 4 + b) * a.nsamples + item];
                             lt[b][0] = l.x; lt[b][1] = l.y; lt[b][2] = l.z;
                         }
                     }
@@ -328,7 +350,7 @@ __device__ void radial_tile(const FinalArgs &a, int32_t face, int32_t c0)
         }
     }
     if (valid) {
-        float4 *out = (BOUNCED ? a.pcells : a.cells) + F.cell_base;
+        float4 *out = (BOUNCED ? a.pcells : a.cells) + U.cell_base;
 #pragma unroll
         for (int b = 0; b < NB; b++)
             out[(size_t)b * ncell + c] = make_float4(L[b][0], L[b][1], L[b][2], W);
@@ -339,7 +361,7 @@ template <bool BOUNCED>
 __global__ __launch_bounds__(kWave) void k_radial(FinalArgs a)
 {
     const int2 tile = a.tiles[blockIdx.x];
-    if (a.faces[tile.x].nb == 4)
+    if (a.faces[a.units[tile.x].face].nb == 4)
         radial_tile<BOUNCED, 4>(a, tile.x, tile.y);
     else
         radial_tile<BOUNCED, 1>(a, tile.x, tile.y);
@@ -382,15 +404,18 @@ __device__ __forceinline__ bool sample_grid(const float4 *cells, int32_t ncell, 
 }
 
 template <int NB>
-__device__ void final_face(const FinalArgs &a, int32_t face)
+__device__ void final_face(const FinalArgs &a, int32_t unit)
 {
-    const FaceDev F = a.faces[face];
+    const UnitDev U = a.units[unit];
+    const FaceDev F = a.faces[U.face];
     const int lane = threadIdx.x;
     const int32_t w = F.w, h = F.h, n = F.w * F.h;
-    const float4 *cells = a.cells + F.cell_base;
-    const float4 *pcells = a.pcells + F.cell_base;
-    uint32_t *out = a.lump + (F.light_ofs >> 2);
-    float4 *mv = a.mvals + F.lux_base;
+    const float4 *cells = a.cells + U.cell_base;
+    const float4 *pcells = a.pcells + U.cell_base;
+    uint32_t *out = a.lump + (U.lump_ofs >> 2);
+    float4 *mv = a.mvals + U.mval_base;
+    // bounced light goes into style 0 only (FinalLightFace.cs:182)
+    const bool bounce = a.bounce && U.slot == 0;
     const bool no_samples = F.s1 == F.s0;
     const float ml = F.minlight;
     int32_t count = 0;
@@ -411,7 +436,7 @@ __device__ void final_face(const FinalArgs &a, int32_t face)
             int32_t i = off ? 0 : u + v * w;
             float lb[NB][3];
             ok = sample_grid<NB>(cells, n, off, i, a.red_errors, lb);
-            if (a.bounce) {
+            if (bounce) {
                 float pv[NB][3];
                 sample_grid<NB>(pcells, n, off, i, a.red_errors, pv);
 #pragma unroll
@@ -467,16 +492,17 @@ __device__ void final_face(const FinalArgs &a, int32_t face)
         }
     }
     __syncthreads();
-    if (lane == 0) out[-1] = vh::encode_one(med[0], med[1], med[2]);
+    // stored before lightofs, in reverse style order (FinalLightFace.cs:270-279)
+    if (lane == 0) a.lump[U.median_ofs >> 2] = vh::encode_one(med[0], med[1], med[2]);
 }
 
 __global__ __launch_bounds__(kWave) void k_final(FinalArgs a)
 {
-    const int32_t face = a.lit_faces[blockIdx.x];
-    if (a.faces[face].nb == 4)
-        final_face<4>(a, face);
+    const int32_t unit = a.lit_units[blockIdx.x];
+    if (a.faces[a.units[unit].face].nb == 4)
+        final_face<4>(a, unit);
     else
-        final_face<1>(a, face);
+        final_face<1>(a, unit);
 }
 
 // The bounce's light as TotalLight: slot 0 the flat light, slots 1-3 the bump
@@ -496,13 +522,15 @@ __global__ void k_plight(const float4 *__restrict__ total, const float4 *__restr
 
 struct FinalData {
     int32_t nfaces = 0, npatch = 0, nsamples = 0, nlit = 0;
-    int64_t nluxels = 0, ncells = 0, lump_bytes = 0, ntiles = 0;
+    int64_t nluxels = 0, ncells = 0, lump_bytes = 0, ntiles = 0, nbtiles = 0;
     std::vector<int32_t> h_face_ptr, h_normals, h_offsets;
+    std::vector<FaceDev> h_faces;
+    uint64_t style_generation = ~0ull;  // the gather styles the layout was made for
     bool bumped_owner = false;  // some bumped face owns a leaf patch
     bool have_luxels = false;   // luxel points given (else made from the frames)
     bool have_lump = false;     // a vh_final_light has filled the lump
     vh::DevBuf faces, luxel_pos, bounds, nbr_ptr, nbr, plist_ptr, plist, porigin, wptr, wpts;
-    vh::DevBuf plight, tiles, lit_faces, cells, pcells, mvals, lump;
+    vh::DevBuf plight, units, tiles, lit_faces, cells, pcells, mvals, lump;
 };
 
 namespace vh {
@@ -511,7 +539,7 @@ void final_release(vh_ctx *c)
     FinalData *D = c->final_;
     if (!D) return;
     for (DevBuf *b : {&D->faces, &D->luxel_pos, &D->bounds, &D->nbr_ptr, &D->nbr, &D->plist_ptr,
-                      &D->plist, &D->porigin, &D->wptr, &D->wpts, &D->plight, &D->tiles,
+                      &D->plist, &D->porigin, &D->wptr, &D->wpts, &D->plight, &D->units, &D->tiles,
                       &D->lit_faces, &D->cells, &D->pcells, &D->mvals, &D->lump})
         b->release();
     delete D;
@@ -537,6 +565,79 @@ int put(DevBuf &b, const T *host, size_t count, const char *what, hipStream_t s)
     int rc = b.ensure(count * sizeof(T) + 16, what);
     if (rc) return rc;
     if (count) VH_HIP(hipMemcpyAsync(b.ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return VH_OK;
+}
+
+// LightmapOffsets.Compute (Rad/Light/LightmapOffsets.cs:60-123) over the
+// gather's face styles: a lit face reserves a median word per style, then 4
+// bytes per luxel per normal per style; its light offset is just past the
+// median words.  The (face, style) units are listed style slot by style
+// slot, so the slot-0 units -- the only ones the bounce reaches -- and their
+// tiles come first.
+int final_layout(FinalData *D, const GatherView &gv, hipStream_t st)
+{
+    const int32_t F = D->nfaces;
+    const std::vector<uint8_t> &sty = *gv.h_face_styles;
+    std::vector<int32_t> offsets(F, -1), nst(F, 0);
+    int64_t bytes = 0;
+    for (int32_t f = 0; f < F; f++) {
+        const FaceDev &o = D->h_faces[f];
+        int k = 0;
+        while (k < 4 && sty[(size_t)f * 4 + k] != 255) k++;
+        if (!o.lit || k == 0) continue;
+        nst[f] = k;
+        bytes += 4 * (int64_t)k;
+        offsets[f] = (int32_t)bytes;
+        bytes += 4 * (int64_t)o.nb * o.w * o.h * k;
+        if (bytes > INT32_MAX) {
+            set_error("the lump would pass 2 GiB");
+            return VH_EINVAL;
+        }
+    }
+    std::vector<UnitDev> units;
+    std::vector<int2> tiles;
+    std::vector<int32_t> lit;
+    int64_t cells = 0, mv = 0, nbtiles = 0;
+    for (int slot = 0; slot < 4; slot++) {
+        for (int32_t f = 0; f < F; f++) {
+            if (slot >= nst[f]) continue;
+            const FaceDev &o = D->h_faces[f];
+            const int64_t n = (int64_t)o.w * o.h;
+            UnitDev u;
+            std::memset(&u, 0, sizeof u);
+            u.face = f;
+            u.slot = slot;
+            u.style = sty[(size_t)f * 4 + slot];
+            u.cell_base = cells;
+            u.mval_base = mv;
+            u.lump_ofs = offsets[f] + 4 * (int64_t)slot * o.nb * n;
+            u.median_ofs = offsets[f] - 4 * (int64_t)(slot + 1);
+            const int32_t id = (int32_t)units.size();
+            units.push_back(u);
+            lit.push_back(id);
+            for (int64_t t0 = 0; t0 < n; t0 += kWave) tiles.push_back(make_int2(id, (int32_t)t0));
+            cells += (int64_t)o.nb * n;
+            mv += n;
+        }
+        if (slot == 0) nbtiles = (int64_t)tiles.size();
+    }
+    D->h_offsets = offsets;
+    D->lump_bytes = bytes;
+    D->ncells = cells;
+    D->nlit = (int32_t)lit.size();
+    D->ntiles = (int64_t)tiles.size();
+    D->nbtiles = nbtiles;
+    int rc;
+    if ((rc = put(D->units, units.data(), units.size(), "final units", st)) ||
+        (rc = put(D->tiles, tiles.data(), tiles.size(), "tiles", st)) ||
+        (rc = put(D->lit_faces, lit.data(), lit.size(), "lit units", st)) ||
+        (rc = D->cells.ensure((size_t)std::max<int64_t>(cells, 1) * sizeof(float4), "cells")) ||
+        (rc = D->pcells.ensure((size_t)std::max<int64_t>(cells, 1) * sizeof(float4), "patch cells")) ||
+        (rc = D->mvals.ensure((size_t)std::max<int64_t>(mv, 1) * sizeof(float4), "median values")) ||
+        (rc = D->lump.ensure((size_t)std::max<int64_t>(bytes, 4), "lump")))
+        return rc;
+    VH_HIP(hipStreamSynchronize(st));
+    D->style_generation = gv.style_generation;
     return VH_OK;
 }
 }  // namespace
@@ -565,9 +666,7 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
     }
     const std::vector<int32_t> &fp = *gv.h_face_ptr, &nc = *gv.h_normals;
     std::vector<FaceDev> fd(F > 0 ? F : 1);
-    std::vector<int32_t> offsets(F, -1), lit;
-    std::vector<int2> tiles;
-    int64_t lux = 0, cells = 0, bytes = 0;
+    int64_t lux = 0;
     for (int32_t f = 0; f < F; f++) {
         const vh_final_face &x = d->faces[f];
         if (x.width < 0 || x.height < 0 || x.width > kMaxSide || x.height > kMaxSide ||
@@ -594,23 +693,7 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
         o.s0 = fp[f];
         o.s1 = fp[f + 1];
         o.lux_base = lux;
-        o.cell_base = cells;
-        o.light_ofs = -1;
-        const int64_t n = (int64_t)x.width * x.height;
-        if (x.lit) {
-            bytes += 4;
-            o.light_ofs = bytes;
-            offsets[f] = (int32_t)bytes;
-            bytes += 4 * (int64_t)o.nb * n;
-            lit.push_back(f);
-            for (int64_t t0 = 0; t0 < n; t0 += kWave) tiles.push_back(make_int2(f, (int32_t)t0));
-            cells += (int64_t)o.nb * n;
-        }
-        lux += n;
-        if (bytes > INT32_MAX) {
-            set_error("the lump would pass 2 GiB");
-            return VH_EINVAL;
-        }
+        lux += (int64_t)x.width * x.height;
     }
     if (F > 0 && d->nbr_ptr[0] != 0) {
         set_error("nbr_ptr must start at 0");
@@ -669,14 +752,10 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
     D->nfaces = F;
     D->npatch = P;
     D->nsamples = gv.nsamples;
-    D->nlit = (int32_t)lit.size();
     D->nluxels = lux;
-    D->ncells = cells;
-    D->lump_bytes = bytes;
-    D->ntiles = (int64_t)tiles.size();
     D->h_face_ptr = fp;
     D->h_normals = nc;
-    D->h_offsets = offsets;
+    D->h_faces.assign(fd.begin(), fd.begin() + F);
     D->bumped_owner = bumped_owner;
     D->have_luxels = d->luxel_pos != nullptr;
     D->have_lump = false;
@@ -697,16 +776,10 @@ int vh_final_load(vh_ctx *c, const vh_final_desc *d)
         (rc = put(D->porigin, porg.data(), (size_t)P, "patch origins", st)) ||
         (rc = put(D->wptr, P > 0 ? d->winding_ptr : zero, (size_t)P + 1, "winding offsets", st)) ||
         (rc = put(D->wpts, d->winding, (size_t)nw * 3, "windings", st)) ||
-        (rc = put(D->tiles, tiles.data(), tiles.size(), "tiles", st)) ||
-        (rc = put(D->lit_faces, lit.data(), lit.size(), "lit faces", st)) ||
-        (rc = D->plight.ensure((size_t)std::max(P, 1) * 4 * sizeof(float4), "patch lights")) ||
-        (rc = D->cells.ensure((size_t)std::max<int64_t>(cells, 1) * sizeof(float4), "cells")) ||
-        (rc = D->pcells.ensure((size_t)std::max<int64_t>(cells, 1) * sizeof(float4), "patch cells")) ||
-        (rc = D->mvals.ensure((size_t)std::max<int64_t>(lux, 1) * sizeof(float4), "median values")) ||
-        (rc = D->lump.ensure((size_t)std::max<int64_t>(bytes, 4), "lump")))
+        (rc = D->plight.ensure((size_t)std::max(P, 1) * 4 * sizeof(float4), "patch lights")))
         return rc;
-    VH_HIP(hipStreamSynchronize(st));
-    return VH_OK;
+    // the layout follows the gather's light styles (style 0 alone until a run)
+    return final_layout(D, gv, st);
 }
 
 int vh_final_layout(vh_ctx *c, int32_t *offsets, int64_t *bytes)
@@ -720,6 +793,13 @@ int vh_final_layout(vh_ctx *c, int32_t *offsets, int64_t *bytes)
     if (!c->final_) {
         set_error("no final-light inputs: call vh_final_load first");
         return VH_ESTATE;
+    }
+    GatherView gv;
+    if (gather_view(c, &gv) && gv.nfaces == c->final_->nfaces &&
+        gv.style_generation != c->final_->style_generation) {
+        VH_HIP(hipSetDevice(c->device));
+        int rc = final_layout(c->final_, gv, c->stream);
+        if (rc) return rc;
     }
     if (offsets && !c->final_->h_offsets.empty())
         std::memcpy(offsets, c->final_->h_offsets.data(), c->final_->h_offsets.size() * 4);
@@ -750,6 +830,11 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
     if (*gv.h_face_ptr != D->h_face_ptr || *gv.h_normals != D->h_normals) {
         set_error("the gather was reloaded with other faces: call vh_final_load again");
         return VH_ESTATE;
+    }
+    if (gv.style_generation != D->style_generation) {
+        VH_HIP(hipSetDevice(c->device));
+        int rc = final_layout(D, gv, c->stream);
+        if (rc) return rc;
     }
     if (capacity < D->lump_bytes) {
         set_error(std::to_string(D->lump_bytes) + " lump bytes need as much room, and " +
@@ -806,9 +891,11 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
     a.plight = (const float4 *)D->plight.ptr;
     a.spos = gv.pos;
     a.lv = o->which ? gv.hdr : gv.ldr;
+    a.fstyles = gv.face_styles;
     a.nsamples = gv.nsamples;
+    a.units = (const UnitDev *)D->units.ptr;
     a.tiles = (const int2 *)D->tiles.ptr;
-    a.lit_faces = (const int32_t *)D->lit_faces.ptr;
+    a.lit_units = (const int32_t *)D->lit_faces.ptr;
     a.cells = (float4 *)D->cells.ptr;
     a.pcells = (float4 *)D->pcells.ptr;
     a.mvals = (float4 *)D->mvals.ptr;
@@ -819,8 +906,8 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
         KernelTimer timer(c, VH_KERNEL_FINAL, st);
         if (D->ntiles > 0) {
             hipLaunchKernelGGL(k_radial<false>, dim3((uint32_t)D->ntiles), dim3(kWave), 0, st, a);
-            if (a.bounce)
-                hipLaunchKernelGGL(k_radial<true>, dim3((uint32_t)D->ntiles), dim3(kWave), 0, st, a);
+            if (a.bounce && D->nbtiles > 0)  // style 0's units only: the first tiles
+                hipLaunchKernelGGL(k_radial<true>, dim3((uint32_t)D->nbtiles), dim3(kWave), 0, st, a);
         }
         if (D->nlit > 0)
             hipLaunchKernelGGL(k_final, dim3((uint32_t)D->nlit), dim3(kWave), 0, st, a);

@@ -30,6 +30,16 @@
 
 struct GatherData {
     int32_t nfaces = 0, nsamples = 0, nlights = 0, pvs_bytes = 0;
+    // light styles: the lights sorted by style group (group 0 is style 0,
+    // then the other styles in ascending number), each group in light order;
+    // per face up to four slots (slot 0 style 0, the rest by first touch)
+    int32_t ngroups_style = 1, nslots = 1;
+    std::vector<int32_t> h_group_style;  // style number of each group
+    std::vector<uint8_t> h_face_styles;  // 4 per face, after a run
+    int64_t dropped_styles = 0;          // (face, style) pairs past the fourth slot, last run
+    uint64_t style_generation = 0;       // changes whenever h_face_styles may have
+    vh::DevBuf light_index, group_ptr, first_touch, slot_group, face_styles, sample_face, stage_ldr,
+        stage_hdr, dropped;
     int32_t n_sky_ambient = 0;  // lights of type sky ambient (the rest light the direct points)
     int32_t sun_samples = 1, sky_count = 0, ao_count = 0;
     float sun_extent = 0.0f, ao_radius = 0.0f;
@@ -52,6 +62,9 @@ void gather_release(vh_ctx *c)
 {
     GatherData *G = c->gather;
     if (!G) return;
+    for (DevBuf *b : {&G->light_index, &G->group_ptr, &G->first_touch, &G->slot_group, &G->face_styles,
+                      &G->sample_face, &G->stage_ldr, &G->stage_hdr, &G->dropped})
+        b->release();
     for (DevBuf *b : {&G->face_ptr, &G->face_nc, &G->face_normals, &G->pos, &G->cluster,
                       &G->lights, &G->pvs, &G->jitter, &G->sky, &G->ao_dirs, &G->chunk_face,
                       &G->chunk_first, &G->out_ldr, &G->out_hdr, &G->out_ao, &G->ticket, &G->ss_frames,
@@ -79,6 +92,10 @@ bool gather_view(vh_ctx *c, GatherView *v)
     v->ldr = (const float4 *)G->out_ldr.ptr;
     v->hdr = (const float4 *)G->out_hdr.ptr;
     v->have_results = G->have_results;
+    v->nslots = G->nslots;
+    v->face_styles = (const uint8_t *)G->face_styles.ptr;
+    v->h_face_styles = &G->h_face_styles;
+    v->style_generation = G->style_generation;
     return true;
 }
 
@@ -95,7 +112,12 @@ struct GatherArgs {
     const float *face_normals;  // 12 per face
     const float *pos;
     const int32_t *cluster;
-    const vh_light *lights;
+    const vh_light *lights;      // sorted by style group
+    const int32_t *light_index;  // each sorted light's index in the caller's list
+    const int32_t *group_ptr;    // style group g: sorted lights [group_ptr[g], group_ptr[g + 1])
+    int32_t ngroups;
+    const int32_t *face_group;   // the resample: per face the one group it lights (-1 none), stride 4
+    unsigned long long *first_touch;  // per (face, group): first (group in face, light) to touch it
     int32_t nlights;
     const uint8_t *pvs;
     int32_t pvs_bytes;
@@ -111,7 +133,7 @@ struct GatherArgs {
     const int32_t *chunk_face;
     const int32_t *chunk_first;
     int64_t nchunks;
-    float4 *out_ldr;  // 4 normal slots x nsamples
+    float4 *out_ldr;  // per style group, 4 normals x nsamples (the resample: one group)
     float4 *out_hdr;
     float *out_ao;
     const float4 *nodes;
@@ -217,14 +239,26 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
     int32_t cl = a.cluster[s];
 
     int nr = 0;
+    // this group's index in its face: the first-touch keys order groups
+    const int32_t grp = group_live ? (gfirst - a.face_ptr[f]) >> 2 : 0;
+    // style groups in turn (one, for the resample); within a group the
+    // lights keep their order, so each style slot sums as the reference does
+    int32_t g_lo = 0, g_hi = a.ngroups;
+    if (a.face_group) {
+        g_lo = a.face_group[(size_t)f * 4];
+        g_hi = g_lo + 1;
+    }
+    for (int32_t g = g_lo; g < g_hi; g++) {
     float4 acc_l[4], acc_h[4];
     for (int n = 0; n < 4; n++) {
         acc_l[n] = make_float4(0, 0, 0, 0);
         acc_h[n] = make_float4(0, 0, 0, 0);
     }
-
-    for (int32_t li = 0; li < a.nlights; li++) {
-        const vh_light &L = a.lights[li];
+    unsigned long long touch = ~0ull;
+    const int32_t k_lo = g >= 0 ? a.group_ptr[g] : 0, k_hi = g >= 0 ? a.group_ptr[g + 1] : 0;
+    for (int32_t k = k_lo; k < k_hi; k++) {
+        const int32_t li = a.light_index[k];
+        const vh_light &L = a.lights[k];
         bool inmask = real && pvs_check(a, li, cl);
         if (!group_any(inmask)) continue;
         // EmitResample (FaceLightJob.cs:474-482): ambient points see only sky
@@ -377,6 +411,8 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
             nz = nz || fx[n] != 0.0f;
         }
         if (!a.no_skip && !group_any(nz)) continue;
+        // FindOrAllocateStyle's order: the first (group, light) to touch it
+        if (touch == ~0ull) touch = (unsigned long long)grp * (unsigned long long)a.nlights + (unsigned long long)li;
         if (real) {
             _Pragma("unroll") for (int n = 0; n < normals; n++) {
                 acc_l[n].x = acc_l[n].x + (fx[n] * L.intensity[0]);
@@ -390,27 +426,45 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
             }
         }
     }
-    if (real) {
-        _Pragma("unroll") for (int n = 0; n < normals; n++) {
-            a.out_ldr[(size_t)n * a.nsamples + s] = acc_l[n];
-            a.out_hdr[(size_t)n * a.nsamples + s] = acc_h[n];
+    // every sample's value is written, zeros included (no clearing pass)
+    if (real && g >= 0) {
+        const size_t og = a.face_group ? 0 : (size_t)g;
+        _Pragma("unroll") for (int n = 0; n < 4; n++) {
+            a.out_ldr[(og * 4 + n) * a.nsamples + s] = acc_l[n];
+            a.out_hdr[(og * 4 + n) * a.nsamples + s] = acc_h[n];
         }
-        if (a.ao_count > 0) {
-            int unblocked = 0;
-            const float *dirs = a.ao_dirs + (size_t)f * a.ao_count * 3;
-            for (int32_t k = 0; k < a.ao_count; k++) {
-                Ray r;
-                r.ox = px;
-                r.oy = py;
-                r.oz = pz;
-                r.dx = dirs[3 * k] * a.ao_radius;
-                r.dy = dirs[3 * k + 1] * a.ao_radius;
-                r.dz = dirs[3 * k + 2] * a.ao_radius;
-                r.reach = 1.0f;
-                if (!blocked(a, stk, r, nr)) unblocked++;
-            }
-            a.out_ao[s] = (float)unblocked / (float)a.ao_count;
+    }
+    if (a.first_touch && g >= 0) {
+        // one atomic per wave and group: the wave's groups are one face's
+        for (int o = 32; o > 0; o >>= 1) {
+            unsigned long long v = __shfl_xor(touch, o, 64);
+            touch = v < touch ? v : touch;
         }
+        if (lane == 0 && touch != ~0ull) atomicMin(&a.first_touch[(size_t)f * a.ngroups + g], touch);
+    }
+    }
+    if (real && a.face_group && a.face_group[(size_t)f * 4] < 0) {
+        // a resample face without the style: its points gather nothing
+        _Pragma("unroll") for (int n = 0; n < 4; n++) {
+            a.out_ldr[(size_t)n * a.nsamples + s] = make_float4(0, 0, 0, 0);
+            a.out_hdr[(size_t)n * a.nsamples + s] = make_float4(0, 0, 0, 0);
+        }
+    }
+    if (real && a.ao_count > 0) {
+        int unblocked = 0;
+        const float *dirs = a.ao_dirs + (size_t)f * a.ao_count * 3;
+        for (int32_t k = 0; k < a.ao_count; k++) {
+            Ray r;
+            r.ox = px;
+            r.oy = py;
+            r.oz = pz;
+            r.dx = dirs[3 * k] * a.ao_radius;
+            r.dy = dirs[3 * k + 1] * a.ao_radius;
+            r.dz = dirs[3 * k + 2] * a.ao_radius;
+            r.reach = 1.0f;
+            if (!blocked(a, stk, r, nr)) unblocked++;
+        }
+        a.out_ao[s] = (float)unblocked / (float)a.ao_count;
     }
     // every lane of the wave is here: one atomic per chunk
     for (int o = 32; o > 0; o >>= 1) nr += __shfl_xor(nr, o, 64);
@@ -441,6 +495,66 @@ __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first
         if ((int64_t)c >= a.nchunks) break;
         light_chunk(a, (int64_t)c, stk);
     }
+}
+
+// FaceLight.FindOrAllocateStyle over a whole face at once (FaceLight.cs:204-222,
+// FaceLightJob.cs:185,435): slot 0 is style 0; the other styles that touched
+// the face take slots 1-3 in the order of their first touch (group in the
+// face, then light); a fifth is dropped.  A thread per face.
+__global__ void k_assign_slots(const unsigned long long *__restrict__ first_touch, int32_t nfaces,
+                               int32_t ngroups, const int32_t *__restrict__ group_style,
+                               int32_t *__restrict__ slot_group, uint8_t *__restrict__ face_styles,
+                               unsigned long long *__restrict__ dropped)
+{
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfaces) return;
+    const unsigned long long *ft = first_touch + (size_t)f * ngroups;
+    int32_t sg[4] = {0, -1, -1, -1};
+    unsigned long long last = 0;  // keys are distinct: each (group, light) touches one style
+    int k = 1;
+    for (; k < 4; k++) {
+        unsigned long long best = ~0ull;
+        int32_t bg = -1;
+        for (int32_t g = 1; g < ngroups; g++) {
+            const unsigned long long v = ft[g];
+            if (v != ~0ull && (k == 1 || v > last) && v < best) {
+                best = v;
+                bg = g;
+            }
+        }
+        if (bg < 0) break;
+        sg[k] = bg;
+        last = best;
+    }
+    int32_t more = 0;
+    if (k == 4)
+        for (int32_t g = 1; g < ngroups; g++) more += ft[g] != ~0ull && ft[g] > last;
+    if (more) atomicAdd(dropped, (unsigned long long)more);
+    for (int j = 0; j < 4; j++) {
+        slot_group[(size_t)f * 4 + j] = sg[j];
+        face_styles[(size_t)f * 4 + j] = sg[j] < 0 ? (uint8_t)255 : (uint8_t)group_style[sg[j]];
+    }
+}
+
+// Per-group light into the faces' slots: thread per (slot, normal, sample).
+__global__ void k_fill_slots(const float4 *__restrict__ stage_l, const float4 *__restrict__ stage_h,
+                             const int32_t *__restrict__ slot_group, const int32_t *__restrict__ sample_face,
+                             int32_t nsamples, int32_t nslots, float4 *__restrict__ out_l,
+                             float4 *__restrict__ out_h)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)4 * nsamples;
+    if (i >= per * nslots) return;
+    const int32_t k = (int32_t)(i / per), n = (int32_t)((i / nsamples) % 4);
+    const int32_t s = (int32_t)(i % nsamples);
+    const int32_t g = slot_group[(size_t)sample_face[s] * 4 + k];
+    float4 l = make_float4(0, 0, 0, 0), h = l;
+    if (g >= 0) {
+        l = stage_l[((size_t)g * 4 + n) * nsamples + s];
+        h = stage_h[((size_t)g * 4 + n) * nsamples + s];
+    }
+    out_l[i] = l;
+    out_h[i] = h;
 }
 
 __global__ void k_encode(const float *__restrict__ rgb, int64_t n, int32_t stride,
@@ -489,6 +603,11 @@ GatherArgs gather_args(vh_ctx *c)
     a.pos = (const float *)G->pos.ptr;
     a.cluster = (const int32_t *)G->cluster.ptr;
     a.lights = (const vh_light *)G->lights.ptr;
+    a.light_index = (const int32_t *)G->light_index.ptr;
+    a.group_ptr = (const int32_t *)G->group_ptr.ptr;
+    a.ngroups = G->ngroups_style;
+    a.face_group = nullptr;
+    a.first_touch = G->ngroups_style > 1 ? (unsigned long long *)G->first_touch.ptr : nullptr;
     a.nlights = G->nlights;
     a.pvs = (const uint8_t *)G->pvs.ptr;
     a.pvs_bytes = G->pvs_bytes;
@@ -504,8 +623,9 @@ GatherArgs gather_args(vh_ctx *c)
     a.chunk_face = (const int32_t *)G->chunk_face.ptr;
     a.chunk_first = (const int32_t *)G->chunk_first.ptr;
     a.nchunks = G->nchunks;
-    a.out_ldr = (float4 *)G->out_ldr.ptr;
-    a.out_hdr = (float4 *)G->out_hdr.ptr;
+    // one style group: straight into slot 0; else per group, then into slots
+    a.out_ldr = (float4 *)(G->ngroups_style > 1 ? G->stage_ldr.ptr : G->out_ldr.ptr);
+    a.out_hdr = (float4 *)(G->ngroups_style > 1 ? G->stage_hdr.ptr : G->out_hdr.ptr);
     a.out_ao = (float *)G->out_ao.ptr;
     a.nodes = (const float4 *)c->nodes.ptr;
     a.tris = (const float4 *)c->tris.ptr;
@@ -571,11 +691,17 @@ __global__ void k_ss_select(const int32_t *__restrict__ sface, const int32_t *__
                             const vh_final_face *__restrict__ frames, const int64_t *__restrict__ lux_ptr,
                             const int32_t *__restrict__ luxmap, const int32_t *__restrict__ st,
                             const float *__restrict__ inten, int32_t S, uint8_t *__restrict__ processed,
-                            uint8_t *__restrict__ flags, int32_t *__restrict__ counts, int32_t garbage)
+                            uint8_t *__restrict__ flags, int32_t *__restrict__ counts, int32_t garbage,
+                            const int32_t *__restrict__ face_group)
 {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const int32_t f = sface[s], nc = face_nc[f];
+    // a face without this style slot has no StyleSupersample for it
+    if (face_group[(size_t)f * 4] < 0) {
+        flags[s] = 0;
+        return;
+    }
     const int32_t w = frames[f].width, h = frames[f].height;
     const int64_t base = lux_ptr[f];
     // the neighbours in the order Step visits them
@@ -752,10 +878,10 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
     }
     for (int32_t l = 0; l < NL; l++) {
         const vh_light &L = d->lights[l];
-        if (L.style != 0) {
+        if (L.style < 0 || L.style > 254) {
             set_error("light " + std::to_string(l) + " has style " + std::to_string(L.style) +
-                      "; only style 0 is supported");
-            return VH_ENOTSUP;
+                      "; a light style runs from 0 to 254 (255 marks an empty slot)");
+            return VH_EINVAL;
         }
         if (L.type != VH_EMIT_POINT && L.type != VH_EMIT_SURFACE && L.type != VH_EMIT_SPOTLIGHT &&
             L.type != VH_EMIT_SKYLIGHT && L.type != VH_EMIT_SKYAMBIENT) {
@@ -788,6 +914,37 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
     G->ao_radius = d->ao_radius;
     G->h_face_ptr.assign(d->face_sample_ptr, d->face_sample_ptr + F + 1);
     G->h_normals.assign(d->face_normal_count, d->face_normal_count + F);
+    // style groups: style 0 first (its slot always exists), then ascending
+    std::vector<int32_t> gstyle{0};
+    for (int32_t l = 0; l < NL; l++)
+        if (d->lights[l].style != 0) gstyle.push_back(d->lights[l].style);
+    std::sort(gstyle.begin() + 1, gstyle.end());
+    gstyle.erase(std::unique(gstyle.begin() + 1, gstyle.end()), gstyle.end());
+    const int32_t NG = (int32_t)gstyle.size();
+    std::vector<int32_t> lindex, gptr(NG + 1, 0);
+    std::vector<vh_light> sorted;
+    for (int32_t g = 0; g < NG; g++) {
+        for (int32_t l = 0; l < NL; l++)
+            if (d->lights[l].style == gstyle[g]) {
+                lindex.push_back(l);
+                sorted.push_back(d->lights[l]);
+            }
+        gptr[g + 1] = (int32_t)lindex.size();
+    }
+    // the group offsets, then each group's style number
+    std::vector<int32_t> gptr_styles(gptr);
+    gptr_styles.insert(gptr_styles.end(), gstyle.begin(), gstyle.end());
+    G->ngroups_style = NG;
+    G->nslots = NG < 4 ? NG : 4;
+    G->h_group_style = gstyle;
+    G->h_face_styles.assign((size_t)F * 4, 255);
+    for (int32_t f = 0; f < F; f++) G->h_face_styles[(size_t)f * 4] = 0;
+    G->style_generation++;
+    std::vector<int32_t> sface((size_t)S > 0 ? S : 1, 0), slot0((size_t)F * 4 + 4, -1);
+    for (int32_t f = 0; f < F; f++) {
+        for (int32_t x = d->face_sample_ptr[f]; x < d->face_sample_ptr[f + 1]; x++) sface[x] = f;
+        slot0[(size_t)f * 4] = 0;
+    }
     // chunks: 16 groups (64 samples) of one face each
     std::vector<int32_t> cf, cs;
     G->h_face_chunk_ptr.assign(F + 1, 0);
@@ -806,15 +963,24 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
         (rc = upload(G->face_normals, d->face_normals, (size_t)F * 12, "face normals", st)) ||
         (rc = upload(G->pos, d->sample_pos, (size_t)S * 3, "sample positions", st)) ||
         (rc = upload(G->cluster, d->sample_cluster, (size_t)S, "sample clusters", st)) ||
-        (rc = upload(G->lights, d->lights, (size_t)NL, "lights", st)) ||
+        (rc = upload(G->lights, sorted.data(), sorted.size(), "lights", st)) ||
+        (rc = upload(G->light_index, lindex.data(), lindex.size(), "light order", st)) ||
+        (rc = upload(G->group_ptr, gptr_styles.data(), gptr_styles.size(), "style groups", st)) ||
+        (rc = upload(G->sample_face, sface.data(), (size_t)S, "sample faces", st)) ||
+        (rc = upload(G->slot_group, slot0.data(), slot0.size(), "style slots", st)) ||
+        (rc = upload(G->face_styles, G->h_face_styles.data(), G->h_face_styles.size(), "face styles", st)) ||
+        (NG > 1 && ((rc = G->first_touch.ensure((size_t)F * NG * 8 + 16, "first touches")) ||
+                    (rc = G->stage_ldr.ensure((size_t)NG * S * 4 * 16 + 16, "LDR lighting per style")) ||
+                    (rc = G->stage_hdr.ensure((size_t)NG * S * 4 * 16 + 16, "HDR lighting per style")))) ||
+        (rc = G->dropped.ensure(16, "dropped styles")) ||
         (rc = upload(G->pvs, d->light_pvs, (size_t)NL * G->pvs_bytes, "light PVS", st)) ||
         (rc = upload(G->jitter, d->sun_jitter, (size_t)(ns - 1) * 3, "sun jitter", st)) ||
         (rc = upload(G->sky, d->sky_dirs, (size_t)G->sky_count * 3, "sky directions", st)) ||
         (rc = upload(G->ao_dirs, d->ao_dirs, (size_t)F * G->ao_count * 3, "AO directions", st)) ||
         (rc = upload(G->chunk_face, cf.data(), cf.size(), "chunks", st)) ||
         (rc = upload(G->chunk_first, cs.data(), cs.size(), "chunks", st)) ||
-        (rc = G->out_ldr.ensure((size_t)S * 4 * 16 + 16, "LDR lighting")) ||
-        (rc = G->out_hdr.ensure((size_t)S * 4 * 16 + 16, "HDR lighting")) ||
+        (rc = G->out_ldr.ensure((size_t)G->nslots * S * 4 * 16 + 16, "LDR lighting")) ||
+        (rc = G->out_hdr.ensure((size_t)G->nslots * S * 4 * 16 + 16, "HDR lighting")) ||
         (rc = G->out_ao.ensure((size_t)S * 4 + 16, "ambient occlusion")) ||
         (rc = G->ticket.ensure(16, "queue ticket")))
         return rc;
@@ -846,11 +1012,14 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
     GatherData *G = c->gather;
     hipStream_t st = c->stream;
     GatherArgs a = gather_args(c);
-    size_t outb = (size_t)G->nsamples * 4 * 16;
-    VH_HIP(hipMemsetAsync(G->out_ldr.ptr, 0, outb, st));
-    VH_HIP(hipMemsetAsync(G->out_hdr.ptr, 0, outb, st));
+    const int32_t NG = G->ngroups_style;
+    // the kernel writes every sample's light; AO only when it is traced
     VH_HIP(hipMemsetAsync(G->out_ao.ptr, 0, (size_t)G->nsamples * 4, st));
     VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 16, st));  // queue ticket, ray count
+    if (NG > 1) {
+        VH_HIP(hipMemsetAsync(G->first_touch.ptr, 0xFF, (size_t)G->nfaces * NG * 8, st));
+        VH_HIP(hipMemsetAsync(G->dropped.ptr, 0, 8, st));
+    }
     if (G->nchunks > 0) {
         KernelTimer timer(c, VH_KERNEL_FACELIGHT, st);
         if (schedule == VH_SCHEDULE_QUEUE) {
@@ -873,11 +1042,31 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
         timer.done();
         VH_HIP(hipGetLastError());
     }
+    unsigned long long dropped = 0;
+    if (NG > 1) {
+        if (G->nfaces > 0)
+            hipLaunchKernelGGL(k_assign_slots, dim3((uint32_t)((G->nfaces + 255) / 256)), dim3(256), 0, st,
+                               (const unsigned long long *)G->first_touch.ptr, G->nfaces, NG,
+                               (const int32_t *)G->group_ptr.ptr + NG + 1, (int32_t *)G->slot_group.ptr,
+                               (uint8_t *)G->face_styles.ptr, (unsigned long long *)G->dropped.ptr);
+        const int64_t n = (int64_t)G->nslots * 4 * G->nsamples;
+        if (n > 0)
+            hipLaunchKernelGGL(k_fill_slots, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                               (const float4 *)G->stage_ldr.ptr, (const float4 *)G->stage_hdr.ptr,
+                               (const int32_t *)G->slot_group.ptr, (const int32_t *)G->sample_face.ptr,
+                               G->nsamples, G->nslots, (float4 *)G->out_ldr.ptr, (float4 *)G->out_hdr.ptr);
+        VH_HIP(hipGetLastError());
+        VH_HIP(hipMemcpyAsync(G->h_face_styles.data(), G->face_styles.ptr, G->h_face_styles.size(),
+                              hipMemcpyDeviceToHost, st));
+        VH_HIP(hipMemcpyAsync(&dropped, G->dropped.ptr, 8, hipMemcpyDeviceToHost, st));
+    }
     unsigned long long rays = 0;
     VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
     VH_HIP(hipStreamSynchronize(st));
     c->stats.rays_gather += (int64_t)rays;
     c->stats.calls++;
+    G->dropped_styles = (int64_t)dropped;
+    G->style_generation++;
     G->have_results = true;
     return VH_OK;
 }
@@ -965,9 +1154,13 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
         (rc = B.counts.ensure((size_t)F * 4 + 16, "selection counts")) ||
         (rc = G->ss_sface.ensure((size_t)S * 4 + 16, "sample faces")))
         return rc;
-    float4 *light = (float4 *)(d->which ? G->out_hdr.ptr : G->out_ldr.ptr);
-    VH_HIP(hipMemsetAsync(B.processed.ptr, 0, (size_t)S, st));
     VH_HIP(hipMemsetAsync(G->ticket.ptr, 0, 16, st));
+    // one StyleSupersample per style slot (FaceLightJob.cs:688-700), each with
+    // its own selection and passes, resampling only its own style's lights
+    for (int32_t slot = 0; slot < G->nslots; slot++) {
+    float4 *light = (float4 *)(d->which ? G->out_hdr.ptr : G->out_ldr.ptr) + (size_t)slot * 4 * S;
+    const int32_t *face_group = (const int32_t *)G->slot_group.ptr + slot;
+    VH_HIP(hipMemsetAsync(B.processed.ptr, 0, (size_t)S, st));
     hipLaunchKernelGGL(k_ss_init, dim3((uint32_t)F), dim3(kWave), 0, st, (const int32_t *)G->face_ptr.ptr,
                        (const int32_t *)G->face_nc.ptr, (const vh_final_face *)B.frames.ptr,
                        (const int64_t *)B.lux_ptr.ptr, (const int32_t *)(d->sample_st ? B.st_in.ptr : nullptr),
@@ -986,7 +1179,7 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
                                (const vh_final_face *)B.frames.ptr, (const int64_t *)B.lux_ptr.ptr,
                                (const int32_t *)B.luxmap.ptr, (const int32_t *)B.st.ptr, (const float *)B.inten.ptr,
                                S, (uint8_t *)B.processed.ptr, (uint8_t *)B.flags.ptr, (int32_t *)B.counts.ptr,
-                               d->unset_is_garbage ? 1 : 0);
+                               d->unset_is_garbage ? 1 : 0, face_group);
             timer.done();
         }
         VH_HIP(hipGetLastError());
@@ -1000,7 +1193,7 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
             set_error("too many samples selected in one pass for 32-bit point indices");
             return VH_EINVAL;
         }
-        if (pass_counts) pass_counts[pass] = total;
+        if (pass_counts) pass_counts[pass] += total;
         // chunks of 64 points within one face, as the gather's own
         dcf.clear(), dcs.clear(), acf.clear(), acs.clear();
         for (int32_t f = 0; f <= F; f++) {
@@ -1048,6 +1241,8 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
             a.ao_count = 0;
             a.light_filter = direct ? 2 : 1;
             a.no_skip = 1;
+            a.face_group = face_group;
+            a.first_touch = nullptr;
             const bool any = direct ? G->nlights > G->n_sky_ambient : G->n_sky_ambient > 0;
             if (a.nchunks > 0 && any)
                 hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)a.nchunks), dim3(kWave), 0, st, a,
@@ -1061,6 +1256,7 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
                            (float *)B.inten.ptr, S);
         timer.done();
         VH_HIP(hipGetLastError());
+    }
     }
     unsigned long long rays = 0;
     VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
@@ -1087,6 +1283,57 @@ int vh_gather_read(vh_ctx *c, float *ldr, float *hdr, float *ao)
     if (ldr) VH_HIP(hipMemcpy(ldr, G->out_ldr.ptr, outb, hipMemcpyDeviceToHost));
     if (hdr) VH_HIP(hipMemcpy(hdr, G->out_hdr.ptr, outb, hipMemcpyDeviceToHost));
     if (ao) VH_HIP(hipMemcpy(ao, G->out_ao.ptr, (size_t)G->nsamples * 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_gather_styles(vh_ctx *c, uint8_t *face_styles, int32_t *slots, int64_t *dropped)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather || !c->gather->have_results) {
+        set_error("no gather results: call vh_gather_run first");
+        return VH_ESTATE;
+    }
+    GatherData *G = c->gather;
+    if (face_styles && !G->h_face_styles.empty())
+        std::memcpy(face_styles, G->h_face_styles.data(), G->h_face_styles.size());
+    if (slots) *slots = G->nslots;
+    if (dropped) *dropped = G->dropped_styles;
+    return VH_OK;
+}
+
+int vh_gather_read_slot(vh_ctx *c, int32_t slot, float *ldr, float *hdr)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->gather || !c->gather->have_results) {
+        set_error("no gather results: call vh_gather_run first");
+        return VH_ESTATE;
+    }
+    GatherData *G = c->gather;
+    if (slot < 0 || slot >= 4) {
+        set_error("a style slot runs from 0 to 3");
+        return VH_EINVAL;
+    }
+    size_t outb = (size_t)G->nsamples * 4 * 16;
+    if (slot >= G->nslots) {
+        // no face has this slot: all zeros, as the reference's empty slots
+        if (ldr) std::memset(ldr, 0, outb);
+        if (hdr) std::memset(hdr, 0, outb);
+        return VH_OK;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    const size_t off = (size_t)slot * G->nsamples * 4;
+    if (ldr) VH_HIP(hipMemcpy(ldr, (float4 *)G->out_ldr.ptr + off, outb, hipMemcpyDeviceToHost));
+    if (hdr) VH_HIP(hipMemcpy(hdr, (float4 *)G->out_hdr.ptr + off, outb, hipMemcpyDeviceToHost));
     return VH_OK;
 }
 

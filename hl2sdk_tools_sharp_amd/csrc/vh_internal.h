@@ -141,9 +141,13 @@ struct GatherView {
     const std::vector<int32_t> *h_face_ptr = nullptr;  // host copies of the two
     const std::vector<int32_t> *h_normals = nullptr;
     const float *pos = nullptr;         // device, 3 per sample
-    const float4 *ldr = nullptr;        // device, slot 0 = the flat normal
+    const float4 *ldr = nullptr;        // device, [style slot][normal][sample]; normal 0 flat
     const float4 *hdr = nullptr;
     bool have_results = false;
+    int32_t nslots = 1;                  // style slots in ldr / hdr
+    const uint8_t *face_styles = nullptr;  // device, 4 style numbers per face (255 empty)
+    const std::vector<uint8_t> *h_face_styles = nullptr;
+    uint64_t style_generation = 0;       // changes when the face styles may have
 };
 bool gather_view(vh_ctx *c, GatherView *v);
 // The loaded patch system and bounce state, for the warm-start network (vh_bounce.hip).

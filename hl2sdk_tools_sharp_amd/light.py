@@ -252,6 +252,24 @@ class HipFaceLighter:
             ao.ctypes.data_as(ctypes.c_void_p)))
         return GatherResult(ldr, hdr, ao)
 
+    def read_styles(self):
+        """The last run's light styles: (face_styles (F, 4) uint8 style numbers
+        per slot, 255 empty; ldr and hdr (4 slots, 4 normals, S, 4); the
+        (face, style) pairs dropped past the fourth slot)."""
+        L = _lib.lib()
+        h = self.tracer.handle
+        F, S = self.faces.nfaces, self.faces.nsamples
+        fs = np.zeros((max(F, 1), 4), np.uint8)
+        slots, dropped = ctypes.c_int32(), ctypes.c_int64()
+        HipRayTracer._raise(L.vh_gather_styles(h, fs.ctypes.data_as(ctypes.c_void_p), ctypes.byref(slots),
+                                               ctypes.byref(dropped)))
+        ldr = np.zeros((4, 4, S, 4), np.float32)
+        hdr = np.zeros((4, 4, S, 4), np.float32)
+        for k in range(4):
+            HipRayTracer._raise(L.vh_gather_read_slot(h, k, ldr[k].ctypes.data_as(ctypes.c_void_p),
+                                                      hdr[k].ctypes.data_as(ctypes.c_void_p)))
+        return fs[:F], ldr, hdr, int(dropped.value)
+
     def supersample(self, frames: np.ndarray, which: int = 0, extra_passes: int = 4, sample_st=None,
                     unset_is_garbage: bool = False) -> list:
         """FaceLightJob's supersampling passes (Rad/Light/FaceLightJob.cs:698-1207)

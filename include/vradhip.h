@@ -354,10 +354,26 @@ typedef struct vh_supersample_desc {
 } vh_supersample_desc;
 int vh_gather_supersample(vh_ctx *ctx, const vh_supersample_desc *desc, int32_t *pass_counts);
 
-/* Copies the last run's results: ldr/hdr 4 floats (RGB, direct sun amount)
- * per (normal, sample), normal-major over 4 normal slots; ao one float per
- * sample.  Any pointer may be NULL. */
+/* Copies the last run's results for style slot 0 (light style 0): ldr/hdr 4
+ * floats (RGB, direct sun amount) per (normal, sample), normal-major over 4
+ * normal slots; ao one float per sample.  Any pointer may be NULL. */
 int vh_gather_read(vh_ctx *ctx, float *ldr, float *hdr, float *ao);
+
+/* Light styles (FaceLight.Styles / FindOrAllocateStyle,
+ * SourceSharp.MapTools/Rad/Light/FaceLight.cs:98-222; FaceLightJob.cs:185,435):
+ * each face has up to MaxLightmaps = 4 style slots; slot 0 is style 0, and
+ * the other styles of the lights that reach the face take slots 1-3 in the
+ * order they first touch it (group by group, light by light); a fifth style's
+ * light is dropped, as the reference's "Too many light styles" warning does.
+ * A light's style is vh_light.style (0-254).  vh_gather_styles copies the
+ * last run's 4 style numbers per face (255 for an empty slot), the number of
+ * slots any face uses (slots, 1-4) and the (face, style) pairs dropped;
+ * vh_gather_read_slot copies one slot's light as vh_gather_read does slot 0
+ * (zeros for a slot no face uses, or where a face has no style in it).
+ * Supersampling (vh_gather_supersample) runs its passes per slot, each
+ * resampling only its own style's lights; vh_patch_lights reads slot 0. */
+int vh_gather_styles(vh_ctx *ctx, uint8_t *face_styles, int32_t *slots, int64_t *dropped);
+int vh_gather_read_slot(vh_ctx *ctx, int32_t slot, float *ldr, float *hdr);
 
 /* StockLightColor.Encode (SourceSharp.MapTools/Rad/Ambient/StockLightColor.cs:136-185)
  * into ColorRGBExp32 (4 bytes: R, G, B, signed exponent) for n colours read
@@ -369,7 +385,7 @@ int vh_encode_rgbe(vh_ctx *ctx, const float *rgb, int64_t n, int32_t stride, uin
  * byte offset face_base(f) + (b * numSamples(f) + j) * 4, face_base the
  * running sum of 4 * normals * samples (FinalLightFace.cs:270 with one style).
  * out needs 4 * sum(normals * samples) bytes. */
-int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacity);
+int vh_lightmap_encode(vh_ctx *ctx, int32_t which, uint8_t *out, int64_t capacity);  /* style 0 */
 
 /* PatchLighting.BuildPatchLights with AddSampleToPatch
  * (SourceSharp.MapTools/Rad/Light/PatchLighting.cs:65-249): the last gather's

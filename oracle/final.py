@@ -209,83 +209,116 @@ def face_patch_lists(nfaces: int, patch_face, patch_leaf) -> List[List[int]]:
 
 def final_light(faces: Sequence[Face], luxels: Sequence[np.ndarray], samples: Sequence[dict],
                 neighbours: Sequence[Sequence[int]], patches: Optional[dict] = None,
-                red_errors: bool = False, only=None) -> List[Optional[dict]]:
-    """Run's per-face loop for style 0 (FinalLightFace.cs:93-299).
+                red_errors: bool = False, only=None, styles=None) -> List[Optional[dict]]:
+    """Run's per-face loop (FinalLightFace.cs:93-299).
 
     samples[f]: {'pos' (n,3), 'bounds' (n,4) minS minT maxS maxT in face f's
     luxel space, 'light' (nb,n,3)}; luxels[f]: (numLuxels,3) world points;
     patches: None when there is no bounce, else {'origin' (P,3), 'winding'
     list of (k,3), 'face' (P,), 'leaf' (P,) bool, 'total' (P,4,3) TotalLight
-    per normal}.  Returns per face {'luxels' (nb, numLuxels, 3) after
-    minlight, 'median' (3,)}, or None for a face that is not lit (or not in
-    ``only``, a set of faces to run when given)."""
+    per normal}.  ``styles`` None: light style 0 alone; returns per face
+    {'luxels' (nb, numLuxels, 3) after minlight, 'median' (3,)}.  ``styles``
+    (F, 4) style numbers per slot (255 empty): samples[f]['light'] is then a
+    list over the face's slots, each (nb,n,3); every style is lit in turn
+    (a neighbour's samples of the slot with the same style NUMBER, bounced
+    light in style 0 only) and each face gives {'luxels' (nstyles, nb,
+    numLuxels, 3), 'median' (nstyles, 3)}.  None for a face that is not lit
+    (or not in ``only``, a set of faces to run when given)."""
     out: List[Optional[dict]] = []
     plist = None
     if patches is not None:
         plist = face_patch_lists(len(faces), patches["face"], patches["leaf"])
+
+    def nstyles(f):
+        if styles is None:
+            return 1
+        return next((k for k in range(4) if styles[f][k] == 255), 4)
+
+    def slot_light(f, k):
+        return samples[f]["light"] if styles is None else samples[f]["light"][k]
+
     for fn, face in enumerate(faces):
-        if not face.lit or (only is not None and fn not in only):
+        if not face.lit or nstyles(fn) == 0 or (only is not None and fn not in only):
             out.append(None)
             continue
-        nb = face.nb
-        bumped = nb == 4
-        rad = Radial(face)
-        sm = samples[fn]
-        # BuildLuxelRadial (FinalLightFace.cs:301-389): own samples, then each
-        # neighbour's, their bounds carried corner by corner into this grid
-        for k in range(len(sm["pos"])):
-            rad.add_direct(sm["pos"][k], *sm["bounds"][k], [sm["light"][b][k] for b in range(nb)],
-                           bumped, bumped)
-        for n in neighbours[fn]:
-            nface, ns = faces[n], samples[n]
-            if not nface.lit or len(ns["pos"]) == 0:
-                continue
-            nbump = nface.nb == 4
-            for k in range(len(ns["pos"])):
-                b0 = ns["bounds"][k]
-                tmp = nface.luxel_to_world(b0[0], b0[1])
-                mn_s, mn_t = face.world_to_luxel(tmp)
-                tmp = nface.luxel_to_world(b0[2], b0[3])
-                mx_s, mx_t = face.world_to_luxel(tmp)
-                light = [ns["light"][b][k] for b in range(nface.nb)]
-                rad.add_direct(ns["pos"][k], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
-        prad = None
-        if plist is not None:
-            # BuildPatchRadial (FinalLightFace.cs:404-416, 489-516): the face's
-            # leaf patches, then each neighbour's; a neighbour's patches are
-            # bumped when that neighbour is
-            prad = Radial(face)
-            for src in [fn] + list(neighbours[fn]):
-                nbump = bumped if src == fn else faces[src].nb == 4
-                for p in plist[src]:
-                    mn_s, mn_t, mx_s, mx_t = patch_range(face, patches["winding"][p])
-                    light = [np.asarray(patches["total"][p][b], np.float32) for b in range(4)]
-                    prad.add_bounced(patches["origin"][p], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
-        lum = np.zeros((nb, len(luxels[fn]), 3), np.float32)
-        red_, green_, blue_ = [], [], []
-        no_samples = len(sm["pos"]) == 0
-        for j in range(len(luxels[fn])):
-            lb, ok = rad.sample(luxels[fn][j], nb, red_errors)
-            if prad is not None:
-                v, _ = prad.sample(luxels[fn][j], nb, red_errors)
-                for b in range(nb):
-                    lb[b] = [f32(lb[b][k] + v[b][k]) for k in range(3)]
-            if no_samples:
-                lb = [[f32(255.0), f32(0.0), f32(0.0)] for _ in range(nb)]
-                ok = False
-            for b in range(nb):
-                c = lb[b]
-                c = [c[k] if c[k] > face.minlight else face.minlight for k in range(3)]
-                if b == 0 and ok:
-                    red_.append(c[0])
-                    green_.append(c[1])
-                    blue_.append(c[2])
-                lum[b, j] = c
-        med = np.zeros(3, np.float32)
-        if red_:
-            med = np.array([median(red_), median(green_), median(blue_)], np.float32)
-        out.append({"luxels": lum, "median": med})
+        per = [_light_style(faces, luxels, samples, neighbours, patches, plist, red_errors, fn, k,
+                            styles, slot_light) for k in range(nstyles(fn))]
+        if styles is None:
+            out.append(per[0])
+        else:
+            out.append({"luxels": np.stack([r["luxels"] for r in per]),
+                        "median": np.stack([r["median"] for r in per])})
     return out
+
+
+def _light_style(faces, luxels, samples, neighbours, patches, plist, red_errors, fn, k, styles, slot_light):
+    """One face's style slot k."""
+    face = faces[fn]
+    nb = face.nb
+    bumped = nb == 4
+    rad = Radial(face)
+    sm = samples[fn]
+    own = slot_light(fn, k)
+    wanted = 0 if styles is None else int(styles[fn][k])
+    # BuildLuxelRadial (FinalLightFace.cs:301-389): own samples, then each
+    # neighbour's samples of the same light style, their bounds carried
+    # corner by corner into this grid
+    for i in range(len(sm["pos"])):
+        rad.add_direct(sm["pos"][i], *sm["bounds"][i], [own[b][i] for b in range(nb)], bumped, bumped)
+    for n in neighbours[fn]:
+        nface, ns = faces[n], samples[n]
+        if styles is None:
+            nk = 0
+        else:
+            nk = next((j for j in range(4) if int(styles[n][j]) == wanted), -1)
+        if nk < 0 or not nface.lit or len(ns["pos"]) == 0:
+            continue
+        nl = slot_light(n, nk)
+        nbump = nface.nb == 4
+        for i in range(len(ns["pos"])):
+            b0 = ns["bounds"][i]
+            tmp = nface.luxel_to_world(b0[0], b0[1])
+            mn_s, mn_t = face.world_to_luxel(tmp)
+            tmp = nface.luxel_to_world(b0[2], b0[3])
+            mx_s, mx_t = face.world_to_luxel(tmp)
+            light = [nl[b][i] for b in range(nface.nb)]
+            rad.add_direct(ns["pos"][i], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
+    prad = None
+    if plist is not None and k == 0:
+        # BuildPatchRadial (FinalLightFace.cs:404-416, 489-516), style 0 only
+        # (:182): the face's leaf patches, then each neighbour's; a
+        # neighbour's patches are bumped when that neighbour is
+        prad = Radial(face)
+        for src in [fn] + list(neighbours[fn]):
+            nbump = bumped if src == fn else faces[src].nb == 4
+            for p in plist[src]:
+                mn_s, mn_t, mx_s, mx_t = patch_range(face, patches["winding"][p])
+                light = [np.asarray(patches["total"][p][b], np.float32) for b in range(4)]
+                prad.add_bounced(patches["origin"][p], mn_s, mn_t, mx_s, mx_t, light, bumped, nbump)
+    lum = np.zeros((nb, len(luxels[fn]), 3), np.float32)
+    red_, green_, blue_ = [], [], []
+    no_samples = len(sm["pos"]) == 0
+    for j in range(len(luxels[fn])):
+        lb, ok = rad.sample(luxels[fn][j], nb, red_errors)
+        if prad is not None:
+            v, _ = prad.sample(luxels[fn][j], nb, red_errors)
+            for b in range(nb):
+                lb[b] = [f32(lb[b][c] + v[b][c]) for c in range(3)]
+        if no_samples:
+            lb = [[f32(255.0), f32(0.0), f32(0.0)] for _ in range(nb)]
+            ok = False
+        for b in range(nb):
+            c = lb[b]
+            c = [c[q] if c[q] > face.minlight else face.minlight for q in range(3)]
+            if b == 0 and ok:
+                red_.append(c[0])
+                green_.append(c[1])
+                blue_.append(c[2])
+            lum[b, j] = c
+    med = np.zeros(3, np.float32)
+    if red_:
+        med = np.array([median(red_), median(green_), median(blue_)], np.float32)
+    return {"luxels": lum, "median": med}
 
 
 def median(values) -> np.float32:
@@ -295,31 +328,35 @@ def median(values) -> np.float32:
     return v[len(v) >> 1]
 
 
-def layout(faces: Sequence[Face]):
-    """LightmapOffsets.Compute for one style (Rad/Light/LightmapOffsets.cs):
-    each lit face reserves its average-colour word, then 4 bytes per luxel
-    per normal; its light offset is just past the average word.  Returns
-    (offsets, -1 for a face not lit; total bytes)."""
+def layout(faces: Sequence[Face], styles=None):
+    """LightmapOffsets.Compute (Rad/Light/LightmapOffsets.cs:60-123): each lit
+    face reserves an average-colour word per style, then 4 bytes per luxel
+    per normal per style; its light offset is just past the average words.
+    ``styles`` (F, 4) or None (style 0 alone).  Returns (offsets, -1 for a
+    face not lit; total bytes)."""
     offs, size = [], 0
-    for face in faces:
-        if not face.lit:
+    for f, face in enumerate(faces):
+        ns = 1 if styles is None else next((k for k in range(4) if styles[f][k] == 255), 4)
+        if not face.lit or ns == 0:
             offs.append(-1)
             continue
-        size += 4
+        size += 4 * ns
         offs.append(size)
-        size += 4 * face.nb * face.w * face.h
+        size += 4 * face.nb * face.w * face.h * ns
     return offs, size
 
 
 def lump_bytes(faces: Sequence[Face], result: Sequence[Optional[dict]]) -> bytes:
-    """The stage's lump for one style: for each lit face in order, the
-    average colour word, then its luxels bump map by bump map (the face's
-    light offset is 4 bytes past its start: FinalLightFace.cs:270, 279)."""
+    """The stage's lump: for each lit face in order, the average colour words
+    in reverse style order, then its luxels style by style, bump map by bump
+    map (the face's light offset is just past the average words; style k's
+    median sits 4 (k + 1) bytes before it: FinalLightFace.cs:270, 279)."""
     parts = []
     for face, r in zip(faces, result):
         if r is None:
             continue
-        parts.append(encode_rgbe(r["median"].reshape(1, 3)).tobytes())
+        med = r["median"].reshape(-1, 3)
+        parts.append(encode_rgbe(med[::-1].copy()).tobytes())
         parts.append(encode_rgbe(r["luxels"].reshape(-1, 3)).tobytes())
     return b"".join(parts)
 
@@ -335,10 +372,11 @@ FACE_DTYPE = np.dtype([
 
 def final_light_c(faces: np.ndarray, luxel_pos, sample_bounds, nbr_ptr, nbr, face_sample_ptr,
                   face_normal_count, sample_pos, light, patches: Optional[dict] = None,
-                  patch_light=None, red_errors: bool = False, threads: int = 1):
+                  patch_light=None, red_errors: bool = False, threads: int = 1, face_styles=None):
     """vc_final_light over whole maps: faces (FACE_DTYPE), the luxels, the
     samples' bounds, neighbour CSR, the gather's faces, sample points and
-    light ((4, S, 4)), and for a bounce the patches ({'origin', 'face',
+    light ((4, S, 4), or (slots, 4, S, 4) with ``face_styles`` (F, 4) style
+    numbers per slot), and for a bounce the patches ({'origin', 'face',
     'child1', 'winding_ptr', 'winding'}) with their TotalLight per normal
     (npatch, 4, 3).  Returns (lump bytes as uint8, offsets)."""
     import ctypes
@@ -354,7 +392,7 @@ def final_light_c(faces: np.ndarray, luxel_pos, sample_bounds, nbr_ptr, nbr, fac
                     ("patch_face", P), ("patch_child1", P), ("winding_ptr", P), ("winding", P),
                     ("face_sample_ptr", P), ("face_normal_count", P), ("sample_pos", P),
                     ("nsamples", ctypes.c_int32), ("light", P), ("patch_light", P),
-                    ("red_errors", ctypes.c_int32)]
+                    ("red_errors", ctypes.c_int32), ("face_styles", P), ("light_slots", ctypes.c_int32)]
 
     keep = []
 
@@ -380,6 +418,10 @@ def final_light_c(faces: np.ndarray, luxel_pos, sample_bounds, nbr_ptr, nbr, fac
     d.nsamples = int(np.asarray(sample_pos).shape[0])
     d.light = c(light, np.float32)
     d.red_errors = 1 if red_errors else 0
+    d.light_slots = 1
+    if face_styles is not None:
+        d.face_styles = c(face_styles, np.uint8)
+        d.light_slots = int(np.asarray(light).reshape(-1, 4, int(np.asarray(sample_pos).shape[0]), 4).shape[0])
     if patches is not None and patch_light is not None:
         d.npatch = len(patches["origin"])
         d.patch_origin = c(patches["origin"], np.float32)

@@ -134,13 +134,17 @@ def _point_faces(fset, sel_face, points):
 
 def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames, light,
                 extra_passes: int = EXTRA_PASSES, sample_st=None, garbage: bool = False, which: int = 0,
-                threads=None):
+                threads=None, face_style=None):
     """The passes over one gather result ``light`` ((4, S, 4), the LDR or HDR
     result named by ``which``, which picks the resample's matching output).
     ``frames`` are the faces' luxel frames (final.FACE_DTYPE; width x height
     luxels); ``sample_st`` (S, 2) each sample's luxel, or None for sample k
-    of a face at (k % width, k // width).  Returns (refined light, the
-    number of samples each pass resampled)."""
+    of a face at (k % width, k // width).  ``face_style`` (F,) makes this
+    one style slot's StyleSupersample (FaceLightJob.cs:688-700): per face
+    the slot's light style, 255 where the face has no such slot (its samples
+    are never selected); the resample then takes only that style's lights
+    (EmitResample, FaceLightJob.cs:486).  Returns (refined light, the number
+    of samples each pass resampled)."""
     from hl2sdk_tools_sharp_amd.light import pack_desc
 
     light = np.array(light, np.float32, copy=True)
@@ -165,7 +169,10 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
     counts = []
     for _pass in range(1, extra_passes + 1):
         g = gradients(inten, lux, lux_ptr, face, s, t, width, height, nc, garbage)
-        sel = np.nonzero(~processed & ~(g < GRADIENT))[0]
+        pick = ~processed & ~(g < GRADIENT)
+        if face_style is not None:
+            pick &= np.asarray(face_style)[face] != 255
+        sel = np.nonzero(pick)[0]
         if len(sel) == 0:
             break
         processed[sel] = True
@@ -176,7 +183,13 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
             sub = _point_faces(fset, face[sel], pts)
             sub.sample_cluster[:] = np.repeat(fset.sample_cluster[sel], pts.shape[1])
             desc = pack_desc(sub, lights, light_pvs, sun_extent, sun_samples, sky_dirs)
-            ldr, hdr, _ = gather(casters, desc, sub.nsamples, threads, light_filter=filt, no_skip=True)
+            fsf = None
+            if face_style is not None:
+                sub_faces = np.unique(face[sel], return_index=True)
+                sub_faces = sub_faces[0][np.argsort(sub_faces[1])]
+                fsf = np.asarray(face_style, np.int32)[sub_faces]
+            ldr, hdr, _ = gather(casters, desc, sub.nsamples, threads, light_filter=filt, no_skip=True,
+                                 face_style_filter=fsf)
             out = hdr if which else ldr
             k = pts.shape[1]
             acc = np.zeros((4, len(sel), 4), np.float32)
@@ -190,3 +203,26 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
         light[:, sel] = np.where(live[..., None], v, cur)
         inten[:, sel] = intensity(light[:, sel], nc[sel])
     return light, counts
+
+
+def supersample_styles(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames, slots,
+                       face_styles, extra_passes: int = EXTRA_PASSES, sample_st=None, garbage: bool = False,
+                       which: int = 0, threads=None):
+    """Every style slot's passes: ``slots`` (4 slots, 4 normals, S, 4) the
+    gather's light per slot, ``face_styles`` (F, 4) its style numbers.  Each
+    slot is a StyleSupersample of its own (its own selection and passes).
+    Returns (refined slots, samples re-lit per pass summed over the slots)."""
+    out = np.array(slots, np.float32, copy=True)
+    total = []
+    for k in range(4):
+        fs = np.asarray(face_styles)[:, k]
+        if not np.any(fs != 255):
+            continue
+        out[k], counts = supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames,
+                                     out[k], extra_passes, sample_st, garbage, which, threads, face_style=fs)
+        for i, n in enumerate(counts):
+            if i < len(total):
+                total[i] += n
+            else:
+                total.append(n)
+    return out, total

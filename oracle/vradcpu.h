@@ -217,9 +217,13 @@ typedef struct vc_final_desc {
     const int32_t *face_sample_ptr, *face_normal_count;
     const float *sample_pos;
     int32_t nsamples;
-    const float *light;
+    const float *light;  /* light_slots x [4][nsamples][4] */
     const float *patch_light;
     int32_t red_errors;
+    /* Light styles: 4 style numbers per face (255 empty; slot 0 style 0),
+     * or NULL for style 0 alone; light then holds light_slots slots. */
+    const uint8_t *face_styles;
+    int32_t light_slots;
 } vc_final_desc;
 
 /* The lump size; offsets (nfaces, may be NULL) each lit face's light offset, -1 otherwise. */

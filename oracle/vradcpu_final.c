@@ -4,7 +4,9 @@
  * Rad/Final/LuxelRadial.cs).  TEST INFRASTRUCTURE ONLY (see vradcpu.h).
  *
  * The same statement as oracle/final.py, in C for maps of realistic size:
- * one style, no displacements, macro textures the identity, the Correct
+ * every light style of a face in turn (the loop over its slots; neighbours
+ * matched by style number; bounced light into style 0 only), no
+ * displacements, macro textures the identity, the Correct
  * sides of SampleRadialEdgeOffByOne and PatchRadialNeighbourBumpFromSelf.
  * Single precision in the reference's order (built -ffp-contract=off); the
  * reference's double promotions in double.  Faces are independent (each
@@ -196,19 +198,31 @@ static int cmp_float(const void *a, const void *b)
     return x < y ? -1 : (x > y ? 1 : 0);
 }
 
+/* The face's light styles: its slots up to the first empty one. */
+static int face_style_count(const vc_final_desc *d, int32_t f)
+{
+    if (!d->face_styles) return 1;
+    int k = 0;
+    while (k < 4 && d->face_styles[4 * (int64_t)f + k] != 255) k++;
+    return k;
+}
+
+/* LightmapOffsets.Compute (Rad/Light/LightmapOffsets.cs:60-123): per lit
+ * face, a median word per style, then 4 bytes per luxel per normal per style. */
 int64_t vc_final_layout(const vc_final_desc *d, int32_t *offsets)
 {
     int64_t size = 0;
     for (int32_t f = 0; f < d->nfaces; f++) {
         const vc_final_face *x = &d->faces[f];
-        if (!x->lit) {
+        const int ns = face_style_count(d, f);
+        if (!x->lit || ns == 0) {
             if (offsets) offsets[f] = -1;
             continue;
         }
-        size += 4;
+        size += 4 * ns;
         if (offsets) offsets[f] = (int32_t)size;
         int nb = d->face_normal_count[f] == 4 ? 4 : 1;
-        size += 4 * (int64_t)nb * x->width * x->height;
+        size += 4 * (int64_t)nb * x->width * x->height * ns;
     }
     return size;
 }
@@ -223,28 +237,43 @@ typedef struct {
     pthread_mutex_t mu;
 } final_job;
 
-static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, float *lb, float *pv,
-                       float *vals)
+/* The slot of face f holding light style `style`, or -1 (the neighbour
+ * match of BuildLuxelRadial, FinalLightFace.cs:340-360). */
+static int slot_of_style(const vc_final_desc *d, int32_t f, int style)
+{
+    if (!d->face_styles) return style == 0 ? 0 : -1;
+    for (int k = 0; k < 4; k++)
+        if (d->face_styles[4 * (int64_t)f + k] == style) return k;
+    return -1;
+}
+
+/* FinalLightFace.Run's body for style slot `slot` of face fn. */
+static void light_face_style(final_job *j, int32_t fn, int slot, radial *rad, radial *prad, float *lb,
+                             float *pv, float *vals)
 {
     const vc_final_desc *d = j->d;
     const vc_final_face *face = &d->faces[fn];
     const int nb = d->face_normal_count[fn] == 4 ? 4 : 1, bumped = nb == 4;
     const int32_t S = d->nsamples;
+    const int style = d->face_styles ? d->face_styles[4 * (int64_t)fn + slot] : 0;
     const float *lp[4];
     radial_reset(rad, face, nb);
     /* BuildLuxelRadial (FinalLightFace.cs:301-389) */
+    const float *own = d->light + (int64_t)slot * 4 * S * 4;
     for (int32_t k = d->face_sample_ptr[fn]; k < d->face_sample_ptr[fn + 1]; k++) {
-        for (int b = 0; b < 4; b++) lp[b] = d->light + ((int64_t)b * S + k) * 4;
+        for (int b = 0; b < 4; b++) lp[b] = own + ((int64_t)b * S + k) * 4;
         const float *bd = d->sample_bounds + (int64_t)k * 4;
         add_direct(rad, d->sample_pos + (int64_t)k * 3, bd[0], bd[1], bd[2], bd[3], lp, bumped, bumped);
     }
     for (int32_t q = d->nbr_ptr[fn]; q < d->nbr_ptr[fn + 1]; q++) {
         int32_t n = d->nbr[q];
         const vc_final_face *nf = &d->faces[n];
-        if (!nf->lit || d->face_sample_ptr[n + 1] == d->face_sample_ptr[n]) continue;
+        const int nslot = slot_of_style(d, n, style);  /* matched by style number */
+        if (nslot < 0 || !nf->lit || d->face_sample_ptr[n + 1] == d->face_sample_ptr[n]) continue;
+        const float *nl = d->light + (int64_t)nslot * 4 * S * 4;
         int nbump = d->face_normal_count[n] == 4;
         for (int32_t k = d->face_sample_ptr[n]; k < d->face_sample_ptr[n + 1]; k++) {
-            for (int b = 0; b < 4; b++) lp[b] = d->light + ((int64_t)b * S + k) * 4;
+            for (int b = 0; b < 4; b++) lp[b] = nl + ((int64_t)b * S + k) * 4;
             const float *bd = d->sample_bounds + (int64_t)k * 4;
             float tmp[3], mns, mnt, mxs, mxt;
             l2w(nf, bd[0], bd[1], tmp);
@@ -254,7 +283,8 @@ static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, floa
             add_direct(rad, d->sample_pos + (int64_t)k * 3, mns, mnt, mxs, mxt, lp, bumped, nbump);
         }
     }
-    const int bounce = d->patch_light != NULL;
+    /* bounced light goes into style 0 only (FinalLightFace.cs:182) */
+    const int bounce = d->patch_light != NULL && slot == 0;
     if (bounce) {
         /* BuildPatchRadial (FinalLightFace.cs:404-416, 489-536) */
         radial_reset(prad, face, nb);
@@ -280,7 +310,7 @@ static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, floa
     const int64_t n = (int64_t)face->width * face->height;
     const int no_samples = d->face_sample_ptr[fn + 1] == d->face_sample_ptr[fn];
     const float ml = face->minlight;
-    uint8_t *out = j->lump + j->light_ofs[fn];
+    uint8_t *out = j->lump + j->light_ofs[fn] + (int64_t)slot * nb * n * 4;
     int64_t cnt = 0;
     for (int64_t jx = 0; jx < n; jx++) {
         float grid[3];
@@ -323,7 +353,15 @@ static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, floa
             med[k] = vals[k * n + (cnt >> 1)];
         }
     }
-    vc_encode_rgbe(med, 1, 3, out - 4);
+    /* stored before lightofs, in reverse style order (:270-279) */
+    vc_encode_rgbe(med, 1, 3, j->lump + j->light_ofs[fn] - 4 * (slot + 1));
+}
+
+static void light_face(final_job *j, int32_t fn, radial *rad, radial *prad, float *lb, float *pv,
+                       float *vals)
+{
+    const int ns = face_style_count(j->d, fn);
+    for (int k = 0; k < ns; k++) light_face_style(j, fn, k, rad, prad, lb, pv, vals);
 }
 
 static void *final_worker(void *arg)
@@ -342,7 +380,7 @@ static void *final_worker(void *arg)
         int32_t f = j->next++;
         pthread_mutex_unlock(&j->mu);
         if (f >= j->d->nfaces) break;
-        if (!j->d->faces[f].lit) continue;
+        if (!j->d->faces[f].lit || j->light_ofs[f] < 0) continue;
         light_face(j, f, &rad, &prad, lb, pv, vals);
     }
     free(rad.wt);
@@ -369,10 +407,11 @@ int vc_final_light(const vc_final_desc *d, uint8_t *lump, int64_t capacity, int3
         lux += n;
         light_ofs[f] = -1;
         if (n > max_cells) max_cells = n;
-        if (!x->lit) continue;
-        size += 4;
+        const int ns = face_style_count(d, f);
+        if (!x->lit || ns == 0) continue;
+        size += 4 * ns;
         light_ofs[f] = size;
-        size += 4 * (int64_t)(d->face_normal_count[f] == 4 ? 4 : 1) * n;
+        size += 4 * (int64_t)(d->face_normal_count[f] == 4 ? 4 : 1) * n * ns;
     }
     if (size > capacity) {
         free(lux_base);

@@ -26,9 +26,10 @@ def oracle_faces(ff: FinalFaces, fset: FaceSet):
 
 
 def oracle_inputs(ff: FinalFaces, fset: FaceSet, light: np.ndarray, fp: FinalPatches = None,
-                  patch_total: np.ndarray = None):
+                  patch_total: np.ndarray = None, face_styles=None):
     """(faces, luxels, samples, neighbours, patches) for oracle.final.final_light;
-    ``light`` is the gather's (4, S, 4) result the device reads."""
+    ``light`` is the gather's (4, S, 4) result the device reads, or with
+    ``face_styles`` (F, 4) its (4 slots, 4, S, 4) light per style slot."""
     faces = oracle_faces(ff, fset)
     n = ff.faces["width"].astype(np.int64) * ff.faces["height"]
     lb = np.concatenate([[0], np.cumsum(n)])
@@ -40,8 +41,11 @@ def oracle_inputs(ff: FinalFaces, fset: FaceSet, light: np.ndarray, fp: FinalPat
     samples = []
     for f, face in enumerate(faces):
         s0, s1 = fset.face_sample_ptr[f], fset.face_sample_ptr[f + 1]
-        samples.append({"pos": fset.sample_pos[s0:s1], "bounds": ff.sample_bounds[s0:s1],
-                        "light": light[:face.nb, s0:s1, :3]})
+        if face_styles is None:
+            lt = light[:face.nb, s0:s1, :3]
+        else:
+            lt = [light[k][:face.nb, s0:s1, :3] for k in range(4)]
+        samples.append({"pos": fset.sample_pos[s0:s1], "bounds": ff.sample_bounds[s0:s1], "light": lt})
     nbrs = [list(ff.nbr[ff.nbr_ptr[f]:ff.nbr_ptr[f + 1]]) for f in range(len(faces))]
     patches = None
     if fp is not None:
@@ -58,7 +62,20 @@ def face_bytes(faces, result, offsets):
     for f, r in enumerate(result):
         if r is None:
             continue
-        out[f] = (int(offsets[f]), fo.lump_bytes([faces[f]], [r]))
+        nst = 1 if r["median"].ndim == 1 else r["median"].shape[0]
+        out[f] = (int(offsets[f]), fo.lump_bytes([faces[f]], [r]), nst)
+    return out
+
+
+def random_styles(nfaces: int, seed: int) -> np.ndarray:
+    """(F, 4) style numbers: slot 0 style 0, then up to three of a few
+    styles in a random order per face."""
+    rng = np.random.default_rng(seed)
+    out = np.full((nfaces, 4), 255, np.uint8)
+    out[:, 0] = 0
+    for f in range(nfaces):
+        k = int(rng.integers(0, 4))
+        out[f, 1:1 + k] = rng.choice([3, 5, 9, 33], size=k, replace=False)
     return out
 
 

@@ -29,8 +29,8 @@ def open_all(casters):
 
 def check_lump(lump, size, want):
     assert lump.shape[0] == size
-    for f, (ofs, b) in want.items():
-        got = lump[ofs - 4:ofs - 4 + len(b)].tobytes()
+    for f, (ofs, b, nst) in want.items():
+        got = lump[ofs - 4 * nst:ofs - 4 * nst + len(b)].tobytes()
         if got != b:
             g = np.frombuffer(got, np.uint8).reshape(-1, 4)
             e = np.frombuffer(b, np.uint8).reshape(-1, 4)

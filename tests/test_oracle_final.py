@@ -279,3 +279,79 @@ def test_the_c_statement_equals_the_python_one(seed, red, bounce, grid):
                                  pd if bounce else None, total, red, threads=3)
     assert list(offs) == fo.layout(faces)[0]
     assert got.tobytes() == want
+
+
+# --- light styles: Run's loop over a face's styles (FinalLightFace.cs:114-125,155,182,270-279) ---
+
+def _styled(face, lights):
+    """Grid samples whose light per slot is uniform ``lights[k]``."""
+    sm = _grid_samples(face, lights[0])
+    n = len(sm["pos"])
+    sm["light"] = [np.broadcast_to(np.asarray(c, np.float32), (face.nb, n, 3)).copy() for c in lights]
+    return sm
+
+
+def test_each_style_is_its_own_map_and_the_medians_sit_before_lightofs_in_reverse_order():
+    from oracle.gather import encode_rgbe
+
+    face = floor_face(width=4, height=3)
+    c0, c1 = [2.0, 4.0, 8.0], [1.0, 0.5, 0.25]
+    styles = np.array([[0, 5, 255, 255]], np.uint8)
+    res = fo.final_light([face], [_luxels(face)], [_styled(face, [c0, c1])], [[]], styles=styles)
+    assert res[0]["luxels"].shape == (2, 1, 12, 3)
+    assert np.all(res[0]["luxels"][0] == np.float32(c0)) and np.all(res[0]["luxels"][1] == np.float32(c1))
+    offs, size = fo.layout([face], styles)
+    assert offs == [8] and size == 8 + 2 * 4 * 12
+    lump = fo.lump_bytes([face], res)
+    e0, e1 = encode_rgbe(np.array([c0], np.float32)).tobytes(), encode_rgbe(np.array([c1], np.float32)).tobytes()
+    assert lump[0:4] == e1 and lump[4:8] == e0  # lightofs - 8: style 1, lightofs - 4: style 0
+    assert lump[8:8 + 48] == e0 * 12 and lump[8 + 48:] == e1 * 12
+
+
+def test_a_neighbours_samples_are_matched_by_style_number_not_slot():
+    """A (styles 0, 5) beside B (styles 0, 7, 5): A's style-5 map takes B's
+    slot 2, so a uniform style-5 light stays uniform across the seam; B's
+    slot 1 (style 7, ten times brighter) never reaches it."""
+    a, b = floor_face(width=4, height=4), floor_face(width=4, height=4, min_s=4)
+    v = [1.0, 2.0, 3.0]
+    sa = _styled(a, [[0, 0, 0], v])
+    sb = _styled(b, [[0, 0, 0], [10.0, 20.0, 30.0], v])
+    styles = np.array([[0, 5, 255, 255], [0, 7, 5, 255]], np.uint8)
+    res = fo.final_light([a, b], [_luxels(a), _luxels(b)], [sa, sb], [[1], [0]], styles=styles)
+    np.testing.assert_allclose(res[0]["luxels"][1, 0], np.broadcast_to(np.float32(v), (16, 3)), rtol=1e-6)
+    np.testing.assert_allclose(res[1]["luxels"][2, 0], np.broadcast_to(np.float32(v), (16, 3)), rtol=1e-6)
+    # matched by slot instead, A's seam column would take B's style-7 light
+    wrong = fo.final_light([a, b], [_luxels(a), _luxels(b)], [sa, dict(sb, light=[sb["light"][0], sb["light"][1]])],
+                           [[1], [0]], styles=np.array([[0, 5, 255, 255], [0, 5, 255, 255]], np.uint8))
+    assert np.any(wrong[0]["luxels"][1, 0, :, 0] > 1.5)
+
+
+def test_bounced_light_goes_into_style_zero_only():
+    face = floor_face(width=9, height=9)
+    sm = _styled(face, [[1, 1, 1], [2, 2, 2]])
+    corners = [face.luxel_to_world(s, t) for s, t in [(2, 2), (4, 2), (4, 4), (2, 4)]]
+    patches = {"origin": np.array([face.luxel_to_world(3, 3)], np.float32), "winding": [np.array(corners)],
+               "face": np.array([0]), "leaf": np.array([True]), "total": np.full((1, 4, 3), 6, np.float32)}
+    res = fo.final_light([face], [_luxels(face)], [sm], [[]], patches, styles=np.array([[0, 4, 255, 255]], np.uint8))
+    j = 3 + 3 * 9
+    assert res[0]["luxels"][0, 0, j, 0] == 7.0 and res[0]["luxels"][1, 0, j, 0] == 2.0
+
+
+@pytest.mark.parametrize("seed,bounce", [(7, True), (8, False)])
+def test_the_c_statement_equals_the_python_one_with_styles(seed, bounce):
+    from final_cases import oracle_inputs, random_case, random_styles
+
+    _, _, fset, ff, fp = random_case(seed)
+    styles = random_styles(fset.nfaces, seed)
+    rng = np.random.default_rng(seed + 20)
+    light = rng.uniform(0, 8, (4, 4, fset.nsamples, 4)).astype(np.float32)
+    total = rng.uniform(0, 30, (fp.npatch, 4, 3)).astype(np.float32) if bounce else None
+    faces, lux, sm, nb, pt = oracle_inputs(ff, fset, light, fp if bounce else None, total, face_styles=styles)
+    want = fo.lump_bytes(faces, fo.final_light(faces, lux, sm, nb, pt, styles=styles))
+    pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
+          "winding": fp.winding}
+    got, offs = fo.final_light_c(ff.faces, ff.luxel_pos, ff.sample_bounds, ff.nbr_ptr, ff.nbr,
+                                 fset.face_sample_ptr, fset.face_normal_count, fset.sample_pos, light,
+                                 pd if bounce else None, total, threads=3, face_styles=styles)
+    assert list(offs) == fo.layout(faces, styles)[0]
+    assert got.tobytes() == want
