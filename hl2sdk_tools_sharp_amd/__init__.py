@@ -14,6 +14,7 @@ from .tracer import (  # noqa: F401
     HipTracerFactory,
     NotSupportedError,
     ObjectDisposedError,
+    OperationCanceledError,
     RayTraceOptions,
     ShadowCasterSet,
     ray_segment,
@@ -29,6 +30,7 @@ __all__ = [
     "HipTracerFactory",
     "NotSupportedError",
     "ObjectDisposedError",
+    "OperationCanceledError",
     "RayTraceOptions",
     "ShadowCasterSet",
     "ray_segment",

@@ -121,6 +121,9 @@ SIGNATURES = {
     "vh_trace_point_lights_dev": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
     "vh_trace_point_lights_noalloc": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
     "vh_sync": (ctypes.c_int, [P, P]),
+    "vh_set_timeout": (ctypes.c_int, [P, I64]),
+    "vh_cancel": (ctypes.c_int, [P]),
+    "vh_cancel_clear": (ctypes.c_int, [P]),
     "vh_get_stats": (ctypes.c_int, [P, ctypes.POINTER(VhStats)]),
     "vh_patches_load": (ctypes.c_int, [P, ctypes.POINTER(VhPatchDesc)]),
     "vh_transfers_build": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]),

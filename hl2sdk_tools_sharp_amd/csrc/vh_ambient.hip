@@ -507,7 +507,7 @@ int vh_ambient_load(vh_ctx *c, const vh_ambient_desc *d)
                            (const AmbientFaceDev *)D->faces.ptr, d->nfaces, (const uint32_t *)D->lump.ptr,
                            (float3 *)D->avg.ptr);
     VH_HIP(hipGetLastError());
-    VH_HIP(hipStreamSynchronize(st));
+    VH_WAIT(c, st);
     return VH_OK;
 }
 
@@ -561,7 +561,7 @@ int vh_ambient_cubes(vh_ctx *c, const float *points, int64_t n, float *cubes)
     }
     VH_HIP(hipGetLastError());
     VH_HIP(hipMemcpyAsync(cubes, D->cubes.ptr, (size_t)n * 72, hipMemcpyDeviceToHost, st));
-    VH_HIP(hipStreamSynchronize(st));
+    VH_WAIT(c, st);
     return VH_OK;
 }
 

@@ -1187,7 +1187,7 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
             (rc = B->total_b.ensure(v4 * 3, "bump totals")))
             return rc;
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     return VH_OK;
 }
 
@@ -1237,9 +1237,10 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
                            (int *)err.ptr, pdepth);
     VH_HIP(hipGetLastError());
     std::vector<int64_t> hcount(R + 1, 0), hoff(R + 1, 0);
+    VH_WAIT(c, s);  // bounded: the copy below then finds the kernels done
     if (R > 0)
         VH_HIP(hipMemcpyAsync(hcount.data(), counts.ptr, (size_t)R * 8, hipMemcpyDeviceToHost, s));
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     for (int32_t k = 0; k < R; k++) hoff[k + 1] = hoff[k] + hcount[k];
     int64_t ncand = hoff[R];
     const size_t nslot = (size_t)(ncand > 0 ? ncand : 1);
@@ -1275,11 +1276,12 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
     std::vector<int64_t> hmade(R + 1, 0);
     int herr = 0;
     unsigned long long hstats[2] = {0, 0};
+    VH_WAIT(c, s);  // bounded: the copy below then finds the kernels done
     if (R > 0)
         VH_HIP(hipMemcpyAsync(hmade.data(), made.ptr, (size_t)R * 8, hipMemcpyDeviceToHost, s));
     VH_HIP(hipMemcpyAsync(&herr, err.ptr, 4, hipMemcpyDeviceToHost, s));
     VH_HIP(hipMemcpyAsync(hstats, stats.ptr, 16, hipMemcpyDeviceToHost, s));
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     if (herr) {
         cleanup();
         set_error(herr & 1 ? "patch descent deeper than the stack"
@@ -1306,7 +1308,7 @@ int vh_transfers_build(vh_ctx *c, int64_t *transfers, int64_t *candidates, int64
         cleanup();
         return rc;
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     cleanup();
     B->ntransfers = total;
     B->have_transfers = true;
@@ -1365,7 +1367,7 @@ int vh_transfers_export(vh_ctx *c, int32_t *counts, void *records, int64_t capac
     if (counts) {
         std::vector<int64_t> rp((size_t)B->npatch + 1);
         VH_HIP(hipMemcpyAsync(rp.data(), B->row_ptr.ptr, rp.size() * 8, hipMemcpyDeviceToHost, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         for (int32_t p = 0; p < B->npatch; p++) {
             const int64_t n = rp[p + 1] - rp[p];
             if (n > INT32_MAX) {
@@ -1382,7 +1384,7 @@ int vh_transfers_export(vh_ctx *c, int32_t *counts, void *records, int64_t capac
                            (const int32_t *)B->src.ptr, (const float *)B->w32.ptr, T, (uint2 *)out.buf.ptr);
         VH_HIP(hipGetLastError());
         VH_HIP(hipMemcpyAsync(records, out.buf.ptr, (size_t)T * 8, hipMemcpyDeviceToHost, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
     }
     return VH_OK;
 }
@@ -1433,7 +1435,7 @@ int vh_transfers_load(vh_ctx *c, const int64_t *row_ptr, const int32_t *src, con
                            (const int64_t *)B->row_ptr.ptr, (const float *)B->w32.ptr,
                            (__half *)B->w16.ptr, (float *)B->rscale.ptr);
     VH_HIP(hipGetLastError());
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     B->ntransfers = T;
     B->have_transfers = true;
     return VH_OK;
@@ -1581,6 +1583,7 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     }
     int32_t it = 0;
     hipError_t failure = hipSuccess;
+    int wrc = VH_OK;
     for (;;) {
         if (graphed)
             failure = hipGraphLaunch(exec, s);
@@ -1588,8 +1591,14 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
             iteration(true);
             failure = hipGetLastError();
         }
-        if (failure == hipSuccess) failure = hipStreamSynchronize(s);
         if (failure != hipSuccess) break;
+        // bounded by the context's deadline and cancel flag; on either, the
+        // iteration in flight (well under a millisecond) ends before its
+        // graph is destroyed
+        if ((wrc = wait_stream(c, s)) != VH_OK) {
+            hipStreamSynchronize(s);
+            break;
+        }
         const double *r = B->h_result;
         double res = r[1] > 0 ? r[0] / r[1] : 0.0;
         if (residuals) residuals[it] = res;
@@ -1612,6 +1621,7 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
         set_error(std::string("bounce iteration failed: ") + hipGetErrorString(failure));
         return VH_EDEVICE;
     }
+    if (wrc != VH_OK) return wrc;
     // the parents' totals, pulled up once at the end as CollectLight leaves them
     for (int32_t lv = 0; lv < B->height; lv++) {
         int32_t a = B->level_ptr[lv], n = B->level_ptr[lv + 1] - a;
@@ -1625,11 +1635,11 @@ int vh_bounce(vh_ctx *c, const vh_bounce_opts *o, const float *emit0, const floa
     VH_HIP(hipGetLastError());
     if (total_out) {
         VH_HIP(hipMemcpyAsync(buf.data(), B->total.ptr, buf.size() * 4, hipMemcpyDeviceToHost, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         for (int32_t p = 0; p < P; p++)
             for (int k = 0; k < 3; k++) total_out[3 * p + k] = buf[4 * p + k];
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     if (iters) *iters = it;
     B->have_total = true;
     return VH_OK;
@@ -1657,7 +1667,7 @@ int vh_bounce_bump_totals(vh_ctx *c, float *out)
     std::vector<float4> buf((size_t)P * 3);
     VH_HIP(hipMemcpyAsync(buf.data(), B->total_b.ptr, buf.size() * sizeof(float4), hipMemcpyDeviceToHost,
                           c->stream));
-    VH_HIP(hipStreamSynchronize(c->stream));
+    VH_WAIT(c, c->stream);
     for (size_t i = 0; i < buf.size(); i++) {
         out[3 * i] = buf[i].x;
         out[3 * i + 1] = buf[i].y;
@@ -1716,11 +1726,11 @@ int vh_patch_lights(vh_ctx *c, int32_t which, const float *sample_area, float *d
     if (direct) {
         std::vector<float> buf((size_t)P * 4);
         VH_HIP(hipMemcpyAsync(buf.data(), d, buf.size() * 4, hipMemcpyDeviceToHost, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         for (int32_t p = 0; p < P; p++)
             for (int k = 0; k < 3; k++) direct[3 * p + k] = buf[4 * p + k];
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     B->have_direct = true;
     return VH_OK;
 }

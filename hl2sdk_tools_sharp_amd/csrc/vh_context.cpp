@@ -1,8 +1,10 @@
 // libvradhip C ABI: context lifetime, errors, scene load and the trace entry
 // points.  Kernels live in vh_trace.hip.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "vh_internal.h"
 
@@ -31,6 +33,54 @@ int DevBuf::ensure(size_t need, const char *what)
     }
     bytes = want;
     return VH_OK;
+}
+
+int check_cancel(vh_ctx *c)
+{
+    if (c->cancel.load(std::memory_order_acquire)) {
+        set_error("canceled: vh_cancel was called on this context (vh_cancel_clear resets it)");
+        return VH_ECANCELED;
+    }
+    return VH_OK;
+}
+
+int wait_stream(vh_ctx *c, hipStream_t s)
+{
+    hipEvent_t ev = nullptr;
+    VH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) {
+        hipEventDestroy(ev);
+        set_error(std::string("hipEventRecord: ") + hipGetErrorString(e));
+        return VH_EDEVICE;
+    }
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    const int64_t limit = c->timeout_ms.load(std::memory_order_relaxed);
+    int rc = VH_OK;
+    for (int spin = 0;; spin++) {
+        e = hipEventQuery(ev);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) {
+            set_error(std::string("waiting on the device: ") + hipGetErrorString(e));
+            rc = VH_EDEVICE;
+            break;
+        }
+        if ((rc = check_cancel(c)) != VH_OK) break;
+        const int64_t ms =
+            std::chrono::duration_cast<std::chrono::milliseconds>(clock::now() - t0).count();
+        if (limit >= 0 && ms >= limit) {
+            set_error("timed out after " + std::to_string(ms) + " ms waiting on the device (limit " +
+                      std::to_string(limit) + " ms, vh_set_timeout); the work goes on and later "
+                      "calls on this context queue behind it");
+            rc = VH_ETIMEOUT;
+            break;
+        }
+        // poll briskly for short kernels, then back off
+        if (spin > 2000) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    hipEventDestroy(ev);
+    return rc;
 }
 
 void DevBuf::release()
@@ -132,10 +182,12 @@ int trace_visibility_host(vh_ctx *c, const float *rays, int32_t rw, float shared
         if ((rc = launch_visibility(c, (const float *)c->rays_buf.ptr, rw, shared_reach, m, tmin,
                                     skip_id, sky, (uint64_t *)c->out_buf.ptr, c->stream)))
             return rc;
-        VH_HIP(hipMemcpyAsync(bits + s0 / 64, c->out_buf.ptr, (size_t)((m + 63) / 64) * 8,
-                              hipMemcpyDeviceToHost, c->stream));
+        // the slab's bits reach the caller only once its kernel is done (a
+        // timeout or cancel between slabs leaves the caller's memory alone)
+        VH_WAIT(c, c->stream);
+        VH_HIP(hipMemcpy(bits + s0 / 64, c->out_buf.ptr, (size_t)((m + 63) / 64) * 8,
+                         hipMemcpyDeviceToHost));
     }
-    VH_HIP(hipStreamSynchronize(c->stream));
     return VH_OK;
 }
 
@@ -155,12 +207,10 @@ int trace_closest_host(vh_ctx *c, const float *rays, int32_t rw, float shared_re
                                  skip_id, (int32_t *)c->out_buf.ptr, (float *)c->out2_buf.ptr,
                                  c->stream)))
             return rc;
-        VH_HIP(hipMemcpyAsync(surface + s0, c->out_buf.ptr, (size_t)m * 4, hipMemcpyDeviceToHost,
-                              c->stream));
-        VH_HIP(hipMemcpyAsync(fraction + s0, c->out2_buf.ptr, (size_t)m * 4,
-                              hipMemcpyDeviceToHost, c->stream));
+        VH_WAIT(c, c->stream);
+        VH_HIP(hipMemcpy(surface + s0, c->out_buf.ptr, (size_t)m * 4, hipMemcpyDeviceToHost));
+        VH_HIP(hipMemcpy(fraction + s0, c->out2_buf.ptr, (size_t)m * 4, hipMemcpyDeviceToHost));
     }
-    VH_HIP(hipStreamSynchronize(c->stream));
     return VH_OK;
 }
 
@@ -456,9 +506,8 @@ int vh_trace_point_lights(vh_ctx *c, const float *points, int64_t npoints, const
                                   (const float *)c->lights_buf.ptr, nlights, tmin,
                                   (uint64_t *)c->out_buf.ptr, c->stream)))
         return rc;
-    VH_HIP(hipMemcpyAsync(bits, c->out_buf.ptr, (size_t)words * 8, hipMemcpyDeviceToHost,
-                          c->stream));
-    VH_HIP(hipStreamSynchronize(c->stream));
+    VH_WAIT(c, c->stream);
+    VH_HIP(hipMemcpy(bits, c->out_buf.ptr, (size_t)words * 8, hipMemcpyDeviceToHost));
     c->stats.rays_visibility += npoints * (int64_t)nlights;
     c->stats.calls++;
     return VH_OK;
@@ -575,7 +624,31 @@ int vh_sync(vh_ctx *c, void *stream)
         return VH_EINVAL;
     }
     VH_HIP(hipSetDevice(c->device));
-    VH_HIP(hipStreamSynchronize(stream_of(c, stream)));
+    return wait_stream(c, stream_of(c, stream));
+}
+
+int vh_set_timeout(vh_ctx *c, int64_t ms)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    c->timeout_ms.store(ms < 0 ? -1 : ms, std::memory_order_relaxed);
+    return VH_OK;
+}
+
+int vh_cancel(vh_ctx *c)
+{
+    if (!c) return VH_EINVAL;
+    c->cancel.store(1, std::memory_order_release);
+    return VH_OK;
+}
+
+int vh_cancel_clear(vh_ctx *c)
+{
+    if (!c) return VH_EINVAL;
+    c->cancel.store(0, std::memory_order_release);
     return VH_OK;
 }
 

@@ -636,7 +636,7 @@ int final_layout(FinalData *D, const GatherView &gv, hipStream_t st)
         (rc = D->mvals.ensure((size_t)std::max<int64_t>(mv, 1) * sizeof(float4), "median values")) ||
         (rc = D->lump.ensure((size_t)std::max<int64_t>(bytes, 4), "lump")))
         return rc;
-    VH_HIP(hipStreamSynchronize(st));
+    VH_HIP(hipStreamSynchronize(st));  // the host vectors above must outlive the copies
     D->style_generation = gv.style_generation;
     return VH_OK;
 }
@@ -875,7 +875,7 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
         if (P > 0)
             VH_HIP(hipMemcpyAsync(D->plight.ptr, pl.data(), pl.size() * sizeof(float4),
                                   hipMemcpyHostToDevice, st));
-        VH_HIP(hipStreamSynchronize(st));
+        VH_WAIT(c, st);
     }
     FinalArgs a;
     a.faces = (const FaceDev *)D->faces.ptr;
@@ -913,8 +913,10 @@ int vh_final_light(vh_ctx *c, const vh_final_opts *o, const float *patch_light, 
             hipLaunchKernelGGL(k_final, dim3((uint32_t)D->nlit), dim3(kWave), 0, st, a);
         timer.done();
     }
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && D->lump_bytes > 0)
+    VH_HIP(hipGetLastError());
+    VH_WAIT(c, st);  // bounded: the lump reaches the caller only once it is made
+    hipError_t e = hipSuccess;
+    if (D->lump_bytes > 0)
         e = hipMemcpyAsync(lump, D->lump.ptr, (size_t)D->lump_bytes, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {

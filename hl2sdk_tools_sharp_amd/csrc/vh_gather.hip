@@ -984,7 +984,7 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
         (rc = G->out_ao.ensure((size_t)S * 4 + 16, "ambient occlusion")) ||
         (rc = G->ticket.ensure(16, "queue ticket")))
         return rc;
-    VH_HIP(hipStreamSynchronize(st));
+    VH_WAIT(c, st);
     return VH_OK;
 }
 
@@ -1056,13 +1056,16 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
                                (const int32_t *)G->slot_group.ptr, (const int32_t *)G->sample_face.ptr,
                                G->nsamples, G->nslots, (float4 *)G->out_ldr.ptr, (float4 *)G->out_hdr.ptr);
         VH_HIP(hipGetLastError());
-        VH_HIP(hipMemcpyAsync(G->h_face_styles.data(), G->face_styles.ptr, G->h_face_styles.size(),
-                              hipMemcpyDeviceToHost, st));
-        VH_HIP(hipMemcpyAsync(&dropped, G->dropped.ptr, 8, hipMemcpyDeviceToHost, st));
+    }
+    // the bounded wait first: the copies below then find the work done
+    VH_WAIT(c, st);
+    if (NG > 1) {
+        VH_HIP(hipMemcpy(G->h_face_styles.data(), G->face_styles.ptr, G->h_face_styles.size(),
+                         hipMemcpyDeviceToHost));
+        VH_HIP(hipMemcpy(&dropped, G->dropped.ptr, 8, hipMemcpyDeviceToHost));
     }
     unsigned long long rays = 0;
-    VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
-    VH_HIP(hipStreamSynchronize(st));
+    VH_HIP(hipMemcpy(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost));
     c->stats.rays_gather += (int64_t)rays;
     c->stats.calls++;
     G->dropped_styles = (int64_t)dropped;
@@ -1184,7 +1187,7 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
         }
         VH_HIP(hipGetLastError());
         VH_HIP(hipMemcpyAsync(counts.data(), B.counts.ptr, (size_t)F * 4, hipMemcpyDeviceToHost, st));
-        VH_HIP(hipStreamSynchronize(st));
+        VH_WAIT(c, st);
         off[0] = 0;
         for (int32_t f = 0; f < F; f++) off[f + 1] = off[f] + counts[f];
         const int32_t total = off[F];
@@ -1260,7 +1263,7 @@ int vh_gather_supersample(vh_ctx *c, const vh_supersample_desc *d, int32_t *pass
     }
     unsigned long long rays = 0;
     VH_HIP(hipMemcpyAsync(&rays, (unsigned long long *)G->ticket.ptr + 1, 8, hipMemcpyDeviceToHost, st));
-    VH_HIP(hipStreamSynchronize(st));
+    VH_WAIT(c, st);
     c->stats.rays_gather += (int64_t)rays;
     return VH_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -70,6 +71,10 @@ struct vh_ctx {
     int cu_count = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     std::mutex mu;
+    // bounded waits: a blocking call gives up after timeout_ms (VH_ETIMEOUT)
+    // and at a cancel request (VH_ECANCELED); the work goes on on the device
+    std::atomic<int64_t> timeout_ms{120000};
+    std::atomic<int> cancel{0};
     std::string identity;
     // scene
     vh::DevBuf nodes, tris;
@@ -185,6 +190,22 @@ bool final_lump(vh_ctx *c, FinalLump *v);
 // Frees the leaf-ambient buffers (vh_ambient.hip).
 void ambient_release(vh_ctx *c);
 }  // namespace vh
+
+namespace vh {
+// Waits for the work enqueued on `s` so far, bounded by the context's
+// deadline and cancel flag (VH_OK, VH_ETIMEOUT, VH_ECANCELED or VH_EDEVICE).
+// On a timeout or cancel the work keeps running on the device; it writes
+// only the context's own buffers, and later calls queue behind it.
+int wait_stream(vh_ctx *c, hipStream_t s);
+// VH_ECANCELED when a cancel is pending (checked at a call's start).
+int check_cancel(vh_ctx *c);
+}  // namespace vh
+
+#define VH_WAIT(c, s)                          \
+    do {                                       \
+        int rc_w_ = vh::wait_stream((c), (s)); \
+        if (rc_w_ != VH_OK) return rc_w_;      \
+    } while (0)
 
 #define VH_HIP(call)                                                                     \
     do {                                                                                 \

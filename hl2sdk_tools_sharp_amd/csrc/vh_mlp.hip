@@ -715,7 +715,7 @@ int leaf_rows(vh_ctx *c, const BounceView &bv, DevBuf &out, int64_t *n, hipStrea
         cl.resize(bv.npatch);
         VH_HIP(hipMemcpyAsync(cl.data(), bv.cluster, cl.size() * 4, hipMemcpyDeviceToHost, s));
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     std::vector<int32_t> rows;
     rows.reserve(leaves.size());
     for (int32_t p : leaves)
@@ -751,7 +751,7 @@ int leaf_rows(vh_ctx *c, const BounceView &bv, DevBuf &out, int64_t *n, hipStrea
         if ((rc = groups->ensure(gp.size() * 4 + 16, "cluster rows"))) return rc;
         VH_HIP(hipMemcpyAsync(groups->ptr, gp.data(), gp.size() * 4, hipMemcpyHostToDevice, s));
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     return VH_OK;
 }
 
@@ -857,7 +857,7 @@ int vh_mlp_init(vh_ctx *c, const float *params, const float *freq, const vh_mlp_
     AdamArgs a = adam_args(D);
     hipLaunchKernelGGL(k_weights, dim3(blocks(a.n)), dim3(256), 0, s, a);
     VH_HIP(hipGetLastError());
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     D->ready = true;
     return VH_OK;
 }
@@ -876,7 +876,7 @@ int vh_mlp_params(vh_ctx *c, float *out)
     MlpData *D = c->mlp;
     std::vector<float> w((size_t)D->L.nparam);
     VH_HIP(hipMemcpyAsync(w.data(), D->master.ptr, w.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    VH_HIP(hipStreamSynchronize(c->stream));
+    VH_WAIT(c, c->stream);
     unpad_params(D->L, w.data(), out);
     return VH_OK;
 }
@@ -908,7 +908,7 @@ int vh_mlp_forward(vh_ctx *c, const float *x, int64_t n, float *y)
     VH_HIP(hipGetLastError());
     std::vector<float> buf((size_t)n * kOutPad);
     VH_HIP(hipMemcpyAsync(buf.data(), D->y.ptr, buf.size() * 4, hipMemcpyDeviceToHost, s));
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     for (int64_t i = 0; i < n; i++)
         for (int k = 0; k < kOut; k++) y[i * kOut + k] = buf[(size_t)i * kOutPad + k];
     return VH_OK;
@@ -940,7 +940,7 @@ int vh_mlp_grad(vh_ctx *c, const float *x, const float *t, int64_t n, float *gra
     float l = 0.0f;
     VH_HIP(hipMemcpyAsync(g.data(), D->grad.ptr, g.size() * 4, hipMemcpyDeviceToHost, s));
     VH_HIP(hipMemcpyAsync(&l, D->loss1.ptr, 4, hipMemcpyDeviceToHost, s));
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     unpad_params(D->L, g.data(), grad);
     if (loss_out) *loss_out = l;
     return VH_OK;
@@ -972,7 +972,7 @@ int vh_mlp_adam(vh_ctx *c, const float *grad, float lr, float beta1, float beta2
     a.state = (const int64_t *)D->state.ptr;
     hipLaunchKernelGGL(k_adam, dim3(blocks(a.n)), dim3(256), 0, s, a);
     VH_HIP(hipGetLastError());
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     D->step += 1;
     return VH_OK;
 }
@@ -1019,7 +1019,7 @@ int vh_mlp_features(vh_ctx *c, const float *emit0, float *x)
     hipLaunchKernelGGL(k_features, dim3(blocks(P)), dim3(256), 0, s, f);
     VH_HIP(hipGetLastError());
     VH_HIP(hipMemcpyAsync(x, D->xf.ptr, (size_t)P * kIn * 4, hipMemcpyDeviceToHost, s));
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     return VH_OK;
 }
 
@@ -1065,7 +1065,7 @@ int vh_mlp_dataset(vh_ctx *c, int32_t op, int64_t *rows)
             VH_HIP(hipMemcpyAsync(X2.ptr, D->X.ptr, (size_t)D->rows * kInPad * 2, hipMemcpyDeviceToDevice, s));
             VH_HIP(hipMemcpyAsync(T2.ptr, D->T.ptr, (size_t)D->rows * kLight * 4, hipMemcpyDeviceToDevice, s));
         }
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         D->X.release();
         D->T.release();
         std::swap(D->X.ptr, X2.ptr);
@@ -1084,7 +1084,7 @@ int vh_mlp_dataset(vh_ctx *c, int32_t op, int64_t *rows)
         f.t = (float *)D->T.ptr + D->rows * kLight;
         hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, s, f);
         VH_HIP(hipGetLastError());
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
     }
     D->rows = need;
     if (rows) *rows = need;
@@ -1230,7 +1230,7 @@ int vh_mlp_predict(vh_ctx *c, const float *emit0, float *start)
             for (int k = 0; k < 3; k++) buf[4 * p + k] = emit0[3 * p + k];
         if ((rc = e.ensure(buf.size() * 4, "emission"))) return rc;
         VH_HIP(hipMemcpyAsync(e.ptr, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         f.e0 = (const float4 *)e.ptr;
     } else {
         f.e0 = bv.direct;
@@ -1260,11 +1260,11 @@ int vh_mlp_predict(vh_ctx *c, const float *emit0, float *start)
     if (start) {
         std::vector<float> buf((size_t)P * 4);
         VH_HIP(hipMemcpyAsync(buf.data(), pred, buf.size() * 4, hipMemcpyDeviceToHost, s));
-        VH_HIP(hipStreamSynchronize(s));
+        VH_WAIT(c, s);
         for (int64_t p = 0; p < P; p++)
             for (int k = 0; k < 3; k++) start[3 * p + k] = buf[4 * p + k];
     }
-    VH_HIP(hipStreamSynchronize(s));
+    VH_WAIT(c, s);
     bounce_pred_ready(c);
     return VH_OK;
 }

@@ -30,7 +30,7 @@ from typing import NamedTuple, Optional
 import numpy as np
 
 from . import _lib
-from ._lib import VH_DECLINED, VH_ENOTSUP, VH_EINVAL, VH_OK, VradHipError
+from ._lib import VH_DECLINED, VH_ECANCELED, VH_EINVAL, VH_ENOTSUP, VH_ETIMEOUT, VH_OK, VradHipError
 
 TRACE_ID_SKY = 0x01000000  # Rad/TraceId.cs:31
 TRACE_ID_OPAQUE = 0x02000000  # Rad/TraceId.cs:34
@@ -45,6 +45,11 @@ HIT_MISS = -1
 
 class NotSupportedError(Exception):
     """Options this tracer cannot honour (the reference's NotSupportedException)."""
+
+
+class OperationCanceledError(Exception):
+    """A call given up at a cancel request (the reference's
+    OperationCanceledException, raised when the CancellationToken fires)."""
 
 
 class ObjectDisposedError(RuntimeError):
@@ -256,6 +261,23 @@ class HipRayTracer:
         return bits
 
     # -- lifetime ---------------------------------------------------------
+    def set_timeout(self, ms: int) -> None:
+        """The bound on each wait on the device (VulkanRayTracer's 120 s fence
+        bound by default); a call past it raises TimeoutError."""
+        self._alive()
+        self._raise(_lib.lib().vh_set_timeout(self._h, int(ms)))
+
+    def cancel(self) -> None:
+        """A CancellationToken firing: calls in flight raise
+        OperationCanceledError at their next wait, later ones at once, until
+        :meth:`clear_cancel`.  Safe from any thread while a call runs."""
+        self._alive()
+        self._raise(_lib.lib().vh_cancel(self._h))
+
+    def clear_cancel(self) -> None:
+        self._alive()
+        self._raise(_lib.lib().vh_cancel_clear(self._h))
+
     def dispose(self) -> None:
         """Waits for in-flight work, then releases the device (VulkanRayTracer.Dispose)."""
         if self._disposed:
@@ -299,6 +321,10 @@ class HipRayTracer:
             raise ValueError(msg)
         if status == VH_ENOTSUP:
             raise NotSupportedError(msg)
+        if status == VH_ETIMEOUT:
+            raise TimeoutError(msg)  # the reference's fence bound: TimeoutException
+        if status == VH_ECANCELED:
+            raise OperationCanceledError(msg)
         raise VradHipError(status, msg)
 
 

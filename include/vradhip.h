@@ -148,7 +148,24 @@ int vh_trace_point_lights_noalloc(vh_ctx *ctx, const float *d_points, int64_t np
                                   const float *d_lights, int32_t nlights, float min_distance,
                                   uint64_t *d_bits, void *stream);
 
-/* Waits for the work enqueued on `stream` (NULL = the context's stream). */
+/* Bounded waits and cancellation, the seam's CancellationToken and fence
+ * bound (SourceSharp.MapTools/Tracing/IRayTracer.cs:162-191 take a
+ * CancellationToken; SourceSharp.MapTools.Gpu/VulkanRayTracer.cs:39-49 bounds
+ * each fence wait at 120 s).  Every blocking call waits on the device through
+ * a deadline of `ms` milliseconds (default 120000; negative: none) and gives up
+ * with VH_ETIMEOUT when it passes.  vh_cancel (any thread, no lock: meant for a
+ * token's callback) makes the calls in flight on the context return
+ * VH_ECANCELED at their next wait, and later calls fail at once, until
+ * vh_cancel_clear.  Either way the work already enqueued runs to its end on
+ * the device, writing only the context's own buffers (results reach the
+ * caller's arrays only after a completed wait), and the context stays usable:
+ * later calls queue behind it. */
+int vh_set_timeout(vh_ctx *ctx, int64_t ms);
+int vh_cancel(vh_ctx *ctx);
+int vh_cancel_clear(vh_ctx *ctx);
+
+/* Waits for the work enqueued on `stream` (NULL = the context's stream),
+ * bounded as above. */
 int vh_sync(vh_ctx *ctx, void *stream);
 
 /* Cumulative counters (IGpuTraceStatistics, SourceSharp.MapTools.Gpu/SlabBatcher.cs:387). */

@@ -91,3 +91,9 @@ def test_last_error_is_per_thread(vh_lib):
     t.join()
     assert seen["msg"] == ""
     assert _lib.last_error() != ""
+
+
+def test_deadline_and_cancel_entry_points_reject_a_null_context(vh_lib):
+    assert vh_lib.vh_set_timeout(None, 1000) == 2
+    assert vh_lib.vh_cancel(None) == 2
+    assert vh_lib.vh_cancel_clear(None) == 2
