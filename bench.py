@@ -38,18 +38,29 @@ FLOP_PER_NODE = 44
 FLOP_PER_TRI = 27
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (packed FMA) peak
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
+
+
+def host_threads():
+    """(threads the CPU baselines use, cores in this process's affinity mask):
+    every core the process may run on, capped by OMP_NUM_THREADS when the host
+    sets one (a GPU box's share of its cores)."""
+    cores = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    used = min(cores, int(cap)) if cap.isdigit() and int(cap) > 0 else cores
+    return used, cores
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel: str):
+def read_traffic(kernel: str, key: str = "hbm_bytes_per_launch"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
     if not os.path.exists(path):
         return None
-    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    m = re.search(rf"^{key}:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -139,6 +150,38 @@ def bounce_leg(a, log):
     _lib.check(L.vh_timing_enable(tr.handle, 0))
     a32 = results["f32"].total_light.astype(np.float64)
     out["f16_vs_f32_rel_l1"] = float(np.abs(results["f16"].total_light - a32).sum() / np.abs(a32).sum())
+    # the HBM roofline of the gather in its FP16 form (the optimised bounce's)
+    ach = out["gather_GBps_f16"]
+    out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": read_traffic("k_gather", "hbm_bytes_f16"),
+                       "kernel": "k_gather<true>", "kernel_ms": round(out["gather_ms_f16"], 4),
+                       "algorithmic_bytes": out["gather_bytes_f16"],
+                       "fp32": {"achieved": round(out["gather_GBps_f32"], 1),
+                                "frac": round(out["gather_GBps_f32"] / PEAK_HBM_GBS, 4),
+                                "traffic": read_traffic("k_gather", "hbm_bytes_f32")}}
+    if not a.no_cpu:
+        # the CPU restatement of Radiosity.GatherLight on the same FP32
+        # transfers, row-parallel, for the FP32 bounce's iteration count
+        import oracle.bounce as ob
+
+        threads, cores = host_threads()
+        rp, src, w = st.read_transfers()
+        shoot = (np.asarray(e0, np.float32) * g.refl).astype(np.float32)
+        iters = results["f32"].iterations
+        first = ob.gather_light(rp, src, w, shoot, threads)  # pages the arrays in
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            ob.gather_light(rp, src, w, shoot, threads)
+        dt = time.perf_counter() - t0
+        per = dt / iters
+        out["cpu_baseline"] = {
+            "value": dt, "unit": "s", "cores": threads, "affinity_cores": cores, "kind": "port",
+            "sample": f"{iters} GatherLight passes (the FP32 bounce's iteration count) over all {T} transfers "
+            f"of the {g.npatch}-patch map; CPU restatement of Radiosity.GatherLight, FP32, rows over pthreads",
+            "per_iteration_ms": per * 1e3, "GBps": out["gather_bytes_f32"] / per / 1e9}
+        log(f"[bench] bounce CPU GatherLight {per * 1e3:.1f} ms per pass on {threads} threads "
+            f"({out['gather_bytes_f32'] / per / 1e9:.1f} GB/s); {iters} passes {dt:.2f} s")
+        del rp, src, w, first
     tr.dispose()
     return out
 
@@ -155,7 +198,7 @@ def batching_leg(a, log):
     oracle on a seeded face subsample (10 % by default), whose GPU result it
     also checks bit for bit."""
     import oracle.gather as og
-    from hl2sdk_tools_sharp_amd import HipTracerFactory, autotune, light, scenes
+    from hl2sdk_tools_sharp_amd import HipTracerFactory, _lib, autotune, light, scenes
 
     t0 = time.perf_counter()
     casters, faces = scenes.batching_workload(nfaces=a.batch_faces, seed=4)
@@ -187,6 +230,28 @@ def batching_leg(a, log):
         q_ms.append(ms)
         rays = r
     q = float(np.median(q_ms))
+    # one counted launch (outside the timing): node visits, triangle tests and
+    # the lane slots the packets occupied
+    L_ = _lib.lib()
+    _lib.check(L_.vh_count_enable(tr.handle, 1))
+    fl.run("queue", wpc)
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(L_.vh_count_read(tr.handle, ctypes.byref(nv), ctypes.byref(nt)))
+    ns, ts = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(L_.vh_count_read_slots(tr.handle, ctypes.byref(ns), ctypes.byref(ts)))
+    _lib.check(L_.vh_count_enable(tr.handle, 0))
+    flops = nv.value * FLOP_PER_NODE + nt.value * FLOP_PER_TRI
+    ach = flops / (q / 1e3) / 1e12
+    lanes = (nv.value + nt.value) / max(64 * (ns.value + ts.value), 1)
+    roofline = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_facelight<true>",
+                "kernel_ms": round(q, 3), "flop_per_launch": flops,
+                "node_visits_per_ray": round(nv.value / max(rays, 1), 3),
+                "triangle_tests_per_ray": round(nt.value / max(rays, 1), 3),
+                "active_lane_fraction": round(lanes, 4),
+                "node_packet_efficiency": round(nv.value / max(64 * ns.value, 1), 4)}
+    log(f"[bench] batching roofline: {ach:.2f} TFLOP/s ({ach / PEAK_FP32_TFLOPS:.1%} of FP32 peak); "
+        f"active-lane fraction {lanes:.3f} over node and triangle slots")
     ref = fl.read()
     pf_ms, _n, pf_rays = fl.timed_run("per_face")
     same = bool(np.array_equal(fl.read().ldr.view(np.uint32), ref.ldr.view(np.uint32)))
@@ -198,7 +263,7 @@ def batching_leg(a, log):
         "queue_ms": q, "queue_rays_per_s": rays / (q / 1e3), "waves_per_cu": wpc,
         "tune_ms": {str(k): round(v, 3) for k, v in tune.ms.items()},
         "per_face_ms": pf_ms, "per_face_rays_per_s": pf_rays / (pf_ms / 1e3),
-        "queue_speedup": pf_ms / q, "queue_equals_per_face": same,
+        "queue_speedup": pf_ms / q, "queue_equals_per_face": same, "roofline": roofline,
     }
     # a seeded face subsample: the CPU baseline and a parity check at the config's scale
     nsub = max(1, int(round(faces.nfaces * a.batch_cpu_fraction)))
@@ -209,7 +274,7 @@ def batching_leg(a, log):
     got = fl.read()
     tr.dispose()
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, cores = host_threads()
         d = light.pack_desc(fs, L, None, kw["sun_extent"], kw["sun_samples"], kw["sky_dirs"],
                             kw["ao_count"], kw["ao_radius"])
         t0 = time.perf_counter()
@@ -219,7 +284,7 @@ def batching_leg(a, log):
                    + np.count_nonzero(hdr.view(np.uint32) != got.hdr.view(np.uint32))
                    + np.count_nonzero(ao != got.ao))
         out["cpu_baseline"] = {
-            "value": sub_rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "value": sub_rays / dt, "unit": "rays/s", "cores": threads, "affinity_cores": cores, "kind": "port",
             "sample": f"{len(sub)} of {faces.nfaces} faces ({fs.nsamples} luxels, {sub_rays} rays, "
             f"{dt:.1f} s); CPU restatement of the reference's gather (sun, sky ambient) plus AO, pthreads",
         }
@@ -285,7 +350,7 @@ def plain_vrad_leg(a, log):
     log(f"[bench] plain vrad GPU pass {best.wall_s * 1e3:.1f} ms: "
         + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in best.stage_s.items()))
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, cores = host_threads()
         t0 = time.perf_counter()
         d = light.pack_desc(faces, lights, pvs, 0.0, 1, None)
         ldr, _hdr, _ao = og.gather(g.casters, d, faces.nsamples, threads=threads)
@@ -298,7 +363,7 @@ def plain_vrad_leg(a, log):
         lump_same_bounce, _ = final_lump_cpu(fin, faces, ldr, best.bounce.total_light, threads)
         b = best.bounce.total_light.astype(np.float64)
         out["cpu_baseline"] = {
-            "value": dt, "unit": "s", "cores": threads, "kind": "port",
+            "value": dt, "unit": "s", "cores": threads, "affinity_cores": cores, "kind": "port",
             "sample": "the whole pass (gather, patch lights, transfer build, bounce, final light "
             "and lump); CPU restatement of the reference's stages, pthreads where the stage is parallel",
         }
@@ -338,8 +403,14 @@ def warm_start_leg(a, log):
     train_seeds = list(range(100, 100 + a.mlp_maps))
     test_seeds = list(range(200, 200 + a.mlp_test_maps))
     t0 = time.perf_counter()
-    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
+    # at most 8 worker processes, ended by close and join (no terminate), so a
+    # profiled run never holds more than a handful of processes
+    pool = mp.get_context("spawn").Pool(min(8, host_threads()[0]))
+    try:
         maps = pool.map(_grid_for_seed, train_seeds + test_seeds)
+    finally:
+        pool.close()
+        pool.join()
     gen_s = time.perf_counter() - t0
     train, test = maps[:len(train_seeds)], maps[len(train_seeds):]
     log(f"[bench] warm start: {len(train)} training maps ({sum(g.npatch for g in train)} patches), "
@@ -372,8 +443,14 @@ def warm_start_leg(a, log):
     log(f"[bench] warm start: {rows} rows from {len(train)} maps in {data_s:.1f} s; {steps} steps in "
         f"{train_s:.2f} s ({flop / (ms.value / 1e3) / 1e12:.1f} TFLOP/s over the training launches); "
         f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    tflops = flop / (ms.value / 1e3) / 1e12
     out = {"train_maps": len(train), "test_maps": len(test), "rows": rows, "epochs": a.mlp_epochs,
-           "batch": a.mlp_batch, "train_s": train_s, "train_tflops": flop / (ms.value / 1e3) / 1e12,
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_BF16_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 5), "traffic": None,
+                        "kernel": "the training step's launches (k_gemm, k_gemm_dw, k_dw_reduce, k_adam, k_batch)",
+                        "kernel_ms_per_step": round(ms.value / max(steps, 1), 5),
+                        "flop_per_step": 6.0 * 36352 * a.mlp_batch},
+           "batch": a.mlp_batch, "train_s": train_s, "train_tflops": tflops,
            "data_s": data_s, "generate_s": gen_s, "prior_gain": prior.gain, "prior_scale": prior.scale,
            "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "maps": []}
     for g in test:
@@ -503,7 +580,7 @@ def final_leg(a, log):
                           "samples_changed": int(np.count_nonzero(np.any(ss_light != ss_base, axis=(0, 2))))}
     log(f"[bench] supersampling: passes re-lit {ss_counts} samples, {ss_rays} rays in {ss_ms:.3f} ms")
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, cores = host_threads()
         pd = {"origin": fp.origin, "face": fp.face, "child1": fp.child1, "winding_ptr": fp.winding_ptr,
               "winding": fp.winding}
         t0 = time.perf_counter()
@@ -512,7 +589,7 @@ def final_leg(a, log):
                                      pd, pl, threads=threads)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {
-            "value": dt * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+            "value": dt * 1e3, "unit": "ms", "cores": threads, "affinity_cores": cores, "kind": "port",
             "sample": "the whole map's final light (radial filter, patch radial, minlight, medians, RGBE); "
             "CPU restatement of FinalLightFace.Run, faces over pthreads"}
         out["lump_equal_cpu"] = bool(np.array_equal(lump_c, lump))
@@ -524,7 +601,7 @@ def final_leg(a, log):
         want = oamb.ambient_cubes(g.casters, afaces, None, lump, pts[:k], threads=threads)
         dt = time.perf_counter() - t0
         out["ambient"]["cpu_baseline"] = {
-            "value": k * 162 / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "value": k * 162 / dt, "unit": "rays/s", "cores": threads, "affinity_cores": cores, "kind": "port",
             "sample": f"the first {k} of {len(pts)} points (x 162 rays); the oracle's BVH closest hit "
             "(pthreads) and the float32 restatement of Accumulate and Project"}
         out["ambient"]["cubes_differing_cpu"] = int(np.sum(cubes[:k].view(np.uint32) != want.view(np.uint32)))
@@ -542,7 +619,7 @@ def final_leg(a, log):
                                             ss_base[:, :ns], which=1, threads=threads)
         dt = time.perf_counter() - t0
         out["supersample"]["cpu_baseline"] = {
-            "value": dt * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+            "value": dt * 1e3, "unit": "ms", "cores": threads, "affinity_cores": cores, "kind": "port",
             "sample": f"the first {nf} of {faces.nfaces} faces ({ns} samples): the oracle's passes, "
             "its gather over pthreads"}
         out["supersample"]["values_differing_cpu"] = int(np.sum(
@@ -576,16 +653,20 @@ def bake_leg(a, log, model):
     kw = dict(hdr=True, sun_samples=1, tol=1e-4, read_direct=True, final_inputs=fin, extra_passes=4)
     light_pass(g, faces, lights, pvs, **kw)  # first-use costs outside the timing
     plain = light_pass(g, faces, lights, pvs, schedule="per_face", weights="f32", **kw)
+    batched = light_pass(g, faces, lights, pvs, schedule="grid", weights="f32", **kw)
     fast = light_pass(g, faces, lights, pvs, schedule="queue", weights="f16", warm_start=model, **kw)
     out = {"patches": g.npatch, "luxels": faces.nsamples, "lights": len(lights)}
-    for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+    forms = (("plain_f32", plain), ("plain_batched_f32", batched), ("optimised_f16_warm", fast))
+    for name, r in forms:
         out[name] = {"wall_s": r.wall_s, "iterations": r.bounce.iterations,
                      "supersample_passes": r.supersample_passes,
                      "stage_s": {k: round(v, 5) for k, v in r.stage_s.items()}}
         log(f"[bench] bake {name}: {r.wall_s:.3f} s, {r.bounce.iterations} iterations: "
             + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in r.stage_s.items()))
     out["speedup"] = plain.wall_s / fast.wall_s
+    out["speedup_vs_plain_batched"] = batched.wall_s / fast.wall_s
     out["lump_rel_l1_optimised_vs_plain"] = lump_rel_l1(fast.lump, plain.lump)
+    out["plain_batched_lump_equals_plain"] = bool(np.array_equal(batched.lump, plain.lump))
     if not a.no_cpu:
         offer = HipTracerFactory(device=0).try_create(g.casters)
         with offer.tracer as tr:
@@ -597,7 +678,7 @@ def bake_leg(a, log, model):
         out["fp64_oracle_s"] = time.perf_counter() - t0
         out["fp64_oracle_iterations"] = it
         leaf = (g.leaf & ~g.sky.astype(bool))[:, None]
-        for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+        for name, r in forms:
             d = np.abs((r.bounce.total_light.astype(np.float64) - ref) * leaf).sum()
             out[name]["rel_l1_vs_fp64"] = float(d / max(np.abs(ref * leaf).sum(), 1e-300))
         # the lightmaps: each form's lump against the lump the same final
@@ -616,15 +697,16 @@ def bake_leg(a, log, model):
             pl = np.zeros((g.npatch, 4, 3), np.float32)
             pl[:, 0] = ref
             ref_lump, out["final_kernel_ms"] = fr.timed_run(hdr=True, bounce="host", patch_light=pl)
-        for name, r in (("plain_f32", plain), ("optimised_f16_warm", fast)):
+        for name, r in forms:
             out[name]["lump_hdr_rel_l1_vs_fp64"] = lump_rel_l1(r.lump, ref_lump)
         log(f"[bench] bake vs FP64 oracle ({it} iterations, {out['fp64_oracle_s']:.1f} s): bounced light "
             f"plain {out['plain_f32']['rel_l1_vs_fp64']:.2e}, optimised "
             f"{out['optimised_f16_warm']['rel_l1_vs_fp64']:.2e}; lightmap HDR plain "
             f"{out['plain_f32']['lump_hdr_rel_l1_vs_fp64']:.2e}, optimised "
             f"{out['optimised_f16_warm']['lump_hdr_rel_l1_vs_fp64']:.2e}")
-    log(f"[bench] bake speed-up {out['speedup']:.1f}x; lightmap HDR rel L1 optimised vs plain "
-        f"{out['lump_rel_l1_optimised_vs_plain']:.2e}")
+    log(f"[bench] bake speed-up {out['speedup']:.1f}x over per-face launches, "
+        f"{out['speedup_vs_plain_batched']:.2f}x over the batched plain form; lightmap HDR rel L1 optimised vs "
+        f"plain {out['lump_rel_l1_optimised_vs_plain']:.2e}")
     return out
 
 
@@ -779,9 +861,11 @@ def main(argv=None) -> int:
     cpu = None
     parity = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU baseline on {threads} threads (~{a.cpu_seconds:.0f} s)")
+        threads, cores = host_threads()
+        log(f"[bench] CPU baseline on {threads} threads of {cores} cores in the affinity mask "
+            f"(~{a.cpu_seconds:.0f} s)")
         cpu, mism = cpu_baseline(w, gpu_bits, a.cpu_seconds, threads)
+        cpu["affinity_cores"] = cores
         parity = {"sample_words_differing": mism}
         log(f"[bench] CPU {cpu['value'] / 1e6:.2f} Mrays/s; sample words differing from GPU: {mism}")
     tr.dispose()

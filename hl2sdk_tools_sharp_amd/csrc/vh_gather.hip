@@ -139,6 +139,7 @@ struct GatherArgs {
     const float4 *nodes;
     const float4 *tris;
     unsigned long long *rays;  // rays traced, summed per wave
+    unsigned long long *counters;  // COUNT launches: node visits, triangle tests, node and triangle slots
     int32_t light_filter;      // 0 every light; 1 sky ambient only; 2 all but sky ambient
     int32_t no_skip;           // ResolveResample: every light's record is added
 };
@@ -184,21 +185,32 @@ __device__ __forceinline__ Ray segment(float px, float py, float pz, float ex, f
 // packet_closest over the wave's stack `wstk`): the rays of one light, one sun
 // or sky direction, or one AO direction of one face leave from the samples of
 // one face, so they travel as a packet.
-__device__ __forceinline__ bool blocked(const GatherArgs &a, int *wstk, const Ray &r, int &nr)
+// Traversal work of one lane (counted only in the COUNT instantiation):
+// node visits, triangle tests, and the wave's node and triangle slots.
+struct Work {
+    uint32_t visit = 0, test = 0, slots[2] = {0, 0};
+};
+
+template <bool COUNT>
+__device__ __forceinline__ bool blocked(const GatherArgs &a, int *wstk, const Ray &r, int &nr, Work &wk)
 {
     const bool fin = ray_finite(r);
     nr += fin ? 1 : 0;
-    return packet_any<false>(a.nodes, a.tris, wstk, r, 0.0f, fin);
+    return packet_any<COUNT>(a.nodes, a.tris, wstk, r, 0.0f, fin, &wk.visit, &wk.test, nullptr,
+                             COUNT ? wk.slots : nullptr);
 }
 
 // LightRayLog.Occlusion: the first hit short of the end, unless it is sky.
-__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *wstk, const Ray &r, int &nr)
+template <bool COUNT>
+__device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *wstk, const Ray &r, int &nr,
+                                               Work &wk)
 {
     const bool fin = ray_finite(r);
     nr += fin ? 1 : 0;
     float t;
     int32_t id;
-    const int32_t h = packet_closest<kClosestShort>(a.nodes, a.tris, wstk, r, 0.0f, fin, t, id);
+    const int32_t h = packet_closest<kClosestShort, -1, COUNT>(a.nodes, a.tris, wstk, r, 0.0f, fin, t, id,
+                                                               &wk.visit, &wk.test, wk.slots);
     return (fin && h >= 0 && (id & VH_TRACE_ID_SKY) == 0) ? 1.0f : 0.0f;
 }
 
@@ -214,9 +226,10 @@ __device__ __forceinline__ float inverse_quadratic(const vh_light &l, float d)
 // One wave lights one chunk: 16 groups of 4 samples of one face.  NN is the
 // face's normal count (1, or 4 when bumped), so every per-normal array is in
 // registers.
-template <int NN>
+template <int NN, bool COUNT>
 __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
 {
+    Work wk;
     int lane = threadIdx.x & 63;
     int sub = lane & 3;
     int32_t f = a.chunk_face[chunk];
@@ -329,7 +342,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                 _Pragma("unroll") for (int n = 1; n < normals; n++)
                     dot[n] = maxps(0.0f, dot3(bn[n - 1][0], bn[n - 1][1], bn[n - 1][2], dx, dy, dz));
                 if (inmask && d0 != 0.0f && fo != 0.0f) {
-                    if (blocked(a, stk, segment(px, py, pz, sx, sy, sz), nr)) dot[0] = 0.0f * dot[0];
+                    if (blocked<COUNT>(a, stk, segment(px, py, pz, sx, sy, sz), nr, wk)) dot[0] = 0.0f * dot[0];
                 }
             }
         } else if (L.type == VH_EMIT_SKYLIGHT) {
@@ -348,7 +361,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                         ez = ez + (jt[2] * js);
                     }
                     float occ = 0.0f;
-                    if (real) occ = sky_occlusion(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz), nr);
+                    if (real) occ = sky_occlusion<COUNT>(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz), nr, wk);
                     total += 1.0f - minps(maxps(occ, 0.0f), 1.0f);
                 }
                 float see = total * (1.0f / (float)a.sun_samples);
@@ -383,11 +396,11 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                 }
                 float frac = 1.0f;
                 if (real && tr && inmask)
-                    frac = 1.0f - sky_occlusion(a, stk,
+                    frac = 1.0f - sky_occlusion<COUNT>(a, stk,
                                                 segment(px, py, pz, (ax * -kMaxTrace) + px,
                                                         (ay * -kMaxTrace) + py,
                                                         (az * -kMaxTrace) + pz),
-                                                nr);
+                                                nr, wk);
                 _Pragma("unroll") for (int n = 0; n < normals; n++) amb[n] = amb[n] + (frac * dts[n]);
             }
             falloff = 1.0f;
@@ -462,30 +475,44 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
             r.dy = dirs[3 * k + 1] * a.ao_radius;
             r.dz = dirs[3 * k + 2] * a.ao_radius;
             r.reach = 1.0f;
-            if (!blocked(a, stk, r, nr)) unblocked++;
+            if (!blocked<COUNT>(a, stk, r, nr, wk)) unblocked++;
         }
         a.out_ao[s] = (float)unblocked / (float)a.ao_count;
     }
     // every lane of the wave is here: one atomic per chunk
     for (int o = 32; o > 0; o >>= 1) nr += __shfl_xor(nr, o, 64);
     if (lane == 0) atomicAdd(a.rays, (unsigned long long)nr);
+    if (COUNT) {
+        unsigned long long v[4] = {wk.visit, wk.test, wk.slots[0], wk.slots[1]};
+        for (int q = 0; q < 4; q++) {
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        }
+        // slots are per wave (every lane holds the same count)
+        if (lane == 0) {
+            atomicAdd(&a.counters[0], v[0]);
+            atomicAdd(&a.counters[1], v[1]);
+            atomicAdd(&a.counters[2], v[2] / 64);
+            atomicAdd(&a.counters[3], v[3] / 64);
+        }
+    }
 }
 
+template <bool COUNT>
 __device__ __forceinline__ void light_chunk(const GatherArgs &a, int64_t chunk, int *stk)
 {
     if (a.face_nc[a.chunk_face[chunk]] == 4)
-        light_chunk_n<4>(a, chunk, stk);
+        light_chunk_n<4, COUNT>(a, chunk, stk);
     else
-        light_chunk_n<1>(a, chunk, stk);
+        light_chunk_n<1, COUNT>(a, chunk, stk);
 }
 
-template <bool PERSISTENT>
+template <bool PERSISTENT, bool COUNT = false>
 __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first_chunk,
                                                      unsigned long long *__restrict__ ticket)
 {
     __shared__ int stk[3 * kMaxDepth];  // the wave's traversal stack
     if (!PERSISTENT) {
-        light_chunk(a, first_chunk + blockIdx.x, stk);
+        light_chunk<COUNT>(a, first_chunk + blockIdx.x, stk);
         return;
     }
     for (;;) {
@@ -493,7 +520,7 @@ __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first
         if (threadIdx.x == 0) c = atomicAdd(ticket, 1ull);
         c = __shfl(c, 0, 64);
         if ((int64_t)c >= a.nchunks) break;
-        light_chunk(a, (int64_t)c, stk);
+        light_chunk<COUNT>(a, (int64_t)c, stk);
     }
 }
 
@@ -630,6 +657,7 @@ GatherArgs gather_args(vh_ctx *c)
     a.nodes = (const float4 *)c->nodes.ptr;
     a.tris = (const float4 *)c->tris.ptr;
     a.rays = (unsigned long long *)G->ticket.ptr + 1;
+    a.counters = (unsigned long long *)c->counters.ptr;
     a.light_filter = 0;
     a.no_skip = 0;
     return a;
@@ -1026,8 +1054,12 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
             int wpc = waves_per_cu > 0 ? waves_per_cu : 20;
             int64_t waves = (int64_t)c->cu_count * wpc;
             if (waves > G->nchunks) waves = G->nchunks;
-            hipLaunchKernelGGL(k_facelight<true>, dim3((uint32_t)waves), dim3(kWave), 0, st, a,
-                               (int64_t)0, (unsigned long long *)G->ticket.ptr);
+            if (c->counting && c->counters.ptr)
+                hipLaunchKernelGGL((k_facelight<true, true>), dim3((uint32_t)waves), dim3(kWave), 0, st, a,
+                                   (int64_t)0, (unsigned long long *)G->ticket.ptr);
+            else
+                hipLaunchKernelGGL(k_facelight<true>, dim3((uint32_t)waves), dim3(kWave), 0, st, a,
+                                   (int64_t)0, (unsigned long long *)G->ticket.ptr);
         } else if (schedule == VH_SCHEDULE_GRID) {
             hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)G->nchunks), dim3(kWave), 0, st,
                                a, (int64_t)0, nullptr);

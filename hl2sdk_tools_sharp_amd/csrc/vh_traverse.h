@@ -489,11 +489,12 @@ __device__ __forceinline__ bool occluder_blocks(const float4 *__restrict__ tris,
 // masks shrink as lanes find hits; ties go to the lower original triangle
 // index, so the answer is the one-ray answer.  Returns the lane's original
 // triangle index or -1, with its distance and caller id.
-template <int MODE, int OCT = -1>
+template <int MODE, int OCT = -1, bool COUNT = false>
 __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nodes,
                                                   const float4 *__restrict__ tris, int *wstk,
                                                   const Ray &r, float tmin, bool want, float &t_out,
-                                                  int32_t &id_out)
+                                                  int32_t &id_out, uint32_t *nvisit = nullptr,
+                                                  uint32_t *ntest = nullptr, uint32_t *slots = nullptr)
 {
     const int lane = lane_id();
     const float ix = inv_saturate(r.dx), iy = inv_saturate(r.dy), iz = inv_saturate(r.dz);
@@ -512,6 +513,10 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
     for (;;) {
         // descend (see packet_any)
         while (ref >= 0) {
+            if (COUNT) {
+                *nvisit += (uint32_t)(mask >> lane) & 1u;
+                ++slots[0];
+            }
             const float4 *nd = nodes + (size_t)ref * 4;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             const float thi = best_t;
@@ -560,6 +565,10 @@ __device__ __forceinline__ int32_t packet_closest(const float4 *__restrict__ nod
                 const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                 const uint32_t meta = (uint32_t)uniform(__float_as_int(t2.w));
                 const int32_t orig = (int32_t)(meta >> kOrigShift);
+                if (COUNT) {
+                    *ntest += act ? 1u : 0u;
+                    ++slots[1];
+                }
                 float t;
                 const bool h = tri_test_uniform(t0, t1, t2, meta, r, t) & (t >= tmin);
                 const bool within = MODE == kClosestInclusive ? (t <= r.reach) : (t < r.reach);

@@ -182,6 +182,20 @@ def bounce(g, row_ptr, src, w, emit0, start=None, mode=0, max_iter=100, tol=0.0,
     return out, res[:n], en[:n], n
 
 
+def gather_light(row_ptr, src, w, shoot, threads=None) -> np.ndarray:
+    """Radiosity.GatherLight over every row (FP32, transfer order), rows
+    shared over threads: add (P, 3) = per row sum of shoot[src] * w."""
+    L = _bind()
+    L.vc_gather_light.argtypes = [ctypes.c_int32] + [ctypes.c_void_p] * 5 + [ctypes.c_int32]
+    rp = np.ascontiguousarray(row_ptr, np.int64)
+    P = len(rp) - 1
+    sh = np.ascontiguousarray(shoot, np.float32).reshape(P, 3)
+    out = np.zeros((P, 3), np.float32)
+    L.vc_gather_light(P, _p(rp), _p(np.ascontiguousarray(src, np.int32)), _p(np.ascontiguousarray(w, np.float32)),
+                      _p(sh), _p(out), int(threads or os.cpu_count() or 1))
+    return out
+
+
 def patch_lights(g, faces, light, sample_area):
     """PatchLighting.BuildPatchLights with AddSampleToPatch
     (Rad/Light/PatchLighting.cs:65-249), numpy, in the reference's float order.

@@ -163,6 +163,10 @@ typedef struct vc_gather_desc {
     const int32_t *face_style_filter;
 } vc_gather_desc;
 
+/* Radiosity.GatherLight for flat rows, FP32, row-parallel (vradcpu_bounce.c). */
+void vc_gather_light(int32_t P, const int64_t *row_ptr, const int32_t *src, const float *w,
+                     const float *shoot, float *add, int32_t threads);
+
 /* Correctly rounded powers (vradcpu_math.c): DetMathF.Pow and
  * DetMath.PowToSingle. */
 float vc_pow_f32(float x, float y);

@@ -483,3 +483,67 @@ void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const floa
     else
         BOUNCE_BODY(float)
 }
+
+/* ---- GatherLight alone, row-parallel (the bounce's CPU baseline) ---- */
+
+typedef struct {
+    int32_t P;
+    const int64_t *row_ptr;
+    const int32_t *src;
+    const float *w, *shoot;
+    float *add;
+    int32_t next;
+    pthread_mutex_t mu;
+} gl_job;
+
+static void *gl_worker(void *arg)
+{
+    gl_job *j = (gl_job *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        int32_t a = j->next;
+        j->next += 1024;
+        pthread_mutex_unlock(&j->mu);
+        if (a >= j->P) break;
+        int32_t b = a + 1024 < j->P ? a + 1024 : j->P;
+        for (int32_t r = a; r < b; r++) {
+            float sx = 0, sy = 0, sz = 0;
+            for (int64_t k = j->row_ptr[r]; k < j->row_ptr[r + 1]; k++) {
+                int32_t s = j->src[k];
+                float wk = j->w[k];
+                sx = sx + (j->shoot[3 * s] * wk);
+                sy = sy + (j->shoot[3 * s + 1] * wk);
+                sz = sz + (j->shoot[3 * s + 2] * wk);
+            }
+            j->add[3 * r] = sx;
+            j->add[3 * r + 1] = sy;
+            j->add[3 * r + 2] = sz;
+        }
+    }
+    return NULL;
+}
+
+/* Radiosity.GatherLight (Rad/Bounce/Radiosity.cs:216-252) for flat rows, the
+ * bounce loop's per-row sum above in FP32 and transfer order: add_r = sum over
+ * the row of shoot[src] * w.  Rows are independent, so they are shared out
+ * over `threads` in blocks of 1,024. */
+void vc_gather_light(int32_t P, const int64_t *row_ptr, const int32_t *src, const float *w,
+                     const float *shoot, float *add, int32_t threads)
+{
+    gl_job j;
+    memset(&j, 0, sizeof j);
+    j.P = P;
+    j.row_ptr = row_ptr;
+    j.src = src;
+    j.w = w;
+    j.shoot = shoot;
+    j.add = add;
+    pthread_mutex_init(&j.mu, NULL);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (int i = 1; i < threads; i++) pthread_create(&th[i], NULL, gl_worker, &j);
+    gl_worker(&j);
+    for (int i = 1; i < threads; i++) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&j.mu);
+}

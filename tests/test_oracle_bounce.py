@@ -247,3 +247,21 @@ def test_a_bump_normal_the_source_is_behind_gets_nothing():
     assert np.all(tb[0, 1] > 0) and np.all(tb[0, 2] > 0)
     # the flat slot: the cosine removed and put back, the flat gather within rounding
     np.testing.assert_allclose(tot[0], [1, 2, 3], rtol=1e-6)
+
+
+def test_threaded_gather_light_is_the_bounces_first_pass():
+    """vc_gather_light (the bounce's CPU baseline) is exactly the flat rows'
+    GatherLight of the bounce: with emission e0, one progressive iteration's
+    bounced light is GatherLight(e0 * reflectivity), bit for bit, whatever the
+    thread count."""
+    from hl2sdk_tools_sharp_amd import rooms
+
+    g = rooms.room_grid(2, 2, size=(384, 320, 192), patch=128, minchop=64, seed=5)
+    rp, src, w, _ = ob.build_transfers(g)
+    e0 = np.random.default_rng(1).uniform(0, 50, (g.npatch, 3)).astype(np.float32)
+    tot, _, _, _ = ob.bounce(g, rp, src, w, e0, max_iter=1)
+    shoot = (e0 * g.refl).astype(np.float32)
+    leaf = g.leaf & ~g.sky.astype(bool)
+    for threads in (1, 5):
+        add = ob.gather_light(rp, src, w, shoot, threads=threads)
+        assert np.array_equal(tot[leaf].astype(np.float32), add[leaf])

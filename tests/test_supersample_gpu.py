@@ -149,3 +149,77 @@ def test_light_pass_supersamples_before_the_patch_handoff(rooms_case):
         st.load_patches()
         direct = st.patch_lights(fl, hdr=False, read=True)
     assert same(r.direct, direct)
+
+
+def _ragged_case(seed=11, nfaces=23):
+    """Floor faces of ragged sizes (3 to 700 samples, 8-unit luxels) laid side
+    by side under a point light and a slab: shadow edges cross many faces,
+    and the faces straddle the selection kernel's 256-sample blocks."""
+    rng = np.random.default_rng(seed)
+    v9, ids = box_triangles(np.array([[-50.0, 40.0, 60.0]]), np.array([[2000.0, 120.0, 70.0]]))
+    casters = ShadowCasterSet(v9, (ids & 0xFFFFFF) | TRACE_ID_OPAQUE)
+    fr = np.zeros(nfaces, FACE_DTYPE)
+    ptr, pos = [0], []
+    x0 = 0.0
+    for f in range(nfaces):
+        w = int(rng.integers(1, 36))
+        h = int(rng.integers(3, 21))
+        if f % 5 == 2:
+            w, h = 1, 3  # tiny faces among the large ones
+        fr[f]["origin"] = (x0 + 4, 4, 0)
+        fr[f]["w2l_s"], fr[f]["w2l_t"] = (1 / 8, 0, 0), (0, 1 / 8, 0)
+        fr[f]["l2w_s"], fr[f]["l2w_t"] = (8, 0, 0), (0, 8, 0)
+        fr[f]["width"], fr[f]["height"], fr[f]["lit"] = w, h, 1
+        for t in range(h):
+            for s in range(w):
+                pos.append((x0 + 4 + 8 * s, 4 + 8 * t, 0.0))
+        ptr.append(len(pos))
+        x0 += 8 * w + 16
+    n = len(pos)
+    normals = np.zeros((nfaces, 4, 3), np.float32)
+    normals[:, 0] = (0, 0, 1)
+    fset = FaceSet(np.array(ptr, np.int32), np.ones(nfaces, np.int32), normals, np.array(pos, np.float32),
+                   np.zeros(n, np.int32))
+    L = light.make_lights(1)
+    L["type"], L["origin"], L["quadratic_attn"] = light.EMIT_POINT, (x0 / 2, 80, 400), 1
+    L["intensity"] = L["intensity_hdr"] = (2e7, 1.5e7, 1e7)
+    return casters, fset, L, fr
+
+
+def test_ragged_faces_straddling_blocks_match_oracle():
+    casters, fset, L, fr = _ragged_case()
+    assert fset.nsamples > 2048 and len(set(np.diff(fset.face_sample_ptr).tolist())) > 10
+    ldr, _, _ = og.gather(casters, light.pack_desc(fset, L, None, 0.0, 1, None), fset.nsamples)
+    want, want_counts = oss.supersample(casters, fset, L, None, 0.0, 1, None, fr, ldr)
+    tr, fl = open_lighter(casters)
+    with tr:
+        fl.load(fset, L, sun_samples=1)
+        fl.run()
+        counts = fl.supersample(fr)
+        got = fl.read()
+    print(f"[supersample] ragged: {fset.nfaces} faces, {fset.nsamples} samples, passes {counts}", flush=True)
+    assert counts == want_counts and counts[0] > 20
+    assert same(got.ldr, want)
+
+
+def test_sky_ambient_alone_skips_the_direct_resample_and_matches_oracle(rooms_case):
+    """A scene lit only by sky ambient: the direct kind of the resample has no
+    light (it is not launched), the ambient points carry everything."""
+    g, fset, _lights, pvs, frames, sky, _ldr, _hdr = rooms_case
+    L = light.make_lights(1)
+    L["type"] = light.EMIT_SKYAMBIENT
+    L["intensity"] = L["intensity_hdr"] = (4000.0, 6000.0, 9000.0)
+    g2 = rooms.room_grid(2, 2, tile=64.0, seed=2, sky_ceiling=True)
+    f2 = rooms.room_faces(g2, 16.0, bumped_every=5)
+    ff2, _ = rooms.room_final_inputs(g2, f2)
+    ldr, hdr, _ = og.gather(g2.casters, light.pack_desc(f2, L, None, 0.0, 1, sky), f2.nsamples)
+    want, want_counts = oss.supersample(g2.casters, f2, L, None, 0.0, 1, sky, ff2.faces, hdr, which=1)
+    tr, fl = open_lighter(g2.casters)
+    with tr:
+        fl.load(f2, L, sun_samples=1, sky_dirs=sky)
+        fl.run()
+        counts = fl.supersample(ff2.faces, which=1)
+        got = fl.read()
+    print(f"[supersample] sky ambient alone: passes {counts}", flush=True)
+    assert counts == want_counts and len(counts) >= 1
+    assert same(got.hdr, want)
