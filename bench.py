@@ -223,10 +223,10 @@ def batching_leg(a, log):
     fl.load(faces, L, **kw)
     fl.run("queue")  # first-use allocations outside the timing
     tune = autotune.tune_gather(fl, use_cache=False, log=log)
-    wpc = tune.waves_per_cu
+    wpc, grain = tune.waves_per_cu, tune.chunks_per_ticket
     q_ms, rays = [], 0
     for _ in range(a.batch_steps):
-        ms, _n, r = fl.timed_run("queue", wpc)
+        ms, _n, r = fl.timed_run("queue", wpc, grain)
         q_ms.append(ms)
         rays = r
     q = float(np.median(q_ms))
@@ -234,7 +234,7 @@ def batching_leg(a, log):
     # the lane slots the packets occupied
     L_ = _lib.lib()
     _lib.check(L_.vh_count_enable(tr.handle, 1))
-    fl.run("queue", wpc)
+    fl.run("queue", wpc, grain)
     nv, nt = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(L_.vh_count_read(tr.handle, ctypes.byref(nv), ctypes.byref(nt)))
     ns, ts = ctypes.c_int64(), ctypes.c_int64()
@@ -255,13 +255,14 @@ def batching_leg(a, log):
     ref = fl.read()
     pf_ms, _n, pf_rays = fl.timed_run("per_face")
     same = bool(np.array_equal(fl.read().ldr.view(np.uint32), ref.ldr.view(np.uint32)))
-    log(f"[bench] batching queue (waves/CU {wpc}): {q:.1f} ms, {rays / (q / 1e3) / 1e9:.2f} Grays/s; "
+    log(f"[bench] batching queue (waves/CU {wpc}, {grain} chunks/ticket): {q:.1f} ms, {rays / (q / 1e3) / 1e9:.2f} Grays/s; "
         f"per-face launches {pf_ms:.1f} ms; bitwise equal: {same}")
     del ref
     out = {
         "faces": faces.nfaces, "luxels": faces.nsamples, "rays_nominal": nominal, "rays_traced": rays,
-        "queue_ms": q, "queue_rays_per_s": rays / (q / 1e3), "waves_per_cu": wpc,
+        "queue_ms": q, "queue_rays_per_s": rays / (q / 1e3), "waves_per_cu": wpc, "chunks_per_ticket": grain,
         "tune_ms": {str(k): round(v, 3) for k, v in tune.ms.items()},
+        "tune_grain_ms": {k: round(v, 3) for k, v in tune.grain_ms.items()},
         "per_face_ms": pf_ms, "per_face_rays_per_s": pf_rays / (pf_ms / 1e3),
         "queue_speedup": pf_ms / q, "queue_equals_per_face": same, "roofline": roofline,
     }
@@ -270,7 +271,7 @@ def batching_leg(a, log):
     sub = np.sort(np.random.default_rng(5).choice(faces.nfaces, nsub, replace=False))
     fs = faces.subset(sub)
     fl.load(fs, L, **kw)
-    _ms, _n, sub_rays = fl.timed_run("queue", wpc)
+    _ms, _n, sub_rays = fl.timed_run("queue", wpc, grain)
     got = fl.read()
     tr.dispose()
     if not a.no_cpu:

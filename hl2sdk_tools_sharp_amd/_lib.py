@@ -133,6 +133,7 @@ SIGNATURES = {
     "vh_bounce": (ctypes.c_int, [P, ctypes.POINTER(VhBounceOpts), P, P, P, P, P, ctypes.POINTER(I32)]),
     "vh_gather_load": (ctypes.c_int, [P, P]),
     "vh_gather_run": (ctypes.c_int, [P, I32, I32]),
+    "vh_gather_run_shaped": (ctypes.c_int, [P, I32, I32, I32]),
     "vh_gather_read": (ctypes.c_int, [P, P, P, P]),
     "vh_gather_styles": (ctypes.c_int, [P, P, ctypes.POINTER(I32), ctypes.POINTER(I64)]),
     "vh_gather_read_slot": (ctypes.c_int, [P, I32, P, P]),

@@ -506,9 +506,12 @@ __device__ __forceinline__ void light_chunk(const GatherArgs &a, int64_t chunk, 
         light_chunk_n<1, COUNT>(a, chunk, stk);
 }
 
+// PERSISTENT: each wave takes `grain` consecutive chunks per ticket until the
+// queue drains (every wave reads the ticket past the end and leaves).
 template <bool PERSISTENT, bool COUNT = false>
 __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first_chunk,
-                                                     unsigned long long *__restrict__ ticket)
+                                                     unsigned long long *__restrict__ ticket,
+                                                     int32_t grain = 1)
 {
     __shared__ int stk[3 * kMaxDepth];  // the wave's traversal stack
     if (!PERSISTENT) {
@@ -517,10 +520,11 @@ __global__ __launch_bounds__(kWave) void k_facelight(GatherArgs a, int64_t first
     }
     for (;;) {
         unsigned long long c = 0;
-        if (threadIdx.x == 0) c = atomicAdd(ticket, 1ull);
+        if (threadIdx.x == 0) c = atomicAdd(ticket, (unsigned long long)grain);
         c = __shfl(c, 0, 64);
         if ((int64_t)c >= a.nchunks) break;
-        light_chunk<COUNT>(a, (int64_t)c, stk);
+        for (int32_t j = 0; j < grain && (int64_t)c + j < a.nchunks; j++)
+            light_chunk<COUNT>(a, (int64_t)c + j, stk);
     }
 }
 
@@ -1018,6 +1022,11 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
 
 int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
 {
+    return vh_gather_run_shaped(c, schedule, waves_per_cu, 1);
+}
+
+int vh_gather_run_shaped(vh_ctx *c, int32_t schedule, int32_t waves_per_cu, int32_t chunks_per_ticket)
+{
     clear_error();
     if (!c) {
         set_error("NULL context");
@@ -1034,6 +1043,10 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
     }
     if (schedule < VH_SCHEDULE_QUEUE || schedule > VH_SCHEDULE_GRID) {
         set_error("bad schedule");
+        return VH_EINVAL;
+    }
+    if (chunks_per_ticket < 1 || chunks_per_ticket > 64) {
+        set_error("chunks_per_ticket runs from 1 to 64");
         return VH_EINVAL;
     }
     VH_HIP(hipSetDevice(c->device));
@@ -1056,10 +1069,10 @@ int vh_gather_run(vh_ctx *c, int32_t schedule, int32_t waves_per_cu)
             if (waves > G->nchunks) waves = G->nchunks;
             if (c->counting && c->counters.ptr)
                 hipLaunchKernelGGL((k_facelight<true, true>), dim3((uint32_t)waves), dim3(kWave), 0, st, a,
-                                   (int64_t)0, (unsigned long long *)G->ticket.ptr);
+                                   (int64_t)0, (unsigned long long *)G->ticket.ptr, chunks_per_ticket);
             else
                 hipLaunchKernelGGL(k_facelight<true>, dim3((uint32_t)waves), dim3(kWave), 0, st, a,
-                                   (int64_t)0, (unsigned long long *)G->ticket.ptr);
+                                   (int64_t)0, (unsigned long long *)G->ticket.ptr, chunks_per_ticket);
         } else if (schedule == VH_SCHEDULE_GRID) {
             hipLaunchKernelGGL(k_facelight<false>, dim3((uint32_t)G->nchunks), dim3(kWave), 0, st,
                                a, (int64_t)0, nullptr);

@@ -238,9 +238,10 @@ class HipFaceLighter:
         self.faces = faces
         HipRayTracer._raise(_lib.lib().vh_gather_load(self.tracer.handle, ctypes.byref(d)))
 
-    def run(self, schedule: str = "queue", waves_per_cu: int = 0) -> None:
+    def run(self, schedule: str = "queue", waves_per_cu: int = 0, chunks_per_ticket: int = 1) -> None:
         code = {"queue": 0, "per_face": 1, "grid": 2}[schedule]
-        HipRayTracer._raise(_lib.lib().vh_gather_run(self.tracer.handle, code, int(waves_per_cu)))
+        HipRayTracer._raise(_lib.lib().vh_gather_run_shaped(self.tracer.handle, code, int(waves_per_cu),
+                                                            int(chunks_per_ticket)))
 
     def read(self) -> GatherResult:
         S = self.faces.nsamples
@@ -304,7 +305,7 @@ class HipFaceLighter:
             _lib.check(L.vh_timing_enable(h, 0))
         return counts, ms.value, self.tracer.stats()["rays_gather"] - before
 
-    def timed_run(self, schedule: str = "queue", waves_per_cu: int = 0) -> tuple:
+    def timed_run(self, schedule: str = "queue", waves_per_cu: int = 0, chunks_per_ticket: int = 1) -> tuple:
         """One run with the gather kernel timed by HIP events on the library's
         stream: (kernel ms summed over its launches, launches, rays traced)."""
         L = _lib.lib()
@@ -312,7 +313,7 @@ class HipFaceLighter:
         before = self.tracer.stats()["rays_gather"]
         _lib.check(L.vh_timing_enable(h, 1))
         try:
-            self.run(schedule, waves_per_cu)
+            self.run(schedule, waves_per_cu, chunks_per_ticket)
             ms, n = ctypes.c_double(), ctypes.c_int64()
             _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_FACELIGHT, ctypes.byref(ms), ctypes.byref(n)))
         finally:

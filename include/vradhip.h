@@ -341,6 +341,12 @@ int vh_gather_load(vh_ctx *ctx, const vh_gather_desc *desc);
  * traced inline.  Results stay on the device; vh_gather_read copies them.
  * waves_per_cu sizes the persistent launch (0 = a default). */
 int vh_gather_run(vh_ctx *ctx, int32_t schedule, int32_t waves_per_cu);
+/* vh_gather_run with the queue's grain: each persistent wave takes
+ * chunks_per_ticket consecutive 64-sample chunks per ticket (1-64; 1 is
+ * vh_gather_run).  Larger grains take fewer atomics and keep a wave on one
+ * face longer; smaller ones balance the queue's tail.  Ignored by the other
+ * schedules.  The autotuner picks it with waves_per_cu. */
+int vh_gather_run_shaped(vh_ctx *ctx, int32_t schedule, int32_t waves_per_cu, int32_t chunks_per_ticket);
 
 /* Supersampling, FaceLightJob's passes after the gather (StyleSupersample,
  * EmitSupersamplePoint / ResolveSupersampleItem,

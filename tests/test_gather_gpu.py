@@ -117,8 +117,13 @@ def test_schedules_bitwise_equal_and_rerun_is_idempotent(c1):
     tr, fl = open_lighter(g.casters)
     with tr:
         fl.load(faces, lights, **kw)
-        outs = [fl.light_faces(schedule=s, waves_per_cu=w)
-                for s, w in (("queue", 0), ("queue", 1), ("queue", 7), ("grid", 0), ("per_face", 0), ("queue", 0))]
+        outs = [fl.light_faces(schedule=s, waves_per_cu=w, chunks_per_ticket=k)
+                for s, w, k in (("queue", 0, 1), ("queue", 1, 1), ("queue", 7, 1), ("grid", 0, 1),
+                                ("per_face", 0, 1), ("queue", 0, 1), ("queue", 4, 2), ("queue", 10, 7),
+                                ("queue", 2, 64))]
+        for k in (0, 65):
+            with pytest.raises(ValueError):
+                fl.run("queue", 4, k)
     for o in outs[1:]:
         assert np.array_equal(o.ldr.view(np.uint32), outs[0].ldr.view(np.uint32))
         assert np.array_equal(o.hdr.view(np.uint32), outs[0].hdr.view(np.uint32))
