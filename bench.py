@@ -441,16 +441,33 @@ def warm_start_leg(a, log):
     _lib.check(L.vh_timing_enable(tr.handle, 0))
     steps = (rows // a.mlp_batch) * a.mlp_epochs
     flop = 6.0 * 36352 * a.mlp_batch * steps  # 2 forward + 4 backward FLOP per weight per row
-    log(f"[bench] warm start: {rows} rows from {len(train)} maps in {data_s:.1f} s; {steps} steps in "
-        f"{train_s:.2f} s ({flop / (ms.value / 1e3) / 1e12:.1f} TFLOP/s over the training launches); "
-        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    params = net.params()
+    # the per-layer kernels on the same data and seed: the same parameters, bit for bit
+    net2 = WarmStartMLP(tr, params=init_params(0), gain=prior.gain, scale=prior.scale)
+    _lib.check(L.vh_timing_enable(tr.handle, 1))
+    t0 = time.perf_counter()
+    net2.train(epochs=a.mlp_epochs, batch=a.mlp_batch, lr=a.mlp_lr, seed=1, fused=False)
+    layer_s = time.perf_counter() - t0
+    ms2 = ctypes.c_double()
+    _lib.check(L.vh_timing_read(tr.handle, _lib.VH_KERNEL_MLP, ctypes.byref(ms2), ctypes.byref(n)))
+    _lib.check(L.vh_timing_enable(tr.handle, 0))
+    same_params = bool(np.array_equal(params.view(np.uint32), net2.params().view(np.uint32)))
+    del net2
+    net = WarmStartMLP(tr, params=params, gain=prior.gain, scale=prior.scale)
     tflops = flop / (ms.value / 1e3) / 1e12
+    us_step = ms.value * 1e3 / max(steps, 1)
+    log(f"[bench] warm start: {rows} rows from {len(train)} maps in {data_s:.1f} s; {steps} steps in "
+        f"{train_s:.2f} s, {us_step:.1f} us per step on the device ({tflops:.1f} TFLOP/s); per-layer kernels "
+        f"{ms2.value * 1e3 / max(steps, 1):.1f} us per step, parameters bit-equal: {same_params}; "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     out = {"train_maps": len(train), "test_maps": len(test), "rows": rows, "epochs": a.mlp_epochs,
            "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_BF16_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 5), "traffic": None,
-                        "kernel": "the training step's launches (k_gemm, k_gemm_dw, k_dw_reduce, k_adam, k_batch)",
+                        "kernel": "the fused training step (k_step_rows + k_step_weights, one hipGraph per step)",
                         "kernel_ms_per_step": round(ms.value / max(steps, 1), 5),
                         "flop_per_step": 6.0 * 36352 * a.mlp_batch},
+           "us_per_step": us_step, "per_layer_us_per_step": ms2.value * 1e3 / max(steps, 1),
+           "per_layer_train_s": layer_s, "fused_equals_per_layer": same_params,
            "batch": a.mlp_batch, "train_s": train_s, "train_tflops": tflops,
            "data_s": data_s, "generate_s": gen_s, "prior_gain": prior.gain, "prior_scale": prior.scale,
            "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "maps": []}

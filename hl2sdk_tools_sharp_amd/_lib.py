@@ -101,7 +101,7 @@ class VhMlpTrainOpts(ctypes.Structure):
     _fields_ = [
         ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("seed", ctypes.c_int32),
         ("use_graph", ctypes.c_int32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
-        ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("fused", ctypes.c_int32),
     ]
 
 

@@ -142,16 +142,25 @@ struct GemmArgs {
 
 enum { EPI_GELU = 0, EPI_OUT = 1, EPI_DGELU = 2, EPI_STORE = 3 };
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1), with the hardware exponential and
+// reciprocal: absolute error a few 1e-7 (the activations are stored as BF16,
+// 2^-8 relative), exactly +-1 where exp(2u) over- or underflows.
+__device__ __forceinline__ float fast_tanh(float u)
+{
+    const float e = __expf(2.0f * u);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
 __device__ __forceinline__ float gelu(float z)
 {
     float u = kGeluC * (z + ((0.044715f * z) * z) * z);
-    return (0.5f * z) * (1.0f + tanhf(u));
+    return (0.5f * z) * (1.0f + fast_tanh(u));
 }
 
 __device__ __forceinline__ float gelu_grad(float z)
 {
     float u = kGeluC * (z + ((0.044715f * z) * z) * z);
-    float t = tanhf(u);
+    float t = fast_tanh(u);
     float du = kGeluC * (1.0f + ((3.0f * 0.044715f) * z) * z);
     return (0.5f * (1.0f + t)) + ((0.5f * z) * (1.0f - t * t)) * du;
 }
@@ -279,21 +288,38 @@ struct AdamArgs {
     int32_t inp[kLayers], outp[kLayers];
 };
 
-// Adam (Kingma and Ba, 2015, Algorithm 1) per FP32 parameter, then the BF16
-// copies of the weights in both layouts.
+// One Adam step (Kingma and Ba, 2015, Algorithm 1) of an FP32 parameter w
+// with moments m, v and gradient gi; bc1, bc2 are the bias corrections
+// 1 - beta^t.  Updates w, m, v in place.
+__device__ __forceinline__ void adam_math(float &w, float &m, float &v, float gi, const AdamArgs &a, float bc1,
+                                          float bc2)
+{
+    const float mi = a.beta1 * m + (1.0f - a.beta1) * gi;
+    const float vi = a.beta2 * v + (1.0f - a.beta2) * (gi * gi);
+    m = mi;
+    v = vi;
+    w = w - a.lr * (mi / bc1) / (sqrtf(vi / bc2) + a.eps);
+}
+
+__device__ __forceinline__ void adam_corrections(const AdamArgs &a, float &bc1, float &bc2, int32_t ahead = 0)
+{
+    const double t = (double)(a.state[0] + ahead + 1);
+    bc1 = (float)(1.0 - pow((double)a.beta1, t));
+    bc2 = (float)(1.0 - pow((double)a.beta2, t));
+}
+
+// Adam per FP32 parameter, then the BF16 copies of the weights in both
+// layouts.
 __global__ __launch_bounds__(256) void k_adam(AdamArgs a)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
-    const double t = (double)(a.state[0] + 1);
-    const float bc1 = (float)(1.0 - pow((double)a.beta1, t));
-    const float bc2 = (float)(1.0 - pow((double)a.beta2, t));
-    const float gi = a.g[i];
-    const float mi = a.beta1 * a.m[i] + (1.0f - a.beta1) * gi;
-    const float vi = a.beta2 * a.v[i] + (1.0f - a.beta2) * (gi * gi);
-    a.m[i] = mi;
-    a.v[i] = vi;
-    const float w = a.w[i] - a.lr * (mi / bc1) / (sqrtf(vi / bc2) + a.eps);
+    float bc1, bc2;
+    adam_corrections(a, bc1, bc2);
+    float w = a.w[i], m = a.m[i], v = a.v[i];
+    adam_math(w, m, v, a.g[i], a, bc1, bc2);
+    a.m[i] = m;
+    a.v[i] = v;
     a.w[i] = w;
     for (int l = 0; l < kLayers; l++) {
         const int64_t off = i - a.woff[l];
@@ -528,6 +554,373 @@ __global__ __launch_bounds__(256) void k_cluster_start(const float *__restrict__
             o[k] = use_prior[k] ? gain * ev[k] : q * f[k];
         }
         start[p] = make_float4(o[0], o[1], o[2], 0.0f);
+    }
+}
+
+// ---- The fused training step: two launches per step ----------------------
+//
+// k_step_rows: a block per 16 batch rows (eight waves; 4,096 rows fill the
+// 256 CUs).  It draws its rows (the affine permutation of k_batch), runs the
+// forward pass through all four layers with the activations and the hidden
+// layers' FP32 pre-activations in LDS, forms the loss and the output
+// gradient, and runs the backward data pass down to the first hidden layer;
+// each layer's input and output gradient go to HBM K-blocked for the weight
+// gradients.  Wave w owns column tile w (one 16 x 16 MFMA tile) of every
+// 128-wide layer; waves 0 and 1 own the two tiles of the 32-wide output
+// layer.  The epilogues (GELU, GELU') are most of the work, four values a
+// lane per layer.
+//
+// k_step_weights: a block per 16 x 16 tile of the four weight gradients.  Its
+// waves share the row splits of k_gemm_dw (each split summed by one wave, in
+// the same K order); wave 0 sums the splits in split order (as k_dw_reduce),
+// so the gradient is bit-identical to the per-layer kernels', applies Adam to
+// the tile's weights (and the layer's bias gradient, for the tiles of row 0),
+// and writes their BF16 copies.  The last block to finish sums the blocks'
+// loss partials and advances the step counters.
+//
+// Every MFMA sees the operands, the K order and the epilogue arithmetic of the
+// per-layer kernels (k_batch, k_gemm, k_loss, k_gemm_dw, k_dw_reduce, k_adam),
+// so the two paths train to the same bits; only the loss value, a sum of
+// squares reported per step, is summed in another order.
+
+constexpr int kFRows = 16;               // batch rows per k_step_rows block
+constexpr int kFWaves = kHidden / 16;    // waves per k_step_rows block: a column tile each
+constexpr int kActStride = kHidden + 8;  // LDS row stride of an activation (BF16)
+constexpr int kZStride = kHidden + 4;    // LDS row stride of a pre-activation (FP32)
+
+struct StepRowsArgs {
+    const __bf16 *X;
+    const float *T;
+    int64_t n;
+    int32_t nb;
+    float gain, inv_scale;
+    const int64_t *state;
+    int32_t ahead;  // this step's place after the counters' step
+    const __bf16 *wb, *wtb;
+    const float *master;
+    int64_t boff[kLayers], wtoff[kLayers];
+    __bf16 *hk[kLayers], *dzk[kLayers];
+    double *loss_part;  // this step's per-block partials
+};
+
+// The B fragments of a wave's NC column tiles (ct0 ..) over all K / 32 steps,
+// B(k, n) = b[n * K + k]: loaded ahead of the layer that uses them, so their
+// global-memory latency overlaps the layer before.
+template <int K, int NC>
+struct BFrags {
+    bf16x8 f[K / 32][NC];
+};
+
+template <int K, int NC>
+__device__ __forceinline__ void load_b(BFrags<K, NC> &bf, const __bf16 *__restrict__ b, int ct0)
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, kg = (lane >> 4) * 8;
+#pragma unroll
+    for (int s = 0; s < K / 32; s++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) bf.f[s][c] = *(const bf16x8 *)(b + (int64_t)((ct0 + c) * 16 + r) * K + s * 32 + kg);
+}
+
+// 16 x 16 MFMA tiles of C = A . B for the wave's NR x NC tiles (row tiles
+// rt0 .. rt0 + NR - 1): A from LDS (row-major, kActStride), K in steps of 32
+// from 0 up, one MFMA per tile and step.
+template <int NR, int NC, int K>
+__device__ __forceinline__ void tiles_mma(const __bf16 *A, const BFrags<K, NC> &bf, int rt0,
+                                          f32x4 (&acc)[NR][NC])
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, kg = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < NR; i++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[i][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < K / 32; s++) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const bf16x8 af = *(const bf16x8 *)(A + ((rt0 + i) * 16 + r) * kActStride + s * 32 + kg);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf.f[s][c], acc[i][c], 0, 0, 0);
+        }
+    }
+}
+
+// Rows i .. i + 3 (batch rows, i a multiple of 4) of column `col` into the
+// K-blocked copy of an ncol-wide matrix: four consecutive slots.
+__device__ __forceinline__ void store_blocked(__bf16 *dst, int64_t i, int ncol, int col, const __bf16 (&v)[4])
+{
+    *(uint2 *)(dst + ((i >> 3) * ncol + col) * 8 + (i & 7)) = *(const uint2 *)v;
+}
+
+// Hidden layer L forward for the wave's tile (column tile w): z into zl
+// (LDS), GELU(z) into dst and the K-blocked copy.
+template <int L>
+__device__ __forceinline__ void step_forward(const StepRowsArgs &a, const BFrags<kInP[L], 1> &bf,
+                                             const __bf16 *src, __bf16 *dst, float *zl, int64_t r0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, rq = (lane >> 4) * 4;
+    f32x4 acc[1][1];
+    tiles_mma<1, 1, kInP[L]>(src, bf, 0, acc);
+    const int col = wave * 16 + r;
+    const float bias = a.master[a.boff[L] + col];
+    __bf16 hv[4];
+    for (int i = 0; i < 4; i++) {
+        const int row = rq + i;
+        const float zz = acc[0][0][i] + bias;
+        zl[row * kZStride + col] = zz;
+        hv[i] = (__bf16)gelu(zz);
+        dst[row * kActStride + col] = hv[i];
+    }
+    store_blocked(a.hk[L + 1], r0 + rq, kHidden, col, hv);
+}
+
+// Backward data through layer L: dZ_{L-1} = (dZ_L W_L^T) * GELU'(z_{L-1}) for
+// the wave's tile, into dst (unless null) and the K-blocked copy.
+template <int L>
+__device__ __forceinline__ void step_backward(const StepRowsArgs &a, const BFrags<kOutP[L], 1> &bf,
+                                              const __bf16 *src, __bf16 *dst, const float *zl, int64_t r0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, rq = (lane >> 4) * 4;
+    f32x4 acc[1][1];
+    tiles_mma<1, 1, kOutP[L]>(src, bf, 0, acc);
+    const int col = wave * 16 + r;
+    __bf16 hv[4];
+    for (int i = 0; i < 4; i++) {
+        const int row = rq + i;
+        hv[i] = (__bf16)(acc[0][0][i] * gelu_grad(zl[row * kZStride + col]));
+        if (dst) dst[row * kActStride + col] = hv[i];
+    }
+    store_blocked(a.dzk[L - 1], r0 + rq, kHidden, col, hv);
+}
+
+__global__ __launch_bounds__(64 * kFWaves) void k_step_rows(StepRowsArgs a)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 act[2][kFRows * kActStride];
+    __shared__ float zs[kLayers - 1][kFRows * kZStride];
+    __shared__ float tgt[kFRows * kOut];
+    __shared__ double lred[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, rq = (lane >> 4) * 4;
+    const int64_t r0 = (int64_t)blockIdx.x * kFRows;
+    const int oc = wave & 1;  // waves 0 and 1: the output layer's column tiles
+
+    // each layer's weight fragments are loaded one layer ahead
+    BFrags<kInPad, 1> f0;
+    load_b(f0, a.wtb + a.wtoff[0], wave);
+    BFrags<kHidden, 1> f1;
+    load_b(f1, a.wtb + a.wtoff[1], wave);
+
+    // the batch rows (k_batch): four threads per row, 16 bytes each
+    if (tid < kFRows * 4) {
+        const int i = tid >> 2, q = tid & 3;
+        const uint64_t am = (uint64_t)a.state[2], c = (uint64_t)a.state[3];
+        const uint64_t pos = (uint64_t)(a.state[1] + a.ahead) * (uint64_t)a.nb + (uint64_t)(r0 + i);
+        const uint64_t j = (am * pos + c) % (uint64_t)a.n;
+        const uint4 v = ((const uint4 *)(a.X + j * kInPad))[q];
+        *(uint4 *)&act[0][i * kActStride + q * 8] = v;
+        const __bf16 *e = (const __bf16 *)&v;
+        const int64_t row = r0 + i;
+        for (int k = 0; k < 8; k++) a.hk[0][((row >> 3) * kInPad + q * 8 + k) * 8 + (row & 7)] = e[k];
+        if (q == 0)
+            for (int k = 0; k < kOut; k++)
+                tgt[i * kOut + k] = (a.T[j * kLight + kOut + k] - a.gain * a.T[j * kLight + k]) * a.inv_scale;
+    }
+    __syncthreads();
+
+    // forward through the hidden layers, the pre-activations kept
+    step_forward<0>(a, f0, act[0], act[1], zs[0], r0);
+    BFrags<kHidden, 1> f2;
+    load_b(f2, a.wtb + a.wtoff[2], wave);
+    __syncthreads();
+    step_forward<1>(a, f1, act[1], act[0], zs[1], r0);
+    BFrags<kHidden, 1> f3;
+    load_b(f3, a.wtb + a.wtoff[3], oc);
+    __syncthreads();
+    step_forward<2>(a, f2, act[0], act[1], zs[2], r0);
+    BFrags<kOutPad, 1> g3;
+    load_b(g3, a.wb + a.wtoff[3], wave);
+    __syncthreads();
+
+    // output layer (waves 0 and 1), loss and its gradient (k_loss): dy into
+    // act[0]
+    if (wave < 2) {
+        double lacc = 0.0;
+        f32x4 acc[1][1];
+        tiles_mma<1, 1, kHidden>(act[1], f3, 0, acc);
+        const float scale = 2.0f / (3.0f * (float)a.nb);
+        const int col = oc * 16 + r;
+        const float bias = a.master[a.boff[kLayers - 1] + col];
+        __bf16 gv[4];
+        for (int i = 0; i < 4; i++) {
+            const int row = rq + i;
+            float d = 0.0f;
+            if (col < kOut) {
+                d = (acc[0][0][i] + bias) - tgt[row * kOut + col];
+                lacc += (double)d * (double)d;
+            }
+            gv[i] = (__bf16)(d * scale);
+            act[0][row * kActStride + col] = gv[i];
+        }
+        store_blocked(a.dzk[kLayers - 1], r0 + rq, kOutPad, col, gv);
+        for (int off = 32; off > 0; off >>= 1) lacc += __shfl_xor(lacc, off, 64);
+        if (lane == 0) lred[wave] = lacc;
+    }
+    BFrags<kHidden, 1> g2;
+    load_b(g2, a.wb + a.wtoff[2], wave);
+    __syncthreads();
+    if (tid == 0) a.loss_part[blockIdx.x] = lred[0] + lred[1];
+
+    // backward data down to the first hidden layer
+    step_backward<3>(a, g3, act[0], act[1], zs[2], r0);
+    BFrags<kHidden, 1> g1;
+    load_b(g1, a.wb + a.wtoff[1], wave);
+    __syncthreads();
+    step_backward<2>(a, g2, act[1], act[0], zs[1], r0);
+    __syncthreads();
+    step_backward<1>(a, g1, act[0], nullptr, zs[0], r0);
+}
+
+struct StepWeightsArgs {
+    const __bf16 *hk[kLayers], *dzk[kLayers];
+    int32_t K, kc, splits;
+    int32_t ahead;  // this step's place after the counters' step
+    AdamArgs adam;
+};
+
+constexpr int kWWaves = 16;  // waves per k_step_weights block
+constexpr int kWTiles = 2 * 16 + 2 * 64;  // 16 x 16 tiles of the four weight gradients
+
+// The tile of block b, XCD-aware: blocks go round-robin over the eight XCDs
+// (b % 8), and the 20 tiles an XCD gets share their operand slices -- per
+// 128 x 128 layer a 2 x 4 patch of tiles (two A and four B column slices
+// through its L2), per 32-wide layer the tiles of one B (or A) slice.
+__device__ __forceinline__ void weight_tile(int b, int &l, int &mt, int &nt)
+{
+    const int x = b & 7, j = b >> 3;
+    if (j < 16) {
+        l = 1 + (j >> 3);
+        const int q = j & 7;
+        mt = 2 * (x & 3) + (q & 1);
+        nt = 4 * (x >> 2) + (q >> 1);
+    } else if (j < 18) {
+        l = 0;
+        mt = j - 16;
+        nt = x;
+    } else {
+        l = kLayers - 1;
+        mt = x;
+        nt = j - 18;
+    }
+}
+
+__global__ __launch_bounds__(64 * kWWaves) void k_step_weights(StepWeightsArgs a)
+{
+    __shared__ f32x4 part[kMaxSplits][64];
+    __shared__ float cpart[kMaxSplits][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int l, mt, nt;
+    weight_tile(blockIdx.x, l, mt, nt);
+    const int M = kInP[l], N = kOutP[l];
+    const int m0 = mt * 16, n0 = nt * 16;
+    const int r = lane & 15, kg = lane >> 4;
+    const int col = n0 + r;
+    const AdamArgs &ad = a.adam;
+    // wave 0's parameters and their Adam state, loaded ahead of the sums
+    float pw[4], pm[4], pv[4], bw = 0.0f, bm = 0.0f, bv = 0.0f;
+    const bool has_bias = m0 == 0 && lane < 16;
+    if (wave == 0) {
+        for (int i = 0; i < 4; i++) {
+            const int64_t p = ad.woff[l] + (int64_t)(m0 + kg * 4 + i) * N + col;
+            pw[i] = ad.w[p];
+            pm[i] = ad.m[p];
+            pv[i] = ad.v[p];
+        }
+        if (has_bias) {
+            bw = ad.w[ad.boff[l] + col];
+            bm = ad.m[ad.boff[l] + col];
+            bv = ad.v[ad.boff[l] + col];
+        }
+    }
+    // the splits of k_gemm_dw, one wave each, eight steps of 32 rows loaded
+    // at a time (a step past the split's end reloads its last step and is
+    // skipped)
+    const __bf16 *A = a.hk[l], *B = a.dzk[l];
+    for (int sp = wave; sp < a.splits; sp += kWWaves) {
+        const int32_t ka = sp * a.kc, kb = min(a.K, ka + a.kc);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        float cs = 0.0f;
+        for (int32_t kbase = ka; kbase < kb; kbase += 256) {
+            bf16x8 af[8], bfr[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int32_t k0 = min(kbase + 32 * q, kb - 32);
+                const int64_t blk = (k0 >> 3) + kg;
+                af[q] = *(const bf16x8 *)(A + (blk * M + m0 + r) * 8);
+                bfr[q] = *(const bf16x8 *)(B + (blk * N + n0 + r) * 8);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (kbase + 32 * q < kb) {
+                    for (int j = 0; j < 8; j++) cs += (float)bfr[q][j];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[q], bfr[q], acc, 0, 0, 0);
+                }
+            }
+        }
+        cs += __shfl_xor(cs, 16, 64);
+        cs += __shfl_xor(cs, 32, 64);
+        part[sp][lane] = acc;
+        cpart[sp][lane] = cs;
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    // the splits in split order (k_dw_reduce), then Adam (k_adam)
+    f32x4 g = {0.0f, 0.0f, 0.0f, 0.0f};
+    float gb = 0.0f;
+    for (int sp = 0; sp < a.splits; sp++) {
+        const f32x4 p = part[sp][lane];
+        for (int i = 0; i < 4; i++) g[i] += p[i];
+        gb += cpart[sp][lane];
+    }
+    float bc1, bc2;
+    adam_corrections(ad, bc1, bc2, a.ahead);
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + kg * 4 + i;
+        const int64_t off = (int64_t)row * N + col;
+        adam_math(pw[i], pm[i], pv[i], g[i], ad, bc1, bc2);
+        ad.w[ad.woff[l] + off] = pw[i];
+        ad.m[ad.woff[l] + off] = pm[i];
+        ad.v[ad.woff[l] + off] = pv[i];
+        ad.wb[ad.wtoff[l] + off] = (__bf16)pw[i];
+        ad.wtb[ad.wtoff[l] + (int64_t)col * M + row] = (__bf16)pw[i];
+    }
+    if (has_bias) {
+        adam_math(bw, bm, bv, gb, ad, bc1, bc2);
+        ad.w[ad.boff[l] + col] = bw;
+        ad.m[ad.boff[l] + col] = bm;
+        ad.v[ad.boff[l] + col] = bv;
+    }
+}
+
+// After a run of `steps` fused steps: each step's loss from its blocks'
+// partials (a wave per step: lane-strided sums, then a fixed shuffle tree),
+// then the counters advance past the run.
+__global__ __launch_bounds__(256) void k_step_tail(const double *__restrict__ part, int32_t nparts,
+                                                   int32_t steps, int32_t nb, float *__restrict__ losses,
+                                                   int64_t *__restrict__ state)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t slot = state[4];
+    for (int k = wave; k < steps; k += 4) {
+        double sum = 0.0;
+        for (int b = lane; b < nparts; b += 64) sum += part[(int64_t)k * nparts + b];
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == 0) losses[slot + k] = (float)(sum / (3.0 * (double)nb));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        state[0] += steps;
+        state[1] += steps;
+        state[4] += steps;
     }
 }
 
@@ -1119,6 +1512,55 @@ int vh_mlp_train(vh_ctx *c, const vh_mlp_train_opts *o, float *epoch_loss)
     aa.eps = o->eps;
     aa.state = (const int64_t *)D->state.ptr;
     int64_t *state = (int64_t *)D->state.ptr;
+    const bool fused = o->fused != 0;
+    StepRowsArgs ra{};
+    StepWeightsArgs wa{};
+    constexpr int32_t kRun = 8;  // fused steps per graph: the counters advance once per run
+    const int32_t nparts = (int32_t)(nb / kFRows);
+    if (fused) {
+        if ((rc = D->loss_part.ensure((size_t)kRun * nparts * 8 + 8, "loss partials"))) return rc;
+        ra.X = (const __bf16 *)D->X.ptr;
+        ra.T = (const float *)D->T.ptr;
+        ra.n = n;
+        ra.nb = (int32_t)nb;
+        ra.gain = D->gain;
+        ra.inv_scale = 1.0f / D->scale;
+        ra.state = state;
+        ra.wb = (const __bf16 *)D->wb.ptr;
+        ra.wtb = (const __bf16 *)D->wtb.ptr;
+        ra.master = (const float *)D->master.ptr;
+        for (int l = 0; l < kLayers; l++) {
+            ra.boff[l] = D->L.b[l];
+            ra.wtoff[l] = D->L.wt[l];
+            ra.hk[l] = (__bf16 *)D->hk[l].ptr;
+            ra.dzk[l] = (__bf16 *)D->dzk[l].ptr;
+            wa.hk[l] = ra.hk[l];
+            wa.dzk[l] = ra.dzk[l];
+        }
+        // the row splits of the per-layer weight gradient (backward())
+        const int32_t K = (int32_t)nb;
+        int32_t splits = K / 256;
+        splits = splits < 1 ? 1 : (splits > kMaxSplits ? kMaxSplits : splits);
+        const int32_t kc = ((K + splits - 1) / splits + 31) / 32 * 32;
+        wa.K = K;
+        wa.kc = kc;
+        wa.splits = (K + kc - 1) / kc;
+        wa.adam = aa;
+    }
+    // `count` fused steps (at most kRun), then the losses and the counters
+    auto fused_run = [&](int32_t count) {
+        for (int32_t k = 0; k < count; k++) {
+            StepRowsArgs r = ra;
+            r.ahead = k;
+            r.loss_part = (double *)D->loss_part.ptr + (int64_t)k * nparts;
+            StepWeightsArgs w = wa;
+            w.ahead = k;
+            hipLaunchKernelGGL(k_step_rows, dim3((uint32_t)nparts), dim3(64 * kFWaves), 0, s, r);
+            hipLaunchKernelGGL(k_step_weights, dim3((uint32_t)kWTiles), dim3(64 * kWWaves), 0, s, w);
+        }
+        hipLaunchKernelGGL(k_step_tail, dim3(1), dim3(256), 0, s, (const double *)D->loss_part.ptr, nparts,
+                           count, (int32_t)nb, (float *)D->losses.ptr, state);
+    };
     auto step = [&]() {
         hipLaunchKernelGGL(k_batch, dim3(blocks(nb)), dim3(256), 0, s, (const __bf16 *)D->X.ptr,
                            (const float *)D->T.ptr, n, (const int64_t *)state, (int32_t)nb, D->gain,
@@ -1130,22 +1572,42 @@ int vh_mlp_train(vh_ctx *c, const vh_mlp_train_opts *o, float *epoch_loss)
         hipLaunchKernelGGL(k_adam, dim3(blocks(aa.n)), dim3(256), 0, s, aa);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, state);
     };
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // the replayed unit: one step of the per-layer kernels, or a run of kRun
+    // fused steps (and one of the epoch's remainder)
+    const int32_t unit = fused ? kRun : 1;
+    const int32_t rem = (int32_t)(spe % unit);
+    hipGraph_t graphs[2] = {nullptr, nullptr};
+    hipGraphExec_t execs[2] = {nullptr, nullptr};
+    auto launch_unit = [&](int32_t count) {
+        if (fused)
+            fused_run(count);
+        else
+            step();
+    };
+    auto release = [&]() {
+        for (int q = 0; q < 2; q++) {
+            if (execs[q]) (void)hipGraphExecDestroy(execs[q]);
+            if (graphs[q]) (void)hipGraphDestroy(graphs[q]);
+        }
+    };
     if (o->use_graph) {
-        VH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        step();
-        hipError_t e = hipStreamEndCapture(s, &graph);
-        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            if (graph) hipGraphDestroy(graph);
-            set_error(std::string("training-step graph failed: ") + hipGetErrorString(e));
-            return VH_EDEVICE;
+        for (int q = 0; q < 2; q++) {
+            const int32_t count = q == 0 ? unit : rem;
+            if (count == 0) continue;
+            VH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_unit(count);
+            hipError_t e = hipStreamEndCapture(s, &graphs[q]);
+            if (e == hipSuccess) e = hipGraphInstantiate(&execs[q], graphs[q], nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                release();
+                set_error(std::string("training-step graph failed: ") + hipGetErrorString(e));
+                return VH_EDEVICE;
+            }
         }
     }
     std::mt19937_64 rng((uint64_t)o->seed * 0x9E3779B97F4A7C15ull + 1);
     hipError_t failure = hipSuccess;
-    std::vector<int64_t> st(5);
+    std::vector<int64_t> st(6);
     for (int32_t ep = 0; ep < o->epochs && failure == hipSuccess; ep++) {
         // a multiplier coprime to n and an offset: a fresh permutation of the rows
         uint64_t a;
@@ -1153,23 +1615,25 @@ int vh_mlp_train(vh_ctx *c, const vh_mlp_train_opts *o, float *epoch_loss)
             a = rng() % (uint64_t)n;
         } while (a == 0 || std::gcd(a, (uint64_t)n) != 1);
         if (n == 1) a = 1;
-        st = {D->step + (int64_t)ep * spe, 0, (int64_t)a, (int64_t)(rng() % (uint64_t)n), (int64_t)ep * spe};
+        st = {D->step + (int64_t)ep * spe, 0, (int64_t)a, (int64_t)(rng() % (uint64_t)n), (int64_t)ep * spe, 0};
         failure = hipMemcpyAsync(state, st.data(), st.size() * 8, hipMemcpyHostToDevice, s);
         if (failure == hipSuccess) failure = hipStreamSynchronize(s);
         KernelTimer timer(c, VH_KERNEL_MLP, s);
-        for (int64_t k = 0; k < spe && failure == hipSuccess; k++) {
-            if (exec) {
-                failure = hipGraphLaunch(exec, s);
+        for (int64_t k = 0; k < spe && failure == hipSuccess;) {
+            const int32_t count = spe - k >= unit ? unit : rem;
+            const int q = count == unit ? 0 : 1;
+            if (execs[q]) {
+                failure = hipGraphLaunch(execs[q], s);
             } else {
-                step();
+                launch_unit(count);
                 failure = hipGetLastError();
             }
+            k += count;
         }
         timer.done();
     }
     if (failure == hipSuccess) failure = hipStreamSynchronize(s);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    release();
     if (failure != hipSuccess) {
         set_error(std::string("training failed: ") + hipGetErrorString(failure));
         return VH_EDEVICE;

@@ -120,10 +120,13 @@ class WarmStartMLP:
         return n.value
 
     def train(self, epochs: int = 10, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-              use_graph: bool = True, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> np.ndarray:
-        """Epochs over the data set; returns each epoch's mean batch loss."""
+              use_graph: bool = True, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8,
+              fused: bool = True) -> np.ndarray:
+        """Epochs over the data set; returns each epoch's mean batch loss.
+        ``fused`` trains with the two-launch step (rows, then weights and
+        Adam), which gives the per-layer kernels' parameters bit for bit."""
         o = _lib.VhMlpTrainOpts(int(epochs), int(batch), int(seed), 1 if use_graph else 0, float(lr),
-                                float(beta1), float(beta2), float(eps))
+                                float(beta1), float(beta2), float(eps), 1 if fused else 0)
         losses = np.zeros(epochs, np.float32)
         self._call(_lib.lib().vh_mlp_train, ctypes.byref(o), _p(losses))
         return losses

@@ -618,6 +618,8 @@ typedef struct vh_mlp_train_opts {
     int32_t seed;      /* the per-epoch row permutations */
     int32_t use_graph; /* replay one captured training step (hipGraph) */
     float lr, beta1, beta2, eps;
+    int32_t fused;     /* the two-launch step (rows, then weights + Adam): the
+                        * same parameters, bit for bit, as the per-layer kernels */
 } vh_mlp_train_opts;
 /* Trains on the data set: per epoch a fresh permutation of the rows, whole
  * batches only; epoch_loss (epochs floats, may be NULL) receives each

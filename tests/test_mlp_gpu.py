@@ -101,17 +101,22 @@ def test_training_is_deterministic_and_learns():
     offer = HipTracerFactory().try_create(g.casters)
     with offer.tracer as tr:
         out = []
-        for graph in (True, False):
+        # per-layer kernels in a graph, the same launched one by one, then the
+        # fused two-launch step (graph and eager)
+        for graph, fused in ((True, False), (False, False), (True, True), (False, True)):
             net = WarmStartMLP(tr, params=init_params(0))
             bounce_map(tr, g)
             net.dataset_clear()
             n = net.dataset_add()
-            losses = net.train(epochs=6, batch=512, lr=2e-3, seed=3, use_graph=graph)
+            losses = net.train(epochs=6, batch=512, lr=2e-3, seed=3, use_graph=graph, fused=fused)
             out.append((net.params(), losses))
-        print(f"[train] {n} rows, losses {out[0][1]}", flush=True)
+        print(f"[train] {n} rows, losses {out[0][1]}, fused {out[2][1]}", flush=True)
     assert n == int((g.leaf & ~g.sky.astype(bool)).sum())
-    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
-    assert np.array_equal(out[0][1], out[1][1])
+    for params, losses in out[1:]:
+        assert np.array_equal(params.view(np.uint32), out[0][0].view(np.uint32))
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[2][1], out[3][1])
+    # the fused step sums the loss (reported only) in another order
+    assert np.allclose(out[2][1], out[0][1], rtol=1e-6, atol=0)
     assert out[0][1][-1] < 0.5 * out[0][1][0]
 
 
