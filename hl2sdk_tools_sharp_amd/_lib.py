@@ -33,6 +33,10 @@ STATUS_NAMES = {
 }
 
 
+VH_BUILD_AUTO, VH_BUILD_HOST, VH_BUILD_DEVICE = 0, 1, 2
+VH_BUILD_DEVICE_MIN = 65536
+
+
 class VhOpenOpts(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int32),
@@ -113,6 +117,8 @@ SIGNATURES = {
     "vh_supports": (ctypes.c_int, [P, F32, I32, I32]),
     "vh_scene_load": (ctypes.c_int, [P, P, P, P, I32]),
     "vh_scene_info": (ctypes.c_int, [P, P, P, P]),
+    "vh_scene_load_with": (ctypes.c_int, [P, P, P, P, I32, I32]),
+    "vh_scene_read": (ctypes.c_int, [P, P, ctypes.c_int64, P, ctypes.c_int64]),
     "vh_trace_visibility": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, I32, P, I64]),
     "vh_trace_closest": (ctypes.c_int, [P, P, I32, F32, I64, F32, I32, P, P, I64]),
     "vh_trace_point_lights": (ctypes.c_int, [P, P, I64, P, I32, F32, P, I64]),

@@ -31,6 +31,7 @@ SOURCES = [
     "vh_mlp.hip",
     "vh_final.hip",
     "vh_ambient.hip",
+    "vh_build.hip",
 ]
 
 FLAGS = [

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 
 #include "vh_internal.h"
@@ -15,23 +16,109 @@ static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 void clear_error() { g_error.clear(); }
 
+namespace {
+
+// The process-wide cache of device blocks, per device.  A request takes the
+// smallest cached block that holds it without wasting more than half of it
+// (plus 2 MiB); blocks beyond kPoolCap bytes of cache are freed instead.
+// Nothing is freed at exit: the driver reclaims it, and the HIP runtime may
+// already be gone when static destructors run.
+struct Block {
+    void *ptr;
+    size_t bytes;
+};
+std::mutex g_pool_mu;
+std::vector<std::vector<Block>> g_pool;
+size_t g_pool_bytes = 0;
+constexpr size_t kPoolCap = (size_t)48 << 30;
+thread_local int g_release_idle = 0;
+
+size_t alloc_size(size_t need)
+{
+    if (need < 256) return 256;
+    const size_t g = need <= ((size_t)1 << 20) ? 4096 : ((size_t)2 << 20);
+    return (need + g - 1) / g * g;
+}
+
+void *pool_take(int dev, size_t need, size_t &got)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (dev < 0 || (size_t)dev >= g_pool.size()) return nullptr;
+    std::vector<Block> &v = g_pool[dev];
+    size_t best = v.size();
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i].bytes >= need && v[i].bytes <= need + need / 2 + ((size_t)2 << 20) &&
+            (best == v.size() || v[i].bytes < v[best].bytes))
+            best = i;
+    if (best == v.size()) return nullptr;
+    Block b = v[best];
+    v[best] = v.back();
+    v.pop_back();
+    g_pool_bytes -= b.bytes;
+    got = b.bytes;
+    return b.ptr;
+}
+
+void pool_give(int dev, void *ptr, size_t bytes)
+{
+    if (!g_release_idle) (void)hipDeviceSynchronize();
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (dev >= 0 && g_pool_bytes + bytes <= kPoolCap) {
+            if ((size_t)dev >= g_pool.size()) g_pool.resize((size_t)dev + 1);
+            g_pool[dev].push_back(Block{ptr, bytes});
+            g_pool_bytes += bytes;
+            return;
+        }
+    }
+    (void)hipFree(ptr);
+}
+
+// Frees every cached block of `dev` (before retrying a failed allocation).
+void pool_flush(int dev)
+{
+    std::vector<Block> v;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (dev < 0 || (size_t)dev >= g_pool.size()) return;
+        v.swap(g_pool[dev]);
+        for (const Block &b : v) g_pool_bytes -= b.bytes;
+    }
+    for (const Block &b : v) (void)hipFree(b.ptr);
+}
+
+}  // namespace
+
+ReleaseIdle::ReleaseIdle() { g_release_idle++; }
+ReleaseIdle::~ReleaseIdle() { g_release_idle--; }
+
 int DevBuf::ensure(size_t need, const char *what)
 {
     if (need <= bytes && ptr) return VH_OK;
-    if (ptr) {
-        hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
+    if (ptr) release();
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) d = 0;
+    const size_t want = alloc_size(need);
+    size_t got = 0;
+    void *p = pool_take(d, want, got);
+    if (!p) {
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            pool_flush(d);
+            e = hipMalloc(&p, want);
+        }
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            set_error(std::string("device allocation of ") + std::to_string(want) + " bytes for " +
+                      what + " failed: " + hipGetErrorString(e));
+            return VH_EDEVICE;
+        }
+        got = want;
     }
-    size_t want = need < 256 ? 256 : need;
-    hipError_t e = hipMalloc(&ptr, want);
-    if (e != hipSuccess) {
-        ptr = nullptr;
-        set_error(std::string("device allocation of ") + std::to_string(want) + " bytes for " +
-                  what + " failed: " + hipGetErrorString(e));
-        return VH_EDEVICE;
-    }
-    bytes = want;
+    ptr = p;
+    bytes = got;
+    dev = d;
     return VH_OK;
 }
 
@@ -85,9 +172,10 @@ int wait_stream(vh_ctx *c, hipStream_t s)
 
 void DevBuf::release()
 {
-    if (ptr) hipFree(ptr);
+    if (ptr) pool_give(dev, ptr, bytes);
     ptr = nullptr;
     bytes = 0;
+    dev = -1;
 }
 
 KernelTimer::KernelTimer(vh_ctx *ctx, int k, hipStream_t st) : c(ctx), kernel(k), s(st)
@@ -256,6 +344,7 @@ void destroy(vh_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    ReleaseIdle idle;  // the stream is drained: the buffers go back to the cache as they are
     bounce_release(c);
     gather_release(c);
     mlp_release(c);
@@ -382,17 +471,65 @@ int vh_supports(const vh_ctx *c, float min_distance, int32_t skip_id, int32_t sk
 
 int vh_scene_load(vh_ctx *c, const float *v9, const int32_t *ids, const uint8_t *flags, int32_t n)
 {
+    return vh_scene_load_with(c, v9, ids, flags, n, VH_BUILD_AUTO);
+}
+
+int vh_scene_load_with(vh_ctx *c, const float *v9, const int32_t *ids, const uint8_t *flags, int32_t n,
+                       int32_t builder)
+{
     clear_error();
     if (!c || !v9) {
         set_error("NULL context or vertices");
         return VH_EINVAL;
     }
+    if (builder < VH_BUILD_AUTO || builder > VH_BUILD_DEVICE) {
+        set_error("bad builder");
+        return VH_EINVAL;
+    }
+    const bool device = builder == VH_BUILD_DEVICE || (builder == VH_BUILD_AUTO && n >= VH_BUILD_DEVICE_MIN);
     Guard g(c);
     VH_HIP(hipSetDevice(c->device));
-    HostScene hs;
-    int rc = build_scene(v9, ids, flags, n, hs);
-    if (rc) return rc;
-    return upload_scene(c, hs);
+    if (!device) {
+        HostScene hs;
+        int rc = build_scene(v9, ids, flags, n, hs);
+        if (rc) return rc;
+        return upload_scene(c, hs);
+    }
+    if (n < 16) {
+        set_error("the device builder takes at least 16 triangles");
+        return VH_EINVAL;
+    }
+    if (n >= kMaxOrigIndex) {
+        set_error("too many triangles for one scene (limit 2^27)");
+        return VH_EINVAL;
+    }
+    (void)flags;  // transparent triangles are opaque here, as in KdRayTracer.TestLines
+    c->have_scene = false;
+    return build_scene_device(c, v9, ids, n);
+}
+
+int vh_scene_read(vh_ctx *c, float *nodes, int64_t node_floats, float *tris, int64_t tri_floats)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    Guard g(c);
+    if (!c->have_scene) {
+        set_error("no scene loaded");
+        return VH_ESTATE;
+    }
+    if ((nodes && node_floats < (int64_t)c->nnodes * 16) || (tris && tri_floats < (int64_t)c->ntri * 12)) {
+        set_error("the output arrays are too small for the scene");
+        return VH_EINVAL;
+    }
+    VH_HIP(hipSetDevice(c->device));
+    VH_WAIT(c, c->stream);
+    if (nodes && c->nnodes > 0)
+        VH_HIP(hipMemcpy(nodes, c->nodes.ptr, (size_t)c->nnodes * 64, hipMemcpyDeviceToHost));
+    if (tris && c->ntri > 0) VH_HIP(hipMemcpy(tris, c->tris.ptr, (size_t)c->ntri * 48, hipMemcpyDeviceToHost));
+    return VH_OK;
 }
 
 int vh_scene_info(const vh_ctx *c, int32_t *triangles, int32_t *nodes, int32_t *depth)

@@ -19,11 +19,25 @@ void clear_error();
 
 // A device allocation that only grows; reused across calls so a batch does
 // not pay hipMalloc each time.
+// Device memory of one context.  Blocks come from, and go back to, a
+// process-wide cache per device (vh_context.cpp), so a context opened for
+// each lighting pass does not pay hipMalloc and hipFree for its buffers
+// again.  release() keeps hipFree's guarantee that the device is done with
+// the block: it synchronizes the device first, unless the caller has
+// already waited for the context's stream (ReleaseIdle).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    int dev = -1;
     int ensure(size_t need, const char *what);
     void release();
+};
+
+// While one of these is alive on a thread, release() skips the device
+// synchronization: the context's stream has been waited for.
+struct ReleaseIdle {
+    ReleaseIdle();
+    ~ReleaseIdle();
 };
 
 // Scene layout in HBM.
@@ -63,6 +77,9 @@ struct HostScene {
 // Builds the intersection-format triangles and the BVH (vh_bvh.cpp).
 int build_scene(const float *v9, const int32_t *ids, const uint8_t *flags, int32_t n,
                 HostScene &out);
+
+// The same build on the GPU, into the context's scene buffers (vh_build.hip).
+int build_scene_device(vh_ctx *c, const float *v9, const int32_t *ids, int32_t n);
 
 }  // namespace vh
 

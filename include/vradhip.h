@@ -91,6 +91,22 @@ int vh_scene_load(vh_ctx *ctx, const float *v9, const int32_t *ids, const uint8_
 /* Scene statistics: triangles, BVH nodes, BVH depth. */
 int vh_scene_info(const vh_ctx *ctx, int32_t *triangles, int32_t *nodes, int32_t *depth);
 
+/* The builder vh_scene_load uses: on the device (the host builder's binned
+ * SAH, level by level on the GPU, the same tree) from VH_BUILD_DEVICE_MIN
+ * triangles up, on host threads below; vh_scene_load_with picks one
+ * explicitly (the device builder needs at least 16 triangles).  The answers
+ * of every trace do not depend on the builder. */
+#define VH_BUILD_AUTO 0
+#define VH_BUILD_HOST 1
+#define VH_BUILD_DEVICE 2
+#define VH_BUILD_DEVICE_MIN 65536
+int vh_scene_load_with(vh_ctx *ctx, const float *v9, const int32_t *ids, const uint8_t *flags, int32_t n,
+                       int32_t builder);
+/* The loaded scene as the kernels see it: nodes (16 floats each, node_floats
+ * >= 16 * nodes) and triangles (12 floats each, tri_floats >= 12 * triangles,
+ * in leaf order); either may be NULL. */
+int vh_scene_read(vh_ctx *ctx, float *nodes, int64_t node_floats, float *tris, int64_t tri_floats);
+
 /* IRayTracer.TraceVisibilityAsync (SourceSharp.MapTools/Tracing/IRayTracer.cs:162-166).
  * rays: n records of record_words floats (7: origin, direction, reach; 6:
  * origin, direction with shared_reach for all; SourceSharp.MapTools.Gpu/RayRecord.cs:78-144).
