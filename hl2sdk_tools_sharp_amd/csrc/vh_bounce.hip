@@ -1135,11 +1135,16 @@ int vh_patches_load(vh_ctx *c, const vh_patch_desc *d)
             B->lo[k] = v < B->lo[k] ? v : B->lo[k];
             B->hi[k] = v > B->hi[k] ? v : B->hi[k];
         }
-    B->level_ptr.assign(1, 0);
-    for (int32_t lv = 1; lv <= H; lv++) {
+    // the parents by height, in patch order within a height (a counting sort)
+    B->level_ptr.assign((size_t)H + 1, 0);
+    for (int32_t p = 0; p < P; p++)
+        if (h[p] > 0) B->level_ptr[h[p]]++;
+    for (int32_t lv = 1; lv <= H; lv++) B->level_ptr[lv] += B->level_ptr[lv - 1];
+    levels.resize((size_t)B->level_ptr[H]);
+    {
+        std::vector<int32_t> fill(B->level_ptr.begin(), B->level_ptr.end() - 1);
         for (int32_t p = 0; p < P; p++)
-            if (h[p] == lv) levels.push_back(p);
-        B->level_ptr.push_back((int32_t)levels.size());
+            if (h[p] > 0) levels[fill[h[p] - 1]++] = p;
     }
     B->nleaves = (int32_t)leaves.size();
     hipStream_t s = c->stream;
