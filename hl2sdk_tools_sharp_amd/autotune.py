@@ -23,7 +23,7 @@ import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
-CANDIDATE_WAVES_PER_CU = (2, 4, 6, 8, 10, 12, 16, 20)
+CANDIDATE_WAVES_PER_CU = (2, 4, 6, 8, 10, 12, 16, 20, 24, 32)
 CANDIDATE_CHUNKS_PER_TICKET = (1, 2, 4, 8)
 
 
