@@ -31,7 +31,7 @@ constexpr int kBlock = 64;
 constexpr int kMaxScratchDevices = 64;
 void *g_scratch_ptr[kMaxScratchDevices] = {};
 std::mutex g_scratch_mu;
-constexpr int kOccluders = 3;  // occluders a point-light lane remembers
+constexpr int kOccluders = 2;  // occluders a point-light lane remembers
 
 __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays, int rw, float shared_reach,
                                         int64_t i)

@@ -13,6 +13,11 @@ import ctypes
 import numpy as np
 import pytest
 
+# torch (the stream and device buffers of the caller) is imported when the
+# module is collected: its own HIP runtime must be loaded before a test loads
+# libvradhip.so and with it the system's, or torch finds no device.
+import torch
+
 import oracle
 from hl2sdk_tools_sharp_amd import HipTracerFactory, _lib, scenes
 
@@ -28,8 +33,6 @@ def _call(fn, h, pts, lts, bits, stream):
 
 
 def test_noalloc_equals_the_allocating_entry_point_twice():
-    import torch
-
     dev = torch.device("cuda:0")
     w = scenes.visibility_workload(nboxes=600, npoints=300_000, nlights=64, seed=9, volume=6000.0)
     offer = HipTracerFactory(device=0).try_create(w.casters)
