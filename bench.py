@@ -36,6 +36,11 @@ METRIC = "shadow rays/s + bounce wall s on 1 MI355X vs plain FP32 GPU at <=1e-3 
 # 10 add/sub, 1 div, 5 compares) -- see the kernel in csrc/vh_trace.hip.
 FLOP_PER_NODE = 44
 FLOP_PER_TRI = 27
+# The light-buffer trace: per ray the cell lookup (3 sub, 3 abs, 2 compares for
+# the major axis, 1 div, 2 mul, 2 add, 2 mul to cell units = 15) and per list
+# entry read the depth compare (1), besides the triangle tests.
+FLOP_PER_LOOKUP = 15
+FLOP_PER_ENTRY = 1
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (packed FMA) peak
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
@@ -758,6 +763,10 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-faces", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-cpu-fraction", type=float, default=0.1)
+    ap.add_argument("--point-light-method", choices=["auto", "bvh", "lb"], default="auto",
+                    help="light buffers (lb) or BVH walks (bvh); auto picks by batch size")
+    ap.add_argument("--lb-cells", type=int, default=0, help="light-buffer cells per cube-face edge (0: 64)")
+    ap.add_argument("--no-walk", action="store_true", help="skip the BVH-walk comparison leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (profiler calibration)")
     a = ap.parse_args(argv)
@@ -820,43 +829,70 @@ def main(argv=None) -> int:
         if st != 0:
             raise RuntimeError(_lib.last_error())
 
+    def counted():
+        """Work counts of one launch (outside any timed region)."""
+        _lib.check(L.vh_count_enable(h, 1))
+        step()
+        torch.cuda.synchronize()
+        nv, nt = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.vh_count_read(h, ctypes.byref(nv), ctypes.byref(nt)))
+        ns, ts = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.vh_count_read_slots(h, ctypes.byref(ns), ctypes.byref(ts)))
+        _lib.check(L.vh_count_enable(h, 0))
+        return nv.value, nt.value, ns.value, ts.value
+
+    def timed(steps):
+        """(ms per step, trace kernel ms per launch, build ms per step) over `steps` steps."""
+        _lib.check(L.vh_timing_enable(h, 1))
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        total_ms = e0.elapsed_time(e1)
+        kms, kl = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_POINT_LIGHTS, ctypes.byref(kms), ctypes.byref(kl)))
+        bms, bl = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_LIGHT_BUFFER, ctypes.byref(bms), ctypes.byref(bl)))
+        _lib.check(L.vh_timing_enable(h, 0))
+        return total_ms / steps, kms.value / max(kl.value, 1), bms.value / steps
+
+    methods = {"auto": _lib.VH_POINT_LIGHTS_AUTO, "bvh": _lib.VH_POINT_LIGHTS_BVH,
+               "lb": _lib.VH_POINT_LIGHTS_LIGHT_BUFFER}
+    tr.set_point_light_method(methods[a.point_light_method], a.lb_cells)
     for i in range(a.warmup):
         step()
         log(f"[bench] warmup {i + 1}/{a.warmup}")
     torch.cuda.synchronize()
-
-    # work counts for the roofline (one counted launch, outside the timed region)
-    _lib.check(L.vh_count_enable(h, 1))
+    _lib.check(L.vh_timing_enable(h, 1))
     step()
     torch.cuda.synchronize()
-    nv, nt = ctypes.c_int64(), ctypes.c_int64()
-    _lib.check(L.vh_count_read(h, ctypes.byref(nv), ctypes.byref(nt)))
-    ns, ts = ctypes.c_int64(), ctypes.c_int64()
-    _lib.check(L.vh_count_read_slots(h, ctypes.byref(ns), ctypes.byref(ts)))
-    _lib.check(L.vh_count_enable(h, 0))
-    node_eff = nv.value / max(ns.value, 1)
-    log(f"[bench] per ray: {nv.value / nrays:.2f} node visits, {nt.value / nrays:.2f} triangle tests; "
-        f"packet efficiency {node_eff:.3f} (node visits per lane slot)")
-
-    _lib.check(L.vh_timing_enable(h, 1))
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(a.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = e0.elapsed_time(e1)
-    kms, kl = ctypes.c_double(), ctypes.c_int64()
-    _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_POINT_LIGHTS, ctypes.byref(kms), ctypes.byref(kl)))
+    bms, bl = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(L.vh_timing_read(h, _lib.VH_KERNEL_LIGHT_BUFFER, ctypes.byref(bms), ctypes.byref(bl)))
     _lib.check(L.vh_timing_enable(h, 0))
-    ms_per_step = total_ms / a.steps
-    kernel_ms = kms.value / max(kl.value, 1)
-    rays_per_s = nrays * a.steps / (total_ms / 1e3)
-    log(f"[bench] {ms_per_step:.3f} ms/step, kernel {kernel_ms:.3f} ms, {rays_per_s / 1e9:.3f} Grays/s")
+    light_buffer = bl.value > 0  # which form the method picked for this batch
+    kernel = "k_point_lights_lb" if light_buffer else "k_point_lights"
+    nv, nt, ns, ts = counted()
+    node_eff = nv / max(ns, 1)
+    if light_buffer:
+        log(f"[bench] light buffers ({a.lb_cells or 64} cells a face edge); per ray: {nv / nrays:.2f} list "
+            f"entries read, {nt / nrays:.2f} triangle tests; lane use {node_eff:.3f} (entries per wave step slot)")
+    else:
+        log(f"[bench] per ray: {nv / nrays:.2f} node visits, {nt / nrays:.2f} triangle tests; "
+            f"packet efficiency {node_eff:.3f} (node visits per lane slot)")
 
-    flops = nv.value * FLOP_PER_NODE + nt.value * FLOP_PER_TRI
+    ms_per_step, kernel_ms, build_ms = timed(a.steps)
+    rays_per_s = nrays * a.steps / (ms_per_step * a.steps / 1e3)
+    log(f"[bench] {ms_per_step:.3f} ms/step, {kernel} {kernel_ms:.3f} ms, light-buffer build {build_ms:.3f} ms, "
+        f"{rays_per_s / 1e9:.3f} Grays/s")
+
+    if light_buffer:
+        flops = nrays * FLOP_PER_LOOKUP + nv * FLOP_PER_ENTRY + nt * FLOP_PER_TRI
+    else:
+        flops = nv * FLOP_PER_NODE + nt * FLOP_PER_TRI
     achieved = flops / (kernel_ms / 1e3) / 1e12
     roofline = {
         "bound": "valu",
@@ -864,16 +900,49 @@ def main(argv=None) -> int:
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": read_traffic("k_point_lights"),
-        "kernel": "k_point_lights",
+        "traffic": read_traffic(kernel),
+        "kernel": kernel,
         "kernel_ms": round(kernel_ms, 4),
         "flop_per_launch": flops,
-        "node_visits_per_ray": round(nv.value / nrays, 3),
-        "triangle_tests_per_ray": round(nt.value / nrays, 3),
-        "node_slots_per_ray": round(ns.value / nrays, 3),
-        "triangle_slots_per_ray": round(ts.value / nrays, 3),
-        "packet_efficiency": round(node_eff, 4),
     }
+    if light_buffer:
+        roofline.update({
+            "lookup_flop_per_ray": FLOP_PER_LOOKUP,
+            "entries_read_per_ray": round(nv / nrays, 3),
+            "triangle_tests_per_ray": round(nt / nrays, 3),
+            "wave_step_slots_per_ray": round(ns / nrays, 3),
+            "lane_use": round(node_eff, 4),
+            "build_ms_per_step": round(build_ms, 4),
+            "cells_per_face_edge": a.lb_cells or 64,
+        })
+    else:
+        roofline.update({
+            "node_visits_per_ray": round(nv / nrays, 3),
+            "triangle_tests_per_ray": round(nt / nrays, 3),
+            "node_slots_per_ray": round(ns / nrays, 3),
+            "triangle_slots_per_ray": round(ts / nrays, 3),
+            "packet_efficiency": round(node_eff, 4),
+        })
+    main_bits = bits.clone()
+
+    walk = None
+    if light_buffer and not a.no_walk:
+        # the BVH walk on the same batch: its time and its bits (both forms are
+        # exact, so every word must agree)
+        tr.set_point_light_method(_lib.VH_POINT_LIGHTS_BVH)
+        step()
+        torch.cuda.synchronize()
+        wv, wt, wsl, _ = counted()
+        w_ms, w_kms, _ = timed(max(2, a.steps // 2))
+        differ = int((bits != main_bits).sum().item())
+        walk = {"ms_per_step": round(w_ms, 4), "kernel_ms": round(w_kms, 4),
+                "rays_per_s": nrays / (w_ms / 1e3), "node_visits_per_ray": round(wv / nrays, 3),
+                "triangle_tests_per_ray": round(wt / nrays, 3), "packet_efficiency": round(wv / max(wsl, 1), 4),
+                "words_differing_from_light_buffer": differ}
+        log(f"[bench] BVH walk on the same batch: {w_ms:.3f} ms/step (kernel {w_kms:.3f} ms), "
+            f"{nrays / (w_ms / 1e3) / 1e9:.3f} Grays/s; words differing from the light buffer: {differ}")
+        tr.set_point_light_method(methods[a.point_light_method], a.lb_cells)
+        bits.copy_(main_bits)
 
     gpu_bits = bits.cpu().numpy().view(np.uint64)
     cpu = None
@@ -913,8 +982,10 @@ def main(argv=None) -> int:
             "rays_per_step": nrays,
             "bvh_nodes": info["nodes"],
             "bvh_depth": info["depth"],
+            "method": "light buffers" if light_buffer else "BVH walk",
         },
         "roofline": roofline,
+        "bvh_walk": walk,
         "cpu_baseline": cpu,
         "parity": parity,
         "bounce_wall_s": None if bounce is None else bounce["wall_s_f32"],

@@ -35,6 +35,7 @@ STATUS_NAMES = {
 
 VH_BUILD_AUTO, VH_BUILD_HOST, VH_BUILD_DEVICE = 0, 1, 2
 VH_BUILD_DEVICE_MIN = 65536
+VH_POINT_LIGHTS_AUTO, VH_POINT_LIGHTS_BVH, VH_POINT_LIGHTS_LIGHT_BUFFER = 0, 1, 2
 
 
 class VhOpenOpts(ctypes.Structure):
@@ -128,6 +129,7 @@ SIGNATURES = {
     "vh_trace_point_lights_noalloc": (ctypes.c_int, [P, P, I64, P, I32, F32, P, P]),
     "vh_sync": (ctypes.c_int, [P, P]),
     "vh_set_timeout": (ctypes.c_int, [P, I64]),
+    "vh_set_point_light_method": (ctypes.c_int, [P, I32, I32]),
     "vh_cancel": (ctypes.c_int, [P]),
     "vh_cancel_clear": (ctypes.c_int, [P]),
     "vh_get_stats": (ctypes.c_int, [P, ctypes.POINTER(VhStats)]),
@@ -180,6 +182,7 @@ VH_KERNEL_MLP = 6
 VH_KERNEL_FINAL = 7
 VH_KERNEL_AMBIENT = 8
 VH_KERNEL_SUPERSAMPLE = 9
+VH_KERNEL_LIGHT_BUFFER = 10
 
 VH_WEIGHTS_F32 = 0
 VH_WEIGHTS_F16 = 1

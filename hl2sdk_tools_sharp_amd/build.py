@@ -26,6 +26,7 @@ SOURCES = [
     "vh_context.cpp",
     "vh_bvh.cpp",
     "vh_trace.hip",
+    "vh_lightbuf.hip",
     "vh_bounce.hip",
     "vh_gather.hip",
     "vh_mlp.hip",

@@ -999,6 +999,7 @@ int build_scene_device(vh_ctx *c, const float *v9, const int32_t *ids, int32_t n
         c->hi[k] = *(const float *)&hi;
     }
     c->have_scene = true;
+    c->scene_gen++;
     return VH_OK;
 }
 

@@ -253,6 +253,7 @@ int upload_scene(vh_ctx *c, const HostScene &hs)
         c->hi[k] = hs.hi[k];
     }
     c->have_scene = true;
+    c->scene_gen++;
     return VH_OK;
 }
 
@@ -352,8 +353,9 @@ void destroy(vh_ctx *c)
     ambient_release(c);
     for (DevBuf *b : {&c->nodes, &c->tris, &c->rays_buf, &c->out_buf, &c->out2_buf, &c->pts_buf,
                       &c->lights_buf, &c->keys_a, &c->keys_b, &c->perm_a, &c->perm_b,
-                      &c->sort_tmp, &c->counters})
+                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_big, &c->lb_rec})
         b->release();
+    if (c->lb_pinned) hipHostFree(c->lb_pinned);
     for (auto &v : c->pending)
         for (auto &pr : v) {
             hipEventDestroy(pr.first);
@@ -647,6 +649,28 @@ int vh_trace_point_lights(vh_ctx *c, const float *points, int64_t npoints, const
     VH_HIP(hipMemcpy(bits, c->out_buf.ptr, (size_t)words * 8, hipMemcpyDeviceToHost));
     c->stats.rays_visibility += npoints * (int64_t)nlights;
     c->stats.calls++;
+    return VH_OK;
+}
+
+int vh_set_point_light_method(vh_ctx *c, int32_t method, int32_t cells)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    if (method < VH_POINT_LIGHTS_AUTO || method > VH_POINT_LIGHTS_LIGHT_BUFFER) {
+        set_error("unknown point-light method " + std::to_string(method));
+        return VH_EINVAL;
+    }
+    if (cells != 0 && (cells < 4 || cells > 256)) {
+        set_error("light-buffer cells per face edge must be 4..256 (or 0 for the default), not " +
+                  std::to_string(cells));
+        return VH_EINVAL;
+    }
+    Guard g(c);
+    c->pl_method = method;
+    c->lb_cells = cells;
     return VH_OK;
 }
 

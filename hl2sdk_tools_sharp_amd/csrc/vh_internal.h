@@ -98,6 +98,17 @@ struct vh_ctx {
     int32_t nnodes = 0, ntri = 0, depth = 0;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool have_scene = false;
+    uint64_t scene_gen = 0;  // changes with every scene load
+    // point-light method (VH_POINT_LIGHTS_*) and the light buffers' cells per
+    // cube-face edge (0: the default)
+    int32_t pl_method = 0, lb_cells = 0;
+    // light buffers (vh_lightbuf.hip): per-triangle boxes of the scene
+    // generation lb_boxes_gen, cell offsets, fill cursors, the lists as
+    // (depth, triangle) and as triangle records, the pairs seen through many
+    // cells, and a pinned word for the list size
+    vh::DevBuf lb_boxes, lb_off, lb_cur, lb_ent, lb_big, lb_rec;
+    uint64_t lb_boxes_gen = ~0ull;
+    uint32_t *lb_pinned = nullptr;
     // per-call staging
     vh::DevBuf rays_buf, out_buf, out2_buf, pts_buf, lights_buf;
     vh::DevBuf keys_a, keys_b, perm_a, perm_b, sort_tmp;
@@ -144,6 +155,13 @@ int launch_closest(vh_ctx *c, const float *d_rays, int32_t rw, float shared_reac
                    hipStream_t s);
 int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const float *d_lights,
                         int32_t nlights, float tmin, uint64_t *d_bits, hipStream_t s);
+// The light-buffer form of the point-light trace (vh_lightbuf.hip), over
+// points already in spatial order; and its default cells per cube-face edge.
+constexpr int kLightBufferCells = 64;
+int lb_resolution(vh_ctx *c);
+int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *order, int64_t npoints,
+                           const float *d_lights, int32_t nlights, float tmin, uint64_t *d_bits,
+                           hipStream_t s);
 // The no-allocation point-light path (vh_trace.hip): noalloc_init finds this
 // device's copy of the fixed working memory (once per device, with the device
 // current); the launch uses it and allocates nothing.

@@ -380,6 +380,13 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
         size_t words = (size_t)((npoints * (int64_t)nlights + 63) / 64);
         VH_HIP(hipMemsetAsync(d_bits, 0, words * 8, s));
     }
+    // light buffers pay a per-light build over every triangle, which a large
+    // batch of points amortises; small batches walk the BVH
+    const bool lb = c->pl_method == VH_POINT_LIGHTS_LIGHT_BUFFER ||
+                    (c->pl_method == VH_POINT_LIGHTS_AUTO && npoints >= 65536 && npoints >= c->ntri);
+    if (lb)
+        return launch_point_lights_lb(c, d_points, (const int32_t *)c->perm_b.ptr, npoints, d_lights,
+                                      nlights, tmin, d_bits, s);
     int64_t blocks = (npoints + kBlock - 1) / kBlock;
     // each wave's stack holds one (node, lane mask) entry per internal level
     int32_t depth = c->depth + 1 < kMaxDepth ? c->depth + 1 : kMaxDepth;

@@ -260,6 +260,14 @@ class HipRayTracer:
         self._raise(st)
         return bits
 
+    def set_point_light_method(self, method: int, cells: int = 0) -> None:
+        """How :meth:`trace_point_lights` finds candidate triangles:
+        ``_lib.VH_POINT_LIGHTS_AUTO`` (default), ``..._BVH`` (wave packet walks)
+        or ``..._LIGHT_BUFFER`` (per-light cubes of ``cells`` x ``cells``
+        direction cells a face, 0 = 64).  The bits are the same either way."""
+        self._alive()
+        self._raise(_lib.lib().vh_set_point_light_method(self._h, int(method), int(cells)))
+
     # -- lifetime ---------------------------------------------------------
     def set_timeout(self, ms: int) -> None:
         """The bound on each wait on the device (VulkanRayTracer's 120 s fence

@@ -138,6 +138,24 @@ int vh_trace_point_lights(vh_ctx *ctx, const float *points, int64_t npoints,
                           const float *lights, int32_t nlights, float min_distance,
                           uint64_t *bits, int64_t bit_words);
 
+/* How vh_trace_point_lights and vh_trace_point_lights_dev find the
+ * triangles a ray may hit (no counterpart in the reference, whose
+ * LightRayLog batches go through the generic tracer; the answers are the same
+ * bits either way, the exact triangle test's):
+ *  - VH_POINT_LIGHTS_BVH: the rays of a wave walk the BVH together;
+ *  - VH_POINT_LIGHTS_LIGHT_BUFFER: per light a cube of `cells` x `cells`
+ *    direction cells on each face, each listing the triangles seen through it
+ *    by depth, built on the device for each call; a ray tests its cell's list
+ *    up to its own depth;
+ *  - VH_POINT_LIGHTS_AUTO (the default): light buffers for batches of at least
+ *    65,536 points and at least as many points as triangles, else the BVH.
+ * `cells` is 4..256 (0: 64).  vh_trace_point_lights_noalloc always walks the
+ * BVH. */
+#define VH_POINT_LIGHTS_AUTO 0
+#define VH_POINT_LIGHTS_BVH 1
+#define VH_POINT_LIGHTS_LIGHT_BUFFER 2
+int vh_set_point_light_method(vh_ctx *ctx, int32_t method, int32_t cells);
+
 /* Device-pointer forms of the three traces above, enqueued on `stream`. */
 int vh_trace_visibility_dev(vh_ctx *ctx, const float *d_rays, int32_t record_words,
                             float shared_reach, int64_t n, float min_distance, int32_t skip_id,
@@ -663,7 +681,9 @@ int vh_mlp_predict(vh_ctx *ctx, const float *emit0, float *start);
 #define VH_KERNEL_FINAL 7 /* vh_final_light's launches, bracketed together */
 #define VH_KERNEL_AMBIENT 8 /* the leaf-ambient fan */
 #define VH_KERNEL_SUPERSAMPLE 9 /* vh_gather_supersample's launches */
-#define VH_KERNEL_COUNT 10
+#define VH_KERNEL_LIGHT_BUFFER 10 /* a light-buffer build: cell counts, scan, lists, sort
+                                     (with the host's read of the list size between them) */
+#define VH_KERNEL_COUNT 11
 int vh_timing_enable(vh_ctx *ctx, int32_t on);
 int vh_timing_read(vh_ctx *ctx, int32_t kernel, double *total_ms, int64_t *launches);
 
@@ -675,7 +695,9 @@ int vh_count_read(vh_ctx *ctx, int64_t *node_visits, int64_t *triangle_tests);
 /* The lane slots the point-light packet walks occupied while counting: 64 per
  * node visit or triangle fetch of a wave, whatever lanes took part, so
  * node_visits / node_slots is the walks' packet efficiency (no counterpart in
- * the reference). */
+ * the reference).  A light-buffer trace counts instead: node_visits = list
+ * entries read, triangle_tests = exact tests, node_slots = 64 per list step of
+ * a wave (as many steps as its longest lane), triangle_slots = 0. */
 int vh_count_read_slots(vh_ctx *ctx, int64_t *node_slots, int64_t *triangle_slots);
 
 #ifdef __cplusplus
