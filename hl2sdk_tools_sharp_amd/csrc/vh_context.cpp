@@ -353,7 +353,7 @@ void destroy(vh_ctx *c)
     ambient_release(c);
     for (DevBuf *b : {&c->nodes, &c->tris, &c->rays_buf, &c->out_buf, &c->out2_buf, &c->pts_buf,
                       &c->lights_buf, &c->keys_a, &c->keys_b, &c->perm_a, &c->perm_b,
-                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_big, &c->lb_rec})
+                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_rec})
         b->release();
     if (c->lb_pinned) hipHostFree(c->lb_pinned);
     for (auto &v : c->pending)
@@ -663,9 +663,9 @@ int vh_set_point_light_method(vh_ctx *c, int32_t method, int32_t cells)
         set_error("unknown point-light method " + std::to_string(method));
         return VH_EINVAL;
     }
-    if (cells != 0 && (cells < 4 || cells > 256)) {
-        set_error("light-buffer cells per face edge must be 4..256 (or 0 for the default), not " +
-                  std::to_string(cells));
+    if (cells != 0 && (cells < 4 || cells > kMaxLightBufferCells)) {
+        set_error("light-buffer cells per face edge must be 4.." + std::to_string(kMaxLightBufferCells) +
+                  " (or 0 for the default), not " + std::to_string(cells));
         return VH_EINVAL;
     }
     Guard g(c);

@@ -103,10 +103,10 @@ struct vh_ctx {
     // cube-face edge (0: the default)
     int32_t pl_method = 0, lb_cells = 0;
     // light buffers (vh_lightbuf.hip): per-triangle boxes of the scene
-    // generation lb_boxes_gen, cell offsets, fill cursors, the lists as
-    // (depth, triangle) and as triangle records, the pairs seen through many
-    // cells, and a pinned word for the list size
-    vh::DevBuf lb_boxes, lb_off, lb_cur, lb_ent, lb_big, lb_rec;
+    // generation lb_boxes_gen, cell offsets and counts, the lists as
+    // (depth, triangle) and as triangle records, and a pinned word for the
+    // list size
+    vh::DevBuf lb_boxes, lb_off, lb_cur, lb_ent, lb_rec;
     uint64_t lb_boxes_gen = ~0ull;
     uint32_t *lb_pinned = nullptr;
     // per-call staging
@@ -158,6 +158,7 @@ int launch_point_lights(vh_ctx *c, const float *d_points, int64_t npoints, const
 // The light-buffer form of the point-light trace (vh_lightbuf.hip), over
 // points already in spatial order; and its default cells per cube-face edge.
 constexpr int kLightBufferCells = 64;
+constexpr int kMaxLightBufferCells = 120;  // a face's counters fit one workgroup's 64 KiB of LDS
 int lb_resolution(vh_ctx *c);
 int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *order, int64_t npoints,
                            const float *d_lights, int32_t nlights, float tmin, uint64_t *d_bits,

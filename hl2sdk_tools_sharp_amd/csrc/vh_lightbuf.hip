@@ -97,92 +97,90 @@ __device__ __forceinline__ bool pair_box(const float4 *__restrict__ boxes, const
     return true;
 }
 
-// One cell of a pair: FILL false counts it, FILL true writes (depth, triangle)
-// at the cell's cursor.  The fill repeats the count's cells exactly; the bound
-// keeps every write inside the cell's own range regardless.
 struct Lists {
     int2 *key;       // (depth bits, triangle) per entry, in fill order
     uint32_t *cell;  // the entry's cell
 };
 
-template <bool FILL>
-__device__ __forceinline__ void cell_put(size_t cell, float dlo, int32_t k, uint32_t *__restrict__ count,
-                                         const uint32_t *__restrict__ off, Lists ls)
-{
-    if (!FILL) {
-        atomicAdd(&count[cell], 1u);
-    } else {
-        const uint32_t pos = atomicAdd(&count[cell], 1u);
-        if (pos < off[cell + 1]) {
-            ls.key[pos] = make_int2(__float_as_int(dlo), k);
-            ls.cell[pos] = (uint32_t)cell;
-        }
-    }
-}
+constexpr int kImageThreads = 256;
+constexpr int kImageParts = 4;    // workgroups per (light, face) image, each over a quarter of the triangles
+constexpr int kInlineCells = 32;  // rectangles with more cells go to the workgroup's shared list
+constexpr int kBigCap = 256;
 
-constexpr int kInlineCells = 32;  // pairs with more cells go to k_lb_big
-
-// One (light, triangle) pair per thread.  A pair seen through more than
-// kInlineCells cells (a box near its light) is listed in big[] by the count
-// pass and left to k_lb_big in both passes, so no thread runs a long serial
-// chain of atomics.
+// kImageParts workgroups per (light, cube face) image, each over its share of
+// the triangles with the image's N x N cell counters in LDS: FILL false counts
+// each cell's triangles and stores the counts as count[(image cell) * parts +
+// part]; FILL true starts each cell's cursor at that slot's offset (after an
+// exclusive scan) and writes the entries, so a cell's list is its parts' runs
+// back to back, [off[cell * parts], off[(cell + 1) * parts]).  A triangle seen
+// through more than kInlineCells cells of the face (a box near its light) is
+// queued and its cells shared out among the workgroup's threads afterwards
+// (the few past the queue's capacity are done by their own thread); either
+// way each (triangle, cell) is put once, so the two passes agree.  LDS atomics
+// only: no two workgroups share a slot.
 template <bool FILL>
-__global__ void k_lb_cells(const float4 *__restrict__ boxes, int32_t ntri,
-                           const float *__restrict__ lights, int32_t nlg, int32_t N,
-                           const uint32_t *__restrict__ pmax_bits, float scene_mag,
-                           uint32_t *__restrict__ count, const uint32_t *__restrict__ off,
-                           Lists ent, uint32_t *__restrict__ nbig, int32_t *__restrict__ big)
+__global__ __launch_bounds__(kImageThreads) void k_lb_image(
+    const float4 *__restrict__ boxes, int32_t ntri, const float *__restrict__ lights, int32_t N,
+    const uint32_t *__restrict__ pmax_bits, float scene_mag, uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ off, Lists ls)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)ntri * nlg) return;
-    const int32_t l = (int32_t)(i / ntri), k = (int32_t)(i - (int64_t)l * ntri);
-    float lo[3], hi[3];
-    if (!pair_box(boxes, lights, l, k, __uint_as_float(*pmax_bits), scene_mag, lo, hi)) return;
-    FaceRect r[6];
-    bool ok[6];
-    int total = 0;
-    for (int f = 0; f < 6; f++) {
-        ok[f] = lb_face_rect(lo, hi, f >> 1, f & 1, N, r[f]);
-        if (ok[f]) total += (r[f].iu1 - r[f].iu0 + 1) * (r[f].iw1 - r[f].iw0 + 1);
-    }
-    if (total > kInlineCells) {
-        if (!FILL) big[atomicAdd(nbig, 1u)] = (int32_t)i;
-        return;
-    }
-    const size_t base = (size_t)l * 6 * N * N;
-    for (int f = 0; f < 6; f++) {
-        if (!ok[f]) continue;
-        for (int iu = r[f].iu0; iu <= r[f].iu1; iu++)
-            for (int iw = r[f].iw0; iw <= r[f].iw1; iw++)
-                cell_put<FILL>(base + ((size_t)f * N + iu) * N + iw, r[f].dlo, k, count, off, ent);
-    }
-}
-
-// The pairs in big[]: a block per pair, its threads over the pair's cells.
-template <bool FILL>
-__global__ void k_lb_big(const float4 *__restrict__ boxes, int32_t ntri, const float *__restrict__ lights,
-                         int32_t N, const uint32_t *__restrict__ pmax_bits, float scene_mag,
-                         uint32_t *__restrict__ count, const uint32_t *__restrict__ off,
-                         Lists ent, const uint32_t *__restrict__ nbig,
-                         const int32_t *__restrict__ big)
-{
-    const uint32_t n = *nbig;
-    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
-        const int64_t i = big[b];
-        const int32_t l = (int32_t)(i / ntri), k = (int32_t)(i - (int64_t)l * ntri);
-        float lo[3], hi[3];
-        if (!pair_box(boxes, lights, l, k, __uint_as_float(*pmax_bits), scene_mag, lo, hi)) continue;
-        const size_t base = (size_t)l * 6 * N * N;
-        for (int f = 0; f < 6; f++) {
-            FaceRect r;
-            if (!lb_face_rect(lo, hi, f >> 1, f & 1, N, r)) continue;
-            const int wu = r.iw1 - r.iw0 + 1;
-            const int cells = (r.iu1 - r.iu0 + 1) * wu;
-            for (int t = threadIdx.x; t < cells; t += blockDim.x) {
-                const int iu = r.iu0 + t / wu, iw = r.iw0 + t % wu;
-                cell_put<FILL>(base + ((size_t)f * N + iu) * N + iw, r.dlo, k, count, off, ent);
+    extern __shared__ uint32_t sh[];
+    __shared__ uint32_t nbig;
+    const int NN = N * N;
+    uint32_t *cnt = sh;
+    int32_t *bigk = (int32_t *)(sh + NN);
+    const int img = blockIdx.x / kImageParts, part = blockIdx.x - img * kImageParts;
+    const int l = img / 6, f = img - 6 * (img / 6);
+    const int a = f >> 1, side = f & 1;
+    const size_t base = (size_t)img * NN;
+    for (int i = threadIdx.x; i < NN; i += blockDim.x) cnt[i] = FILL ? off[(base + i) * kImageParts + part] : 0u;
+    // every write stays below the image's end, whatever happens
+    const uint32_t img_end = FILL ? off[(base + NN) * kImageParts] : 0u;
+    if (threadIdx.x == 0) nbig = 0;
+    __syncthreads();
+    const float pmax = __uint_as_float(*pmax_bits);
+    auto put = [&](int c, float dlo, int32_t k) {
+        if (!FILL) {
+            atomicAdd(&cnt[c], 1u);
+        } else {
+            const uint32_t pos = atomicAdd(&cnt[c], 1u);
+            if (pos < img_end) {
+                ls.key[pos] = make_int2(__float_as_int(dlo), k);
+                ls.cell[pos] = (uint32_t)(base + c);
             }
         }
+    };
+    const int32_t k0 = (int32_t)(((int64_t)ntri * part) / kImageParts);
+    const int32_t k1 = (int32_t)(((int64_t)ntri * (part + 1)) / kImageParts);
+    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += blockDim.x) {
+        float lo[3], hi[3];
+        FaceRect r;
+        if (!pair_box(boxes, lights, l, k, pmax, scene_mag, lo, hi) || !lb_face_rect(lo, hi, a, side, N, r))
+            continue;
+        if ((r.iu1 - r.iu0 + 1) * (r.iw1 - r.iw0 + 1) > kInlineCells) {
+            const uint32_t q = atomicAdd(&nbig, 1u);
+            if (q < (uint32_t)kBigCap) {
+                bigk[q] = k;
+                continue;
+            }
+        }
+        for (int iu = r.iu0; iu <= r.iu1; iu++)
+            for (int iw = r.iw0; iw <= r.iw1; iw++) put(iu * N + iw, r.dlo, k);
+    }
+    __syncthreads();
+    const uint32_t nb = nbig < (uint32_t)kBigCap ? nbig : (uint32_t)kBigCap;
+    for (uint32_t q = 0; q < nb; q++) {
+        const int32_t k = bigk[q];
+        float lo[3], hi[3];
+        FaceRect r;
+        pair_box(boxes, lights, l, k, pmax, scene_mag, lo, hi);
+        lb_face_rect(lo, hi, a, side, N, r);
+        const int wu = r.iw1 - r.iw0 + 1, cells = (r.iu1 - r.iu0 + 1) * wu;
+        for (int t = threadIdx.x; t < cells; t += blockDim.x) put((r.iu0 + t / wu) * N + r.iw0 + t % wu, r.dlo, k);
+    }
+    if (!FILL) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < NN; i += blockDim.x) count[(base + i) * kImageParts + part] = cnt[i];
     }
 }
 
@@ -219,8 +217,9 @@ __global__ void k_lb_records(const int2 *__restrict__ key, const uint32_t *__res
     // every entry was written by the fill when the count and the fill agree,
     // which they do by construction; anything else is left alone
     if (c >= ncell || (uint32_t)e.y >= (uint32_t)ntri) return;
-    const uint32_t b = off[c], end = off[c + 1];
+    const uint32_t b = off[(size_t)c * kImageParts], end = off[(size_t)(c + 1) * kImageParts];
     uint32_t rank = 0;
+#pragma unroll 4
     for (uint32_t i = b; i < end; i++) rank += key_less(key[i], e) ? 1u : 0u;
     const uint32_t at = b + rank;
     if (at >= end) return;
@@ -331,11 +330,11 @@ __global__ __launch_bounds__(kBlock) void k_point_lights_lb(
         float depth;
         if (have && ray_finite(r) && lb_cell_of(r.dx, r.dy, r.dz, N, face, iu, iw, depth)) {
             const size_t cell = (((size_t)l * 6 + face) * N + iu) * N + iw;
-            const uint32_t end = off[cell + 1];
+            const uint32_t end = off[(cell + 1) * kImageParts];
             const uint32_t dbits = __float_as_uint(depth);
             // each step loads the next entry's depth and plane before testing
             // this one, so the loads of consecutive steps overlap
-            uint32_t j = off[cell];
+            uint32_t j = off[cell * kImageParts];
             uint32_t dj = j < end ? rdepth[j] : 0xffffffffu;
             float4 pl = j < end ? rplane[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             while (j < end) {
@@ -407,15 +406,14 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
     }
     const int32_t nlg_max = nlights < 64 ? nlights : 64;
     const size_t ncell = (size_t)6 * N * N * nlg_max;
-    if ((rc = c->lb_off.ensure((ncell + 1) * 4, "light-buffer cells")) ||
-        (rc = c->lb_cur.ensure(ncell * 4 + 256, "light-buffer cursors")) ||
-        (rc = c->lb_big.ensure((size_t)ntri * nlg_max * 4, "light-buffer large pairs")))
+    const size_t nslot = ncell * kImageParts;  // (cell, part) slots
+    if ((rc = c->lb_off.ensure((nslot + 1) * 4, "light-buffer cells")) ||
+        (rc = c->lb_cur.ensure(nslot * 4 + 256, "light-buffer counts")))
         return rc;
     if (!c->lb_pinned) VH_HIP(hipHostMalloc((void **)&c->lb_pinned, 64, hipHostMallocDefault));
-    // two words past the cursors and the scan's extra element (which each
-    // group clears): the points' largest coordinate and the large-pair count
-    uint32_t *d_pmax = (uint32_t *)((char *)c->lb_cur.ptr + ncell * 4 + 128);
-    uint32_t *d_nbig = d_pmax + 1;
+    // a word past the counts and the scan's extra element (which each group
+    // clears): the points' largest coordinate
+    uint32_t *d_pmax = (uint32_t *)((char *)c->lb_cur.ptr + nslot * 4 + 128);
     VH_HIP(hipMemsetAsync(d_pmax, 0, 4, s));
     hipLaunchKernelGGL(k_lb_pmax, dim3(256), dim3(256), 0, s, d_points, npoints, d_pmax);
     VH_HIP(hipGetLastError());
@@ -423,39 +421,32 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
     for (int k = 0; k < 3; k++) scene_mag = fmaxf(scene_mag, fmaxf(fabsf(c->lo[k]), fabsf(c->hi[k])));
     size_t scan_bytes = 0;
     VH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t *)c->lb_cur.ptr,
-                                            (uint32_t *)c->lb_off.ptr, (int)(ncell + 1), s));
+                                            (uint32_t *)c->lb_off.ptr, (int)(nslot + 1), s));
     if ((rc = c->sort_tmp.ensure(scan_bytes, "scan scratch"))) return rc;
     const int64_t blocks = ((npoints + kBlock - 1) / kBlock + 7) / 8 * 8;  // a multiple of 8 (XCD split)
-    const uint32_t big_blocks = (uint32_t)(c->cu_count > 0 ? c->cu_count * 8 : 2048);
-    int32_t *big = (int32_t *)c->lb_big.ptr;
     for (int32_t g0 = 0; g0 < nlights; g0 += 64) {
         const int32_t nlg = nlights - g0 < 64 ? nlights - g0 : 64;
         const size_t nc = (size_t)6 * N * N * nlg;
         const float *gl = d_lights + (size_t)g0 * 3;
-        const int64_t pairs = (int64_t)ntri * nlg;
-        const uint32_t pblocks = (uint32_t)((pairs + 255) / 256);
         const float4 *boxes = (const float4 *)c->lb_boxes.ptr;
         uint32_t *cur = (uint32_t *)c->lb_cur.ptr;
         const uint32_t *off = (const uint32_t *)c->lb_off.ptr;
         KernelTimer build(c, VH_KERNEL_LIGHT_BUFFER, s);
-        VH_HIP(hipMemsetAsync(cur, 0, (nc + 1) * 4, s));
-        VH_HIP(hipMemsetAsync(d_nbig, 0, 4, s));
-        hipLaunchKernelGGL(k_lb_cells<false>, dim3(pblocks), dim3(256), 0, s, boxes, ntri, gl, nlg, N, d_pmax,
-                           scene_mag, cur, nullptr, Lists{nullptr, nullptr}, d_nbig, big);
-        hipLaunchKernelGGL(k_lb_big<false>, dim3(big_blocks), dim3(256), 0, s, boxes, ntri, gl, N, d_pmax,
-                           scene_mag, cur, nullptr, Lists{nullptr, nullptr}, (const uint32_t *)d_nbig,
-                           (const int32_t *)big);
+        const size_t lds = ((size_t)N * N + kBigCap) * 4;
+        const size_t ns = nc * kImageParts;
+        VH_HIP(hipMemsetAsync(cur + ns, 0, 4, s));
+        hipLaunchKernelGGL(k_lb_image<false>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes, ntri,
+                           gl, N, d_pmax, scene_mag, cur, nullptr, Lists{nullptr, nullptr});
         VH_HIP(hipGetLastError());
         VH_HIP(hipcub::DeviceScan::ExclusiveSum(c->sort_tmp.ptr, scan_bytes, cur, (uint32_t *)c->lb_off.ptr,
-                                                (int)(nc + 1), s));
-        VH_HIP(hipMemcpyAsync(c->lb_pinned, (uint32_t *)c->lb_off.ptr + nc, 4, hipMemcpyDeviceToHost, s));
+                                                (int)(ns + 1), s));
+        VH_HIP(hipMemcpyAsync(c->lb_pinned, (uint32_t *)c->lb_off.ptr + ns, 4, hipMemcpyDeviceToHost, s));
         VH_WAIT(c, s);
         const uint32_t total = *(volatile uint32_t *)c->lb_pinned;
         // keys (8 B) and cells (4 B) in fill order; records: depth (4 B), plane (16 B), edges (32 B)
         if ((rc = c->lb_ent.ensure((size_t)total * 12 + 64, "light-buffer lists")) ||
             (rc = c->lb_rec.ensure((size_t)total * 52 + 256, "light-buffer records")))
             return rc;
-        VH_HIP(hipMemcpyAsync(cur, off, nc * 4, hipMemcpyDeviceToDevice, s));
         Lists ent;
         ent.key = (int2 *)c->lb_ent.ptr;
         ent.cell = (uint32_t *)((char *)c->lb_ent.ptr + (size_t)total * 8);
@@ -463,10 +454,8 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
         rec.plane = (float4 *)c->lb_rec.ptr;
         rec.edge = (float4 *)((char *)c->lb_rec.ptr + (size_t)total * 16);
         rec.depth = (uint32_t *)((char *)c->lb_rec.ptr + (size_t)total * 48);
-        hipLaunchKernelGGL(k_lb_cells<true>, dim3(pblocks), dim3(256), 0, s, boxes, ntri, gl, nlg, N, d_pmax,
-                           scene_mag, cur, off, ent, d_nbig, big);
-        hipLaunchKernelGGL(k_lb_big<true>, dim3(big_blocks), dim3(256), 0, s, boxes, ntri, gl, N, d_pmax,
-                           scene_mag, cur, off, ent, (const uint32_t *)d_nbig, (const int32_t *)big);
+        hipLaunchKernelGGL(k_lb_image<true>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes, ntri,
+                           gl, N, d_pmax, scene_mag, nullptr, off, ent);
         VH_HIP(hipGetLastError());
         if (total > 0)
             hipLaunchKernelGGL(k_lb_records, dim3((total + 255) / 256), dim3(256), 0, s, (const int2 *)ent.key,

@@ -149,7 +149,7 @@ int vh_trace_point_lights(vh_ctx *ctx, const float *points, int64_t npoints,
  *    up to its own depth;
  *  - VH_POINT_LIGHTS_AUTO (the default): light buffers for batches of at least
  *    65,536 points and at least as many points as triangles, else the BVH.
- * `cells` is 4..256 (0: 64).  vh_trace_point_lights_noalloc always walks the
+ * `cells` is 4..120 (0: 64).  vh_trace_point_lights_noalloc always walks the
  * BVH. */
 #define VH_POINT_LIGHTS_AUTO 0
 #define VH_POINT_LIGHTS_BVH 1
