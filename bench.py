@@ -7,9 +7,12 @@ FP32 shadow rays per step, generated on the GPU in vrad's correct-mode
 parameterisation.  Inputs are resident in HBM before the timed region.
 
 One JSON line on stdout.  `value` = shadow rays/s over the timed steps.
-`roofline` is for the traversal kernel (k_point_lights): achieved algorithmic
-FP32 work per launch (counted node visits x 44 + triangle tests x 27 FLOP)
-divided by the kernel's mean duration from HIP events on its stream.
+`roofline` is for the step's trace kernel: k_point_lights_lb (light buffers,
+the default at this size: counted cell lookups x 15 + list entries x 1 +
+triangle tests x 27 FLOP) or k_point_lights (the BVH walk: node visits x 44 +
+triangle tests x 27 FLOP), divided by the kernel's mean duration from HIP
+events on its stream.  `bvh_walk` times the walk on the same batch and
+compares its bits with the light buffer's.
 `cpu_baseline` is the oracle's CPU restatement (BVH + the reference's exact
 triangle test, any hit) timed on a bounded sample of the same points.
 """
@@ -901,6 +904,7 @@ def main(argv=None) -> int:
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
         "traffic": read_traffic(kernel),
+        "traffic_source": f"profiles/{kernel}_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)",
         "kernel": kernel,
         "kernel_ms": round(kernel_ms, 4),
         "flop_per_launch": flops,
