@@ -999,6 +999,8 @@ int build_scene_device(vh_ctx *c, const float *v9, const int32_t *ids, int32_t n
         c->hi[k] = *(const float *)&hi;
     }
     c->have_scene = true;
+    std::swap(c->verts, dv);  // the caller's triangles stay on the device (dv takes the old copy to release)
+    c->have_verts = true;
     c->scene_gen++;
     return VH_OK;
 }

@@ -253,6 +253,7 @@ int upload_scene(vh_ctx *c, const HostScene &hs)
         c->hi[k] = hs.hi[k];
     }
     c->have_scene = true;
+    c->have_verts = false;
     c->scene_gen++;
     return VH_OK;
 }
@@ -353,7 +354,7 @@ void destroy(vh_ctx *c)
     ambient_release(c);
     for (DevBuf *b : {&c->nodes, &c->tris, &c->rays_buf, &c->out_buf, &c->out2_buf, &c->pts_buf,
                       &c->lights_buf, &c->keys_a, &c->keys_b, &c->perm_a, &c->perm_b,
-                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_rec})
+                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_tverts, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_rec, &c->verts})
         b->release();
     if (c->lb_pinned) hipHostFree(c->lb_pinned);
     for (auto &v : c->pending)
@@ -495,7 +496,12 @@ int vh_scene_load_with(vh_ctx *c, const float *v9, const int32_t *ids, const uin
         HostScene hs;
         int rc = build_scene(v9, ids, flags, n, hs);
         if (rc) return rc;
-        return upload_scene(c, hs);
+        if ((rc = upload_scene(c, hs))) return rc;
+        if ((rc = c->verts.ensure((size_t)n * 36, "scene vertices"))) return rc;
+        VH_HIP(hipMemcpyAsync(c->verts.ptr, v9, (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+        VH_HIP(hipStreamSynchronize(c->stream));
+        c->have_verts = true;
+        return VH_OK;
     }
     if (n < 16) {
         set_error("the device builder takes at least 16 triangles");
@@ -507,6 +513,7 @@ int vh_scene_load_with(vh_ctx *c, const float *v9, const int32_t *ids, const uin
     }
     (void)flags;  // transparent triangles are opaque here, as in KdRayTracer.TestLines
     c->have_scene = false;
+    c->have_verts = false;
     return build_scene_device(c, v9, ids, n);
 }
 

@@ -99,14 +99,18 @@ struct vh_ctx {
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool have_scene = false;
     uint64_t scene_gen = 0;  // changes with every scene load
+    // the caller's triangles (9 floats each, in the caller's order), kept by
+    // vh_scene_load for the light buffers' rasterisation
+    vh::DevBuf verts;
+    bool have_verts = false;
     // point-light method (VH_POINT_LIGHTS_*) and the light buffers' cells per
     // cube-face edge (0: the default)
     int32_t pl_method = 0, lb_cells = 0;
-    // light buffers (vh_lightbuf.hip): per-triangle boxes of the scene
-    // generation lb_boxes_gen, cell offsets and counts, the lists as
+    // light buffers (vh_lightbuf.hip): per-triangle boxes and vertices (leaf
+    // order) of the scene generation lb_boxes_gen, cell offsets and counts, the lists as
     // (depth, triangle) and as triangle records, and a pinned word for the
     // list size
-    vh::DevBuf lb_boxes, lb_off, lb_cur, lb_ent, lb_rec;
+    vh::DevBuf lb_boxes, lb_tverts, lb_off, lb_cur, lb_ent, lb_rec;
     uint64_t lb_boxes_gen = ~0ull;
     uint32_t *lb_pinned = nullptr;
     // per-call staging

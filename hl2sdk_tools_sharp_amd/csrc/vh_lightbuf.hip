@@ -59,6 +59,17 @@ __global__ void k_lb_boxes(const float4 *__restrict__ nodes, int32_t nnodes,
     }
 }
 
+// Each triangle's vertices in leaf order (3 float4 a triangle, w unused),
+// gathered from the caller's order so the build's loads are coalesced.
+__global__ void k_lb_tverts(const float4 *__restrict__ tris, int32_t ntri, const float *__restrict__ verts,
+                            float4 *__restrict__ tv)
+{
+    const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= ntri) return;
+    const float *v = verts + (size_t)(__float_as_uint(tris[(size_t)k * 3 + 2].w) >> kOrigShift) * 9;
+    for (int q = 0; q < 3; q++) tv[(size_t)k * 3 + q] = make_float4(v[q * 3 + 0], v[q * 3 + 1], v[q * 3 + 2], 0.0f);
+}
+
 // The largest finite |coordinate| of the points, as float bits (non-negative
 // floats order as their bits).  *out must start at 0.
 __global__ void k_lb_pmax(const float *__restrict__ pts, int64_t n, uint32_t *__restrict__ out)
@@ -84,8 +95,7 @@ __device__ __forceinline__ bool pair_box(const float4 *__restrict__ boxes, const
     const float Lx = lights[l * 3 + 0], Ly = lights[l * 3 + 1], Lz = lights[l * 3 + 2];
     if (!(fabsf(Lx) <= 3.4028235e38f && fabsf(Ly) <= 3.4028235e38f && fabsf(Lz) <= 3.4028235e38f))
         return false;  // a ray to a non-finite light is never traced
-    const float lmag = fmaxf(fmaxf(fabsf(Lx), fabsf(Ly)), fabsf(Lz));
-    const float delta = 4.76837158203125e-7f * (lmag + pmax + scene_mag) + 9.3132257e-10f;  // 2^-21 (...) + 2^-30
+    const float delta = lb_delta(Lx, Ly, Lz, pmax, scene_mag);
     const float4 blo = boxes[(size_t)k * 2], bhi = boxes[(size_t)k * 2 + 1];
     if (!(blo.x <= bhi.x)) return false;  // an empty leaf box holds no triangle
     lo[0] = blo.x - Lx - delta;
@@ -118,11 +128,35 @@ constexpr int kBigCap = 256;
 // (the few past the queue's capacity are done by their own thread); either
 // way each (triangle, cell) is put once, so the two passes agree.  LDS atomics
 // only: no two workgroups share a slot.
+// The cells of triangle k (leaf order) on face (a, side) of light l
+// (lb_cells_*): its widened leaf box, cut to its widened projection when the
+// scene's vertices are kept (tv, leaf order; loaded only for a rectangle worth
+// cutting).  Returns false when none.
+__device__ __forceinline__ bool pair_cells(const float4 *__restrict__ boxes, const float4 *__restrict__ tv,
+                                           const float *__restrict__ lights, int32_t l, int32_t k, int a, int side,
+                                           int N, float pmax, float scene_mag, float pad, LbCells &pc)
+{
+    float lo[3], hi[3];
+    if (!pair_box(boxes, lights, l, k, pmax, scene_mag, lo, hi) || !lb_cells_box(lo, hi, a, side, N, pc))
+        return false;
+    if (!tv || !lb_cells_cut(pc)) return true;
+    const float Lx = lights[l * 3 + 0], Ly = lights[l * 3 + 1], Lz = lights[l * 3 + 2];
+    float v[9];
+    for (int q = 0; q < 3; q++) {
+        const float4 p = tv[(size_t)k * 3 + q];
+        v[q * 3 + 0] = p.x - Lx;
+        v[q * 3 + 1] = p.y - Ly;
+        v[q * 3 + 2] = p.z - Lz;
+    }
+    const float r = (pad + 2.0f * lb_delta(Lx, Ly, Lz, pmax, scene_mag)) * 1.0009765625f;
+    return lb_cells_raster(v, a, side, N, r, pc);
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(kImageThreads) void k_lb_image(
-    const float4 *__restrict__ boxes, int32_t ntri, const float *__restrict__ lights, int32_t N,
-    const uint32_t *__restrict__ pmax_bits, float scene_mag, uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ off, Lists ls)
+    const float4 *__restrict__ boxes, const float4 *__restrict__ tv, int32_t ntri,
+    const float *__restrict__ lights, int32_t N, const uint32_t *__restrict__ pmax_bits,
+    float scene_mag, float pad, uint32_t *__restrict__ count, const uint32_t *__restrict__ off, Lists ls)
 {
     extern __shared__ uint32_t sh[];
     __shared__ uint32_t nbig;
@@ -153,30 +187,30 @@ __global__ __launch_bounds__(kImageThreads) void k_lb_image(
     const int32_t k0 = (int32_t)(((int64_t)ntri * part) / kImageParts);
     const int32_t k1 = (int32_t)(((int64_t)ntri * (part + 1)) / kImageParts);
     for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += blockDim.x) {
-        float lo[3], hi[3];
-        FaceRect r;
-        if (!pair_box(boxes, lights, l, k, pmax, scene_mag, lo, hi) || !lb_face_rect(lo, hi, a, side, N, r))
-            continue;
-        if ((r.iu1 - r.iu0 + 1) * (r.iw1 - r.iw0 + 1) > kInlineCells) {
+        LbCells pc;
+        if (!pair_cells(boxes, tv, lights, l, k, a, side, N, pmax, scene_mag, pad, pc)) continue;
+        if ((pc.iu1 - pc.iu0 + 1) * (pc.iw1 - pc.iw0 + 1) > kInlineCells) {
             const uint32_t q = atomicAdd(&nbig, 1u);
             if (q < (uint32_t)kBigCap) {
                 bigk[q] = k;
                 continue;
             }
         }
-        for (int iu = r.iu0; iu <= r.iu1; iu++)
-            for (int iw = r.iw0; iw <= r.iw1; iw++) put(iu * N + iw, r.dlo, k);
+        for (int iu = pc.iu0; iu <= pc.iu1; iu++)
+            for (int iw = pc.iw0; iw <= pc.iw1; iw++)
+                if (lb_cells_has(pc, iu, iw, N)) put(iu * N + iw, pc.box.dlo, k);
     }
     __syncthreads();
     const uint32_t nb = nbig < (uint32_t)kBigCap ? nbig : (uint32_t)kBigCap;
     for (uint32_t q = 0; q < nb; q++) {
         const int32_t k = bigk[q];
-        float lo[3], hi[3];
-        FaceRect r;
-        pair_box(boxes, lights, l, k, pmax, scene_mag, lo, hi);
-        lb_face_rect(lo, hi, a, side, N, r);
-        const int wu = r.iw1 - r.iw0 + 1, cells = (r.iu1 - r.iu0 + 1) * wu;
-        for (int t = threadIdx.x; t < cells; t += blockDim.x) put((r.iu0 + t / wu) * N + r.iw0 + t % wu, r.dlo, k);
+        LbCells pc;
+        pair_cells(boxes, tv, lights, l, k, a, side, N, pmax, scene_mag, pad, pc);
+        const int wu = pc.iw1 - pc.iw0 + 1, cells = (pc.iu1 - pc.iu0 + 1) * wu;
+        for (int t = threadIdx.x; t < cells; t += blockDim.x) {
+            const int iu = pc.iu0 + t / wu, iw = pc.iw0 + t % wu;
+            if (lb_cells_has(pc, iu, iw, N)) put(iu * N + iw, pc.box.dlo, k);
+        }
     }
     if (!FILL) {
         __syncthreads();
@@ -397,10 +431,15 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
     const int N = lb_resolution(c);
     const int32_t ntri = c->ntri;
     if ((rc = c->lb_boxes.ensure((size_t)ntri * 32, "light-buffer boxes"))) return rc;
+    if (c->have_verts && (rc = c->lb_tverts.ensure((size_t)ntri * 48, "light-buffer vertices"))) return rc;
     if (c->lb_boxes_gen != c->scene_gen) {
         hipLaunchKernelGGL(k_lb_boxes, dim3((uint32_t)((c->nnodes + 255) / 256)), dim3(256), 0, s,
                            (const float4 *)c->nodes.ptr, c->nnodes, (const float4 *)c->tris.ptr,
                            (float4 *)c->lb_boxes.ptr);
+        if (c->have_verts)
+            hipLaunchKernelGGL(k_lb_tverts, dim3((uint32_t)((ntri + 255) / 256)), dim3(256), 0, s,
+                               (const float4 *)c->tris.ptr, ntri, (const float *)c->verts.ptr,
+                               (float4 *)c->lb_tverts.ptr);
         VH_HIP(hipGetLastError());
         c->lb_boxes_gen = c->scene_gen;
     }
@@ -419,6 +458,8 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
     VH_HIP(hipGetLastError());
     float scene_mag = 0.0f;
     for (int k = 0; k < 3; k++) scene_mag = fmaxf(scene_mag, fmaxf(fabsf(c->lo[k]), fabsf(c->hi[k])));
+    const float pad = lb_pad(scene_mag);
+    const float4 *tv = c->have_verts ? (const float4 *)c->lb_tverts.ptr : nullptr;
     size_t scan_bytes = 0;
     VH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t *)c->lb_cur.ptr,
                                             (uint32_t *)c->lb_off.ptr, (int)(nslot + 1), s));
@@ -435,8 +476,8 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
         const size_t lds = ((size_t)N * N + kBigCap) * 4;
         const size_t ns = nc * kImageParts;
         VH_HIP(hipMemsetAsync(cur + ns, 0, 4, s));
-        hipLaunchKernelGGL(k_lb_image<false>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes, ntri,
-                           gl, N, d_pmax, scene_mag, cur, nullptr, Lists{nullptr, nullptr});
+        hipLaunchKernelGGL(k_lb_image<false>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes,
+                           tv, ntri, gl, N, d_pmax, scene_mag, pad, cur, nullptr, Lists{nullptr, nullptr});
         VH_HIP(hipGetLastError());
         VH_HIP(hipcub::DeviceScan::ExclusiveSum(c->sort_tmp.ptr, scan_bytes, cur, (uint32_t *)c->lb_off.ptr,
                                                 (int)(ns + 1), s));
@@ -454,8 +495,8 @@ int launch_point_lights_lb(vh_ctx *c, const float *d_points, const int32_t *orde
         rec.plane = (float4 *)c->lb_rec.ptr;
         rec.edge = (float4 *)((char *)c->lb_rec.ptr + (size_t)total * 16);
         rec.depth = (uint32_t *)((char *)c->lb_rec.ptr + (size_t)total * 48);
-        hipLaunchKernelGGL(k_lb_image<true>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes, ntri,
-                           gl, N, d_pmax, scene_mag, nullptr, off, ent);
+        hipLaunchKernelGGL(k_lb_image<true>, dim3((uint32_t)(nlg * 6 * kImageParts)), dim3(kImageThreads), lds, s, boxes,
+                           tv, ntri, gl, N, d_pmax, scene_mag, pad, nullptr, off, ent);
         VH_HIP(hipGetLastError());
         if (total > 0)
             hipLaunchKernelGGL(k_lb_records, dim3((total + 255) / 256), dim3(256), 0, s, (const int2 *)ent.key,
