@@ -72,6 +72,15 @@ def read_traffic(kernel: str, key: str = "hbm_bytes_per_launch"):
     return float(m.group(1)) if m else None
 
 
+def read_profile_value(name: str, key: str):
+    """A figure from a committed rocprofv3 summary (profiles/<name>.txt), if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(rf"^{key}:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(w, gpu_bits, target_s: float, threads: int):
     import oracle
 
@@ -905,6 +914,9 @@ def main(argv=None) -> int:
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
         "traffic": read_traffic(kernel),
         "traffic_source": f"profiles/{kernel}_traffic.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)",
+        # the limit the kernel meets: vector instruction issue (at most 1 wave-instruction per CU-cycle),
+        # from the SQ counters of the committed profile
+        "valu_issue_per_cu_cycle": read_profile_value(f"{kernel}_visibility_128M", "valu_issue_per_cu_cycle"),
         "kernel": kernel,
         "kernel_ms": round(kernel_ms, 4),
         "flop_per_launch": flops,
