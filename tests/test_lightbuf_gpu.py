@@ -173,3 +173,29 @@ def test_light_buffer_visibility_config_full():
     assert n.value == 1  # one 64-light group, built once
     frac = unpack_bits(b, w.rays).mean()
     assert 0.5 < frac < 0.95
+
+
+@pytest.mark.parametrize("nx,tile,cells", [(3, 0.0, 0), (5, 128.0, 0), (4, 256.0, 9)])
+def test_light_buffer_room_grid(nx, tile, cells):
+    """Closed rooms joined by doorways, the lights inside the rooms and the
+    points one unit off the walls (the bounce map's direct-light call, at a
+    size the oracle finishes in seconds): most rays end on a wall, every
+    triangle lies in an axis plane (many in a light's own), and tiled faces
+    give many small triangles near each light."""
+    from hl2sdk_tools_sharp_amd import rooms
+    if tile:
+        g = rooms.room_grid(nx, nx, size=rooms.BOUNCE_ROOM, door=rooms.BOUNCE_DOOR, patch=tile,
+                            minchop=tile, tile=tile, nlights=2 * nx * nx, seed=7)
+    else:
+        g = rooms.room_grid(nx, nx, nlights=12, seed=5)
+    pts = rooms.sample_points(g)
+    print(f"[light buffer] room grid {nx}x{nx} tile {tile}: {g.casters.count} tris, "
+          f"{len(pts)} points x {len(g.lights)} lights", flush=True)
+    tris = oracle.convert_triangles(g.casters.vertices, g.casters.ids)
+    want = oracle.CpuBvh(g.casters.vertices, tris).point_lights(pts, g.lights)
+    with make_tracer(g.casters.vertices, g.casters.ids) as tr:
+        a, b = both(tr, pts, g.lights, 0.0, cells)
+    assert np.array_equal(a, want)
+    assert np.array_equal(b, want)
+    frac = unpack_bits(b, len(pts) * len(g.lights)).mean()
+    assert 0.3 < frac < 0.999
