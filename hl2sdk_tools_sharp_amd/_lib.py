@@ -140,6 +140,7 @@ SIGNATURES = {
     "vh_transfers_load": (ctypes.c_int, [P, P, P, P]),
     "vh_bounce": (ctypes.c_int, [P, ctypes.POINTER(VhBounceOpts), P, P, P, P, P, ctypes.POINTER(I32)]),
     "vh_gather_load": (ctypes.c_int, [P, P]),
+    "vh_gather_skybox": (ctypes.c_int, [P, P]),
     "vh_gather_run": (ctypes.c_int, [P, I32, I32]),
     "vh_gather_run_shaped": (ctypes.c_int, [P, I32, I32, I32]),
     "vh_gather_read": (ctypes.c_int, [P, P, P, P]),

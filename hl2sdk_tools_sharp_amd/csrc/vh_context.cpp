@@ -354,7 +354,8 @@ void destroy(vh_ctx *c)
     ambient_release(c);
     for (DevBuf *b : {&c->nodes, &c->tris, &c->rays_buf, &c->out_buf, &c->out2_buf, &c->pts_buf,
                       &c->lights_buf, &c->keys_a, &c->keys_b, &c->perm_a, &c->perm_b,
-                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_tverts, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_rec, &c->verts})
+                      &c->sort_tmp, &c->counters, &c->lb_boxes, &c->lb_tverts, &c->lb_off, &c->lb_cur, &c->lb_ent, &c->lb_rec, &c->verts,
+                      &c->sb_cam, &c->sb_nodes, &c->sb_planes, &c->sb_recurse})
         b->release();
     if (c->lb_pinned) hipHostFree(c->lb_pinned);
     for (auto &v : c->pending)

@@ -142,6 +142,12 @@ struct GatherArgs {
     unsigned long long *counters;  // COUNT launches: node visits, triangle tests, node and triangle slots
     int32_t light_filter;      // 0 every light; 1 sky ambient only; 2 all but sky ambient
     int32_t no_skip;           // ResolveResample: every light's record is added
+    // the 3D skybox (vh_gather_skybox); sb_ncam 0 when there is none
+    int32_t sb_ncam, sb_nnodes, sb_nleaves;
+    const float4 *sb_cam;        // origin, world_to_sky
+    const int4 *sb_nodes;        // children, plane type
+    const float4 *sb_planes;     // the node's plane: normal, dist
+    const uint8_t *sb_recurse;   // per leaf
 };
 
 __device__ __forceinline__ float maxps(float a, float b) { return a > b ? a : b; }
@@ -214,6 +220,78 @@ __device__ __forceinline__ float sky_occlusion(const GatherArgs &a, int *wstk, c
     return (fin && h >= 0 && (id & VH_TRACE_ID_SKY) == 0) ? 1.0f : 0.0f;
 }
 
+// Whether a sky test starting at p continues into the skyboxes:
+// CompiledBspTree.LeafFromPoint (Rad/Light/CompiledBspTree.cs) from node 0 --
+// an axial plane (type < 3) compares its one coordinate, any other the dot
+// product, and a point on the plane goes to the front child; a child past the
+// last node is leaf 0 -- then the leaf's flag (DirectLightGatherer.LaneRecurses,
+// DirectLightGatherer.cs:1611-1621: an area of the map with no camera in it).
+// The host has checked that no descent meets a node twice, so it ends within
+// sb_nnodes steps.
+__device__ __forceinline__ bool skybox_recurses(const GatherArgs &a, float px, float py, float pz)
+{
+    int32_t node = 0;
+    for (int32_t step = 0; node >= 0 && step <= a.sb_nnodes; step++) {
+        if (node >= a.sb_nnodes) {
+            node = -1;  // leaf 0
+            break;
+        }
+        const int4 nd = a.sb_nodes[node];
+        const float4 pl = a.sb_planes[node];
+        float dist;
+        if (nd.z < 3)
+            dist = (nd.z == 0 ? px : (nd.z == 1 ? py : pz)) - pl.w;
+        else
+            dist = dot3(pl.x, pl.y, pl.z, px, py, pz) - pl.w;
+        node = dist < 0.0f ? nd.y : nd.x;
+    }
+    const int32_t leaf = -1 - node;
+    return node < 0 && leaf < a.sb_nleaves && a.sb_recurse[leaf] != 0;
+}
+
+// TestLineDoesHitSky (DirectLightGatherer.cs:510-714) for this lane's segment
+// p -> e: the occlusion, before its clamp to [0, 1].  `want` lanes trace their
+// first ray (LightRayLog.Occlusion: a hit short of the end that is not sky);
+// with a skybox, a lane whose start recurses (`rec`) in a group whose first
+// rays are not all blocked casts one more ray per camera from origin +
+// start * world_to_sky along its direction normalised (Vec3.Normalise: three
+// divides by the length, a zero vector for a zero length), MaxTraceLength
+// long, and adds its answer as the reference does: occlusion + 1 - fraction.
+// All four lanes of a group call this together (the group's "all blocked"
+// is a shuffle); padding lanes pass want = false, and the group's other lanes
+// then decide it, as the padding copies the last real lane.
+template <bool COUNT>
+__device__ __forceinline__ float sky_test(const GatherArgs &a, int *wstk, bool want, bool rec, float px,
+                                          float py, float pz, float ex, float ey, float ez, int &nr, Work &wk)
+{
+    float occ = 0.0f;
+    if (want) occ = sky_occlusion<COUNT>(a, wstk, segment(px, py, pz, ex, ey, ez), nr, wk);
+    if (a.sb_ncam == 0) return occ;
+    const bool fully = group_all(!want || occ >= 1.0f);
+    const bool go = want && rec && !fully;
+    if (__builtin_amdgcn_ballot_w64(go) == 0) return occ;
+    const float vx = ex - px, vy = ey - py, vz = ez - pz;
+    const float len = sqrtf((vx * vx) + (vy * vy) + (vz * vz));
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (len != 0.0f) {
+        ux = vx / len;
+        uy = vy / len;
+        uz = vz / len;
+    }
+    for (int32_t k = 0; k < a.sb_ncam; k++) {
+        const float4 cam = a.sb_cam[k];
+        const float sx = cam.x + (px * cam.w), sy = cam.y + (py * cam.w), sz = cam.z + (pz * cam.w);
+        const float tx = sx + (ux * kMaxTrace), ty = sy + (uy * kMaxTrace), tz = sz + (uz * kMaxTrace);
+        float inner = 0.0f;
+        if (go) inner = sky_occlusion<COUNT>(a, wstk, segment(sx, sy, sz, tx, ty, tz), nr, wk);
+        if (go) {
+            const float fraction = 1.0f - minps(maxps(inner, 0.0f), 1.0f);
+            occ = occ + 1.0f - fraction;
+        }
+    }
+    return occ;
+}
+
 __device__ __forceinline__ float inverse_quadratic(const vh_light &l, float d)
 {
     float f = d * d;
@@ -250,6 +328,9 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
     float py = a.pos[3 * (size_t)s + 1] + n0y;
     float pz = a.pos[3 * (size_t)s + 2] + n0z;
     int32_t cl = a.cluster[s];
+    // whether this lane's sky tests continue into the skyboxes (its start is
+    // its point: FaceLightJob passes no epsilon, FaceLightJob.cs:392,514)
+    const bool rec = a.sb_ncam > 0 && skybox_recurses(a, px, py, pz);
 
     int nr = 0;
     // this group's index in its face: the first-touch keys order groups
@@ -360,8 +441,9 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                         ey = ey + (jt[1] * js);
                         ez = ez + (jt[2] * js);
                     }
-                    float occ = 0.0f;
-                    if (real) occ = sky_occlusion<COUNT>(a, stk, segment(px, py, pz, ex + px, ey + py, ez + pz), nr, wk);
+                    // EmitSkyLight traces every real lane (DirectLightGatherer.cs:780-790)
+                    const float occ =
+                        sky_test<COUNT>(a, stk, real, rec, px, py, pz, ex + px, ey + py, ez + pz, nr, wk);
                     total += 1.0f - minps(maxps(occ, 0.0f), 1.0f);
                 }
                 float see = total * (1.0f / (float)a.sun_samples);
@@ -374,7 +456,7 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                              see;
             }
         } else if (L.type == VH_EMIT_SKYAMBIENT) {
-            // AmbientSky (DirectLightGatherer.cs:847-1012), no 3D skybox
+            // AmbientSky (DirectLightGatherer.cs:847-1012)
             float sumdot = 0.0f;
             float amb[4] = {0, 0, 0, 0}, poss[4] = {0, 0, 0, 0};
             for (int32_t k = 0; k < a.sky_count; k++) {
@@ -394,13 +476,14 @@ __device__ void light_chunk_n(const GatherArgs &a, int64_t chunk, int *stk)
                     poss[n] = (valid && v2 ? 1.0f : 0.0f) + poss[n];
                     tr = tr || v2;
                 }
-                float frac = 1.0f;
-                if (real && tr && inmask)
-                    frac = 1.0f - sky_occlusion<COUNT>(a, stk,
-                                                segment(px, py, pz, (ax * -kMaxTrace) + px,
-                                                        (ay * -kMaxTrace) + py,
-                                                        (az * -kMaxTrace) + pz),
-                                                nr, wk);
+                // EmitAmbientRays / ResolveAmbientRays (DirectLightGatherer.cs:1014-1069):
+                // with a skybox every real lane is traced through the sky test
+                // (the recursion couples the group's lanes); without one only a
+                // lane whose answer can count
+                const bool want = a.sb_ncam > 0 ? real : (real && tr && inmask);
+                const float occ = sky_test<COUNT>(a, stk, want, rec, px, py, pz, (ax * -kMaxTrace) + px,
+                                                  (ay * -kMaxTrace) + py, (az * -kMaxTrace) + pz, nr, wk);
+                const float frac = want ? 1.0f - minps(maxps(occ, 0.0f), 1.0f) : 1.0f;
                 _Pragma("unroll") for (int n = 0; n < normals; n++) amb[n] = amb[n] + (frac * dts[n]);
             }
             falloff = 1.0f;
@@ -666,6 +749,13 @@ GatherArgs gather_args(vh_ctx *c)
     a.counters = (unsigned long long *)c->counters.ptr;
     a.light_filter = 0;
     a.no_skip = 0;
+    a.sb_ncam = c->sb_ncam;
+    a.sb_nnodes = c->sb_nnodes;
+    a.sb_nleaves = c->sb_nleaves;
+    a.sb_cam = (const float4 *)c->sb_cam.ptr;
+    a.sb_nodes = (const int4 *)c->sb_nodes.ptr;
+    a.sb_planes = (const float4 *)c->sb_planes.ptr;
+    a.sb_recurse = (const uint8_t *)c->sb_recurse.ptr;
     return a;
 }
 
@@ -1019,6 +1109,89 @@ int vh_gather_load(vh_ctx *c, const vh_gather_desc *d)
         (rc = G->ticket.ensure(16, "queue ticket")))
         return rc;
     VH_WAIT(c, st);
+    return VH_OK;
+}
+
+int vh_gather_skybox(vh_ctx *c, const vh_skybox_desc *d)
+{
+    clear_error();
+    if (!c) {
+        set_error("NULL context");
+        return VH_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!d || d->ncameras == 0) {
+        c->sb_ncam = 0;
+        return VH_OK;
+    }
+    const int32_t NC = d->ncameras, NN = d->nnodes, NP = d->nplanes, NLF = d->nleaves;
+    if (NC < 0 || NC > VH_SKYBOX_MAX_CAMERAS) {
+        set_error("a skybox holds 0 to " + std::to_string(VH_SKYBOX_MAX_CAMERAS) + " cameras");
+        return VH_EINVAL;
+    }
+    if (NN < 0 || NP < 0 || NLF < 0 || !d->camera_origin || !d->camera_world_to_sky ||
+        (NN > 0 && (!d->node_plane || !d->node_children)) ||
+        (NP > 0 && (!d->plane_normal || !d->plane_dist || !d->plane_type)) || (NLF > 0 && !d->leaf_recurses)) {
+        set_error("incomplete skybox descriptor");
+        return VH_EINVAL;
+    }
+    std::vector<float4> cam(NC);
+    for (int32_t k = 0; k < NC; k++) {
+        const float *o = d->camera_origin + 3 * (size_t)k;
+        const float w = d->camera_world_to_sky[k];
+        if (!std::isfinite(o[0]) || !std::isfinite(o[1]) || !std::isfinite(o[2]) || !std::isfinite(w)) {
+            set_error("sky camera " + std::to_string(k) + " is not finite");
+            return VH_EINVAL;
+        }
+        cam[k] = make_float4(o[0], o[1], o[2], w);
+    }
+    // node records (children, plane type) and each node's plane; the walk
+    // from node 0 must reach no node twice, so every descent ends
+    std::vector<int4> nodes((size_t)NN > 0 ? NN : 1);
+    std::vector<float4> planes((size_t)NN > 0 ? NN : 1);
+    for (int32_t i = 0; i < NN; i++) {
+        const int32_t p = d->node_plane[i];
+        if (p < 0 || p >= NP) {
+            set_error("skybox node " + std::to_string(i) + " names plane " + std::to_string(p) + " of " +
+                      std::to_string(NP));
+            return VH_EINVAL;
+        }
+        nodes[i] = make_int4(d->node_children[2 * (size_t)i], d->node_children[2 * (size_t)i + 1],
+                             d->plane_type[p], 0);
+        planes[i] = make_float4(d->plane_normal[3 * (size_t)p], d->plane_normal[3 * (size_t)p + 1],
+                                d->plane_normal[3 * (size_t)p + 2], d->plane_dist[p]);
+    }
+    if (NN > 0) {
+        std::vector<uint8_t> seen(NN, 0);
+        std::vector<int32_t> todo{0};
+        seen[0] = 1;
+        while (!todo.empty()) {
+            const int32_t i = todo.back();
+            todo.pop_back();
+            for (int k = 0; k < 2; k++) {
+                const int32_t ch = k == 0 ? nodes[i].x : nodes[i].y;
+                if (ch < 0 || ch >= NN) continue;
+                if (seen[ch]) {
+                    set_error("skybox node " + std::to_string(ch) + " is reached twice from node 0");
+                    return VH_EINVAL;
+                }
+                seen[ch] = 1;
+                todo.push_back(ch);
+            }
+        }
+    }
+    VH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    int rc;
+    if ((rc = upload(c->sb_cam, cam.data(), cam.size(), "sky cameras", st)) ||
+        (rc = upload(c->sb_nodes, nodes.data(), (size_t)NN, "skybox BSP nodes", st)) ||
+        (rc = upload(c->sb_planes, planes.data(), (size_t)NN, "skybox BSP planes", st)) ||
+        (rc = upload(c->sb_recurse, d->leaf_recurses, (size_t)NLF, "skybox leaves", st)))
+        return rc;
+    VH_WAIT(c, st);
+    c->sb_ncam = NC;
+    c->sb_nnodes = NN;
+    c->sb_nleaves = NLF;
     return VH_OK;
 }
 

@@ -113,6 +113,11 @@ struct vh_ctx {
     vh::DevBuf lb_boxes, lb_tverts, lb_off, lb_cur, lb_ent, lb_rec;
     uint64_t lb_boxes_gen = ~0ull;
     uint32_t *lb_pinned = nullptr;
+    // the 3D skybox behind the gather's sky rays (vh_gather_skybox): cameras
+    // as (origin, world_to_sky), per BSP node (children, plane type) and its
+    // plane (normal, dist), per leaf whether a start there recurses
+    vh::DevBuf sb_cam, sb_nodes, sb_planes, sb_recurse;
+    int32_t sb_ncam = 0, sb_nnodes = 0, sb_nleaves = 0;
     // per-call staging
     vh::DevBuf rays_buf, out_buf, out2_buf, pts_buf, lights_buf;
     vh::DevBuf keys_a, keys_b, perm_a, perm_b, sort_tmp;

@@ -184,10 +184,15 @@ class VhGatherDesc(ctypes.Structure):
 
 def pack_desc(faces: FaceSet, lights: np.ndarray, light_pvs: Optional[np.ndarray], sun_extent: float,
               sun_samples: int, sky_dirs: Optional[np.ndarray], ao_count: int = 0,
-              ao_radius: float = 40.0):
+              ao_radius: float = 40.0, skybox=None):
     """Builds the descriptor (shared by the product and the oracle, whose
-    struct has the same layout) and the arrays it points into."""
+    struct has the same layout) and the arrays it points into; ``skybox``
+    (skybox.Skybox or None) rides along as keep["skybox_desc"], the
+    vh_skybox_desc the device loads with vh_gather_skybox."""
     keep = {}
+    if skybox is not None and skybox.ncameras > 0:
+        keep["skybox"] = skybox
+        keep["skybox_desc"] = skybox.pack()
 
     def c(name, a, dt):
         keep[name] = None if a is None else np.ascontiguousarray(a, dtype=dt)
@@ -232,11 +237,17 @@ class HipFaceLighter:
 
     def load(self, faces: FaceSet, lights: np.ndarray, light_pvs=None, sun_extent: float = 0.0,
              sun_samples: int = SUN_AREA_LIGHT_SAMPLES, sky_dirs=None, ao_count: int = 0,
-             ao_radius: float = 40.0) -> None:
-        d, keep = pack_desc(faces, lights, light_pvs, sun_extent, sun_samples, sky_dirs, ao_count, ao_radius)
+             ao_radius: float = 40.0, skybox=None) -> None:
+        """Uploads the faces, lights and tables; ``skybox`` (skybox.Skybox)
+        gives the sky rays their 3D-skybox stage, None removes it."""
+        d, keep = pack_desc(faces, lights, light_pvs, sun_extent, sun_samples, sky_dirs, ao_count, ao_radius,
+                            skybox)
         self._keep = (d, keep)
         self.faces = faces
-        HipRayTracer._raise(_lib.lib().vh_gather_load(self.tracer.handle, ctypes.byref(d)))
+        L = _lib.lib()
+        HipRayTracer._raise(L.vh_gather_load(self.tracer.handle, ctypes.byref(d)))
+        sb = keep.get("skybox_desc")
+        HipRayTracer._raise(L.vh_gather_skybox(self.tracer.handle, None if sb is None else ctypes.byref(sb)))
 
     def run(self, schedule: str = "queue", waves_per_cu: int = 0, chunks_per_ticket: int = 1) -> None:
         code = {"queue": 0, "per_face": 1, "grid": 2}[schedule]

@@ -646,3 +646,103 @@ def room_ambient_points(g: RoomGrid, per_room: int = 32, seed: int = 5, margin: 
     hi = g.room_hi - margin
     u = rng.random((len(lo), per_room, 3))
     return (lo[:, None, :] + u * (hi - lo)[:, None, :]).reshape(-1, 3).astype(np.float32)
+
+
+@dataclass
+class SkyboxMap:
+    """A room grid lit through sky ceilings, with two 3D skyboxes beside it:
+    the grid, the casters (the grid's and the skyboxes'), the map's sky-camera
+    entities and compiled BSP, and the skybox table the gather loads."""
+
+    grid: RoomGrid
+    casters: ShadowCasterSet
+    entities: list
+    tree: object  # skybox.CompiledBspTree
+    leaf_area: np.ndarray
+    area_count: int
+    skybox: object  # skybox.Skybox
+
+
+SKYBOX_AWAY = -8192.0  # the skyboxes' x, far outside the grid
+
+
+def skybox_grid(nx: int = 2, ny: int = 2, seed: int = 11, nlights: int = 4, closed: bool = False,
+                open_skybox: bool = False) -> SkyboxMap:
+    """An ``nx`` x ``ny`` room grid (``ny`` >= 2) whose ceilings are sky, and
+    two skybox rooms 512 x 512 x 256 at x = SKYBOX_AWAY (opaque floors, sky
+    walls and ceilings), each with a sky camera that sees the whole grid
+    scaled down inside it: camera 0 at scale 16, camera 1 at scale 32 (a
+    third camera, scale 0, is ignored).  An opaque roof over the scaled image
+    of room (0, 0) in skybox 0 and over room (nx-1, ny-1) in skybox 1 shades
+    those rooms' sky rays in the skybox, and a tower in skybox 0 stands in
+    the way of slanted ones.
+
+    The BSP: x = -4096 splits the skyboxes (leaf 1, area 1, the cameras'
+    area) from the grid; y between the first two rows of rooms splits row 1
+    and beyond (leaf 2, area 2) from row 0, which the plane 0.6 x + 0.8 y =
+    300 cuts into its far side (leaf 3, area 0) and a corner of room (0, 0)
+    (leaf 4, area 1: no recursion there, as in the skybox itself).
+
+    ``closed`` gives the grid opaque ceilings (no sky ray of the grid reaches
+    sky); ``open_skybox`` leaves out the roofs and the tower (every skybox ray
+    goes to sky)."""
+    from .scenes import box_triangles
+    from .skybox import CompiledBspTree, Skybox
+
+    if ny < 2:
+        raise ValueError("skybox_grid needs two rows of rooms")
+    g = room_grid(nx, ny, sky_ceiling=not closed, nlights=nlights, seed=seed)
+    W, D, H = (float(x) for x in (g.room_hi[0] - g.room_lo[0]))
+    wall = float(g.room_lo[1][0] - g.room_hi[0][0]) if nx > 1 else 8.0
+    ext = np.array([g.room_hi[:, 0].max(), g.room_hi[:, 1].max(), H], np.float64)
+    v9 = [g.casters.vertices]
+    ids = [g.casters.ids]
+    base = int(np.max(g.casters.ids & 0x00FFFFFF)) + 1
+    entities, cams = [], []
+    for k, (scale, cy, roof_room) in enumerate(((16.0, -2048.0, 0), (32.0, 2048.0, nx * ny - 1))):
+        c = np.array([SKYBOX_AWAY, cy, 0.0])
+        lo, hi = c - (256.0, 256.0, 0.0), c + (256.0, 256.0, 256.0)
+        for ax, sign, at, sky in ((2, +1, lo, False), (2, -1, hi, True), (0, +1, lo, True), (0, -1, hi, True),
+                                  (1, +1, lo, True), (1, -1, hi, True)):
+            flo, fhi = lo.copy(), hi.copy()
+            flo[ax] = fhi[ax] = at[ax]
+            w = _rect_winding(flo, fhi, ax, sign)
+            v9.append(np.array([np.concatenate([w[0], w[1], w[2]]), np.concatenate([w[0], w[2], w[3]])],
+                               np.float32))
+            ids.append(np.full(2, (TRACE_ID_SKY if sky else TRACE_ID_OPAQUE) | base, np.int32))
+            base += 1
+        # the camera sees the grid's box [0, ext] scaled by 1 / scale, centred
+        # on the skybox room, 32 units over its floor
+        origin = c + np.array([0.0, 0.0, 32.0]) - 0.5 * np.array([ext[0], ext[1], 0.0]) / scale
+        origin = origin.astype(np.float32)
+        cams.append((origin, scale))
+        rlo = (g.room_lo[roof_room] / scale + origin).astype(np.float64)
+        rhi = (g.room_hi[roof_room] / scale + origin).astype(np.float64)
+        # the roof overhangs the room's image by less than half a scaled wall
+        roof_lo = np.array([rlo[0] - 0.1, rlo[1] - 0.1, rhi[2] + 24.0])
+        roof_hi = np.array([rhi[0] + 0.1, rhi[1] + 0.1, rhi[2] + 28.0])
+        boxes_lo, boxes_hi = ([], []) if open_skybox else ([roof_lo], [roof_hi])
+        if k == 0 and not open_skybox:  # a tower beside the scaled grid
+            boxes_lo.append(np.array([origin[0] + ext[0] / scale + 8.0, origin[1], 0.0]))
+            boxes_hi.append(np.array([origin[0] + ext[0] / scale + 40.0, origin[1] + ext[1] / scale, 120.0]))
+        if boxes_lo:
+            bv, bid = box_triangles(np.array(boxes_lo), np.array(boxes_hi), first_id=base)
+            base += 6 * len(boxes_lo)
+            v9.append(bv)
+            ids.append(bid)
+        entities.append({"classname": "SKY_CAMERA" if k == 0 else "sky_camera",
+                         "origin": "%r %r %r" % tuple(float(x) for x in origin), "scale": "%g" % scale})
+        if k == 0:
+            entities.append({"classname": "sky_camera", "origin": "%r %r %r" % tuple(float(x) for x in origin),
+                             "scale": "0"})
+    casters = ShadowCasterSet(np.concatenate(v9).astype(np.float32), np.concatenate(ids).astype(np.int32))
+    row = float(g.room_hi[0][1]) + 0.5 * wall
+    tree = CompiledBspTree(
+        node_plane=[0, 1, 2],
+        node_children=[[1, -2], [-3, 2], [-4, -5]],
+        plane_normal=[[1, 0, 0], [0, 1, 0], [0.6, 0.8, 0.0]],
+        plane_dist=[-4096.0, row, 300.0],
+        plane_type=[0, 1, 3])
+    leaf_area = np.array([0, 1, 2, 0, 1], np.int32)
+    sb = Skybox.from_map(entities, tree, leaf_area, 3)
+    return SkyboxMap(g, casters, entities, tree, leaf_area, 3, sb)

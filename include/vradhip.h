@@ -320,8 +320,8 @@ int vh_bounce_bump_totals(vh_ctx *ctx, float *out);
 
 /* A DirectLight (SourceSharp.MapTools/Rad/Light/DirectLight.cs) as the gather
  * reads it; intensity is the LDR pass's, intensity_hdr the HDR pass's (the
- * two passes differ only there, DirectLightBuilder.cs:218-231).  Only style 0
- * is supported here. */
+ * two passes differ only there, DirectLightBuilder.cs:218-231).  style is the
+ * light style, 0-254 (the slots are described at vh_gather_styles). */
 typedef struct vh_light {
     int32_t type, style;
     float origin[3], normal[3], intensity[3], intensity_hdr[3];
@@ -363,6 +363,44 @@ typedef struct vh_gather_desc {
 
 /* Uploads the faces, samples, lights and tables (kept on the device). */
 int vh_gather_load(vh_ctx *ctx, const vh_gather_desc *desc);
+
+/* The 3D skybox behind the gather's sky rays: the map's sky cameras
+ * (SkyCameras, SourceSharp.MapTools/Rad/Light/SkyCameras.cs) and the compiled
+ * BSP that CompiledBspTree.LeafFromPoint (Rad/Light/CompiledBspTree.cs)
+ * descends.  With a camera loaded every sky test (TestLineDoesHitSky,
+ * Rad/Light/DirectLightGatherer.cs:510-714: each sun ray, and each sky-ambient
+ * direction, which then traces every real lane) continues into the skyboxes:
+ * a lane whose start lies in a leaf with leaf_recurses set, in a 4-sample group
+ * whose first rays did not all find something other than sky, casts one more
+ * ray per camera, from origin + start * world_to_sky along its first ray's
+ * normalised direction, MaxTraceLength long; a hit short of the end that is not
+ * sky adds 1 to its occlusion, which is clamped to [0, 1].  Each lane looks up
+ * its own start (the Correct side of StockQuirk.SkyboxRecursionFromLaneZero).
+ * Nodes (dnodes): a plane index and two children, >= 0 a node and < 0 the leaf
+ * -1 - child; a child past the last node is leaf 0.  Planes (dplanes): normal,
+ * dist and type; a type below 3 tests the one coordinate it names (0 x, 1 y,
+ * any other z), else the dot product; a point on the plane is in front.
+ * leaf_recurses: per leaf 1 when the leaf's area is one of the map's areas and
+ * no camera stands in it.  The skybox stays until replaced; ncameras 0 (or a
+ * NULL desc) removes it.  VH_EINVAL: more than VH_SKYBOX_MAX_CAMERAS cameras,
+ * a non-finite camera, a plane index out of range, or nodes that reach a node
+ * twice from node 0. */
+#define VH_SKYBOX_MAX_CAMERAS 64
+typedef struct vh_skybox_desc {
+    int32_t ncameras;
+    const float *camera_origin;       /* 3 per camera */
+    const float *camera_world_to_sky; /* 1 / scale per camera */
+    int32_t nnodes;
+    const int32_t *node_plane;
+    const int32_t *node_children; /* 2 per node */
+    int32_t nplanes;
+    const float *plane_normal; /* 3 per plane */
+    const float *plane_dist;
+    const int32_t *plane_type;
+    int32_t nleaves;
+    const uint8_t *leaf_recurses;
+} vh_skybox_desc;
+int vh_gather_skybox(vh_ctx *ctx, const vh_skybox_desc *desc);
 
 #define VH_SCHEDULE_QUEUE 0    /* one persistent launch; waves take 64-sample chunks from a queue */
 #define VH_SCHEDULE_PER_FACE 1 /* one launch per face (the unbatched form) */

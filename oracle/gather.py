@@ -76,9 +76,33 @@ def encode_rgbe(rgb: np.ndarray) -> np.ndarray:
 def _desc_type(base):
     class Desc(ctypes.Structure):
         _fields_ = list(base._fields_) + [("light_filter", ctypes.c_int32), ("no_skip", ctypes.c_int32),
-                                          ("face_style_filter", ctypes.c_void_p)]
+                                          ("face_style_filter", ctypes.c_void_p), ("skybox", ctypes.c_void_p)]
 
     return Desc
+
+
+def bsp_leaf_from_point(skybox, p) -> int:
+    """The oracle's CompiledBspTree.LeafFromPoint over a skybox.Skybox's BSP."""
+    L = _bind()
+    L.vc_bsp_leaf_from_point.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.vc_bsp_leaf_from_point.restype = ctypes.c_int32
+    desc = skybox.pack()
+    q = np.ascontiguousarray(p, np.float32).reshape(3)
+    return int(L.vc_bsp_leaf_from_point(ctypes.byref(desc), _p(q)))
+
+
+def sky_resolve(count: int, first_occ, inner, mask: int) -> np.ndarray:
+    """ResolveSkyTest's arithmetic over given answers: first_occ 4 first-ray
+    occlusions, inner (cameras, 4) skybox-ray occlusions, mask the lanes that
+    recurse; the 4 visible fractions."""
+    L = _bind()
+    L.vc_sky_resolve.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                 ctypes.c_int32, ctypes.c_void_p]
+    f = np.ascontiguousarray(first_occ, np.float32).reshape(4)
+    i = np.ascontiguousarray(inner, np.float32).reshape(-1, 4)
+    out = np.zeros(4, np.float32)
+    L.vc_sky_resolve(int(count), _p(f), len(i), _p(i), int(mask), _p(out))
+    return out
 
 
 def set_mutation(which: int) -> None:
@@ -111,6 +135,9 @@ def _prepare(casters, desc_and_keep, light_filter, no_skip, face_style_filter):
     if face_style_filter is not None:
         fsf = np.ascontiguousarray(face_style_filter, np.int32)
         d.face_style_filter = fsf.ctypes.data
+    sb = _keep.get("skybox_desc")
+    if sb is not None:  # the 3D skybox: vc_skybox has vh_skybox_desc's layout
+        d.skybox = ctypes.addressof(sb)
     tris = convert_triangles(casters.vertices, casters.ids)
     bvh = CpuBvh(casters.vertices, tris)
     return d, bvh, fsf

@@ -134,7 +134,7 @@ def _point_faces(fset, sel_face, points):
 
 def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames, light,
                 extra_passes: int = EXTRA_PASSES, sample_st=None, garbage: bool = False, which: int = 0,
-                threads=None, face_style=None):
+                threads=None, face_style=None, skybox=None):
     """The passes over one gather result ``light`` ((4, S, 4), the LDR or HDR
     result named by ``which``, which picks the resample's matching output).
     ``frames`` are the faces' luxel frames (final.FACE_DTYPE; width x height
@@ -143,8 +143,10 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
     one style slot's StyleSupersample (FaceLightJob.cs:688-700): per face
     the slot's light style, 255 where the face has no such slot (its samples
     are never selected); the resample then takes only that style's lights
-    (EmitResample, FaceLightJob.cs:486).  Returns (refined light, the number
-    of samples each pass resampled)."""
+    (EmitResample, FaceLightJob.cs:486).  ``skybox`` (skybox.Skybox) gives
+    the resample's sky tests their 3D-skybox stage, each point looking up its
+    own leaf.  Returns (refined light, the number of samples each pass
+    resampled)."""
     from hl2sdk_tools_sharp_amd.light import pack_desc
 
     light = np.array(light, np.float32, copy=True)
@@ -182,7 +184,7 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
         for pts, filt in ((direct, 2), (amb, 1)):
             sub = _point_faces(fset, face[sel], pts)
             sub.sample_cluster[:] = np.repeat(fset.sample_cluster[sel], pts.shape[1])
-            desc = pack_desc(sub, lights, light_pvs, sun_extent, sun_samples, sky_dirs)
+            desc = pack_desc(sub, lights, light_pvs, sun_extent, sun_samples, sky_dirs, skybox=skybox)
             fsf = None
             if face_style is not None:
                 sub_faces = np.unique(face[sel], return_index=True)
@@ -207,7 +209,7 @@ def supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_d
 
 def supersample_styles(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames, slots,
                        face_styles, extra_passes: int = EXTRA_PASSES, sample_st=None, garbage: bool = False,
-                       which: int = 0, threads=None):
+                       which: int = 0, threads=None, skybox=None):
     """Every style slot's passes: ``slots`` (4 slots, 4 normals, S, 4) the
     gather's light per slot, ``face_styles`` (F, 4) its style numbers.  Each
     slot is a StyleSupersample of its own (its own selection and passes).
@@ -219,7 +221,8 @@ def supersample_styles(casters, fset, lights, light_pvs, sun_extent, sun_samples
         if not np.any(fs != 255):
             continue
         out[k], counts = supersample(casters, fset, lights, light_pvs, sun_extent, sun_samples, sky_dirs, frames,
-                                     out[k], extra_passes, sample_st, garbage, which, threads, face_style=fs)
+                                     out[k], extra_passes, sample_st, garbage, which, threads, face_style=fs,
+                                     skybox=skybox)
         for i, n in enumerate(counts):
             if i < len(total):
                 total[i] += n

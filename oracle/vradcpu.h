@@ -126,6 +126,37 @@ void vc_bounce(int32_t P, const int64_t *row_ptr, const int32_t *src, const floa
 #define VC_EMIT_SKYLIGHT 3
 #define VC_EMIT_SKYAMBIENT 5
 
+/* A 3D skybox (Rad/Light/SkyCameras.cs; the BSP as CompiledBspTree.LeafFromPoint,
+ * Rad/Light/CompiledBspTree.cs, walks it): the cameras, as origin and 1 / scale,
+ * the nodes (a plane index and two children, >= 0 a node, < 0 the leaf
+ * -1 - child), the planes (normal, dist, type) and per leaf whether a sky
+ * test starting in it recurses.  The same layout as the product's
+ * vh_skybox_desc. */
+typedef struct vc_skybox {
+    int32_t ncameras;
+    const float *camera_origin;
+    const float *camera_world_to_sky;
+    int32_t nnodes;
+    const int32_t *node_plane;
+    const int32_t *node_children;
+    int32_t nplanes;
+    const float *plane_normal;
+    const float *plane_dist;
+    const int32_t *plane_type;
+    int32_t nleaves;
+    const uint8_t *leaf_recurses;
+} vc_skybox;
+
+/* CompiledBspTree.LeafFromPoint: the leaf index of p. */
+int32_t vc_bsp_leaf_from_point(const vc_skybox *s, const float p[3]);
+/* ResolveSkyTest's arithmetic (DirectLightGatherer.cs:661-714, Correct
+ * compliance) over given answers, as the reference's replay tests drive it:
+ * first_occ 4 first-ray occlusions (lanes past count are ignored and copy
+ * the last real one), inner ncam x 4 skybox-ray occlusions, mask the lanes
+ * that recurse; writes 4 visible fractions. */
+void vc_sky_resolve(int32_t count, const float *first_occ, int32_t ncam, const float *inner, int32_t mask,
+                    float *fraction);
+
 /* A DirectLight (Rad/Light/DirectLight.cs) as the gather reads it. */
 typedef struct vc_light {
     int32_t type, style;
@@ -161,6 +192,8 @@ typedef struct vc_gather_desc {
     /* EmitResample's style test (FaceLightJob.cs:486): per face, the light
      * style the resample lights (-1 or NULL: every style). */
     const int32_t *face_style_filter;
+    /* The 3D skybox behind the sky tests, or NULL (vh_gather_skybox). */
+    const vc_skybox *skybox;
 } vc_gather_desc;
 
 /* Radiosity.GatherLight for flat rows, FP32, row-parallel (vradcpu_bounce.c). */

@@ -10,6 +10,9 @@
  *   Rad/Light/FaceLightJob.cs  -- EmitGroup / ResolveGroup: 4-sample groups,
  *       the per-lane PVS mask, (dot * mask) * falloff, AddLight per normal
  *   Rad/Light/LightingValue.cs -- AddLight
+ *   Rad/Light/DirectLightGatherer.cs -- TestLineDoesHitSky's 3D-skybox
+ *       recursion (:510-714, Correct side), LaneRecurses (:1611-1621);
+ *       Rad/Light/CompiledBspTree.cs -- LeafFromPoint
  *   Rad/Light/DirectionalSampler.cs -- HaltonSequence, DirectionalSampler
  *   Rad/Ambient/StockLightColor.cs:136-185 -- Encode, CalcExponent
  * plus the build's own ambient-occlusion term (no reference counterpart).
@@ -136,6 +139,10 @@ enum {
     MUT_SUN_FORWARD = 11,     /* the sun's rays cast along its direction, not back */
     MUT_SUN_HALF_LENGTH = 12, /* the sun's rays half MaxTraceLength long */
     MUT_STYLE_MERGED = 13,    /* every style's light added to slot 0 */
+    MUT_SKYBOX_LANE_ZERO = 14, /* every lane recurses as lane zero does (stock's quirk) */
+    MUT_SKYBOX_UNSCALED = 15,  /* skybox rays start at origin + start, not scaled */
+    MUT_SKYBOX_NO_FULLY = 16,  /* skybox rays cast for a group whose first rays are all blocked */
+    MUT_SKYBOX_NOT_ADDED = 17, /* a skybox hit replaces the first ray's answer instead of adding */
 };
 static int g_mut = 0;
 
@@ -253,6 +260,129 @@ static int find_or_allocate_style(uint8_t *styles, int style)
     return -1;
 }
 
+/* ---- the 3D skybox: TestLineDoesHitSky's recursion (DirectLightGatherer.cs:510-714,
+ * Correct compliance), SkyCameras (SkyCameras.cs), CompiledBspTree.LeafFromPoint ---- */
+
+#define SKYBOX_MAX_CAMERAS 64
+
+/* CompiledBspTree.LeafFromPoint: from node 0, an axial plane (type < 3)
+ * compares the coordinate it names (0 x, 1 y, any other z), any other the
+ * dot product; a point on the plane goes to the front child (children[0]); a
+ * node index past the last node answers leaf 0.  The walk is cut after
+ * nnodes steps (-1), which only a cyclic node table reaches. */
+int32_t vc_bsp_leaf_from_point(const vc_skybox *s, const float p[3])
+{
+    int32_t node = 0;
+    for (int32_t step = 0; node >= 0; step++) {
+        if (node >= s->nnodes) return 0;
+        if (step > s->nnodes) return -1;
+        const int32_t pl = s->node_plane[node];
+        const int32_t type = s->plane_type[pl];
+        float dist;
+        if (type < 3)
+            dist = (type == 0 ? p[0] : (type == 1 ? p[1] : p[2])) - s->plane_dist[pl];
+        else
+            dist = dot3(s->plane_normal + 3 * (size_t)pl, p) - s->plane_dist[pl];
+        node = dist < 0.0f ? s->node_children[2 * (size_t)node + 1] : s->node_children[2 * (size_t)node];
+    }
+    return -1 - node;
+}
+
+/* DirectLightGatherer.LaneRecurses (:1611-1621) with the area test folded
+ * into the per-leaf flag. */
+static int lane_recurses(const vc_skybox *s, const float p[3])
+{
+    int32_t leaf = vc_bsp_leaf_from_point(s, p);
+    return leaf >= 0 && leaf < s->nleaves && s->leaf_recurses[leaf] != 0;
+}
+
+/* ResolveSkyTest's recursion (:661-706): occ holds the 4 lanes' first-ray
+ * occlusions (padding copies), inner ncam x 4 skybox answers; a group whose
+ * lanes are all blocked takes none. */
+static void sky_resolve(float occ[LANES], int32_t ncam, const float *inner, int mask)
+{
+    int fully = 1;
+    for (int lane = 0; lane < LANES; lane++) fully &= occ[lane] >= 1.0f;
+    if (mask == 0 || fully) return;
+    for (int32_t c = 0; c < ncam; c++)
+        for (int lane = 0; lane < LANES; lane++) {
+            float fraction = 1.0f - minps(maxps(inner[c * LANES + lane], 0.0f), 1.0f);
+            if ((mask >> lane) & 1) {
+                if (g_mut == MUT_SKYBOX_NOT_ADDED)
+                    occ[lane] = 1.0f - fraction;
+                else
+                    occ[lane] = occ[lane] + 1.0f - fraction;
+            }
+        }
+}
+
+void vc_sky_resolve(int32_t count, const float *first_occ, int32_t ncam, const float *inner, int32_t mask,
+                    float *fraction)
+{
+    float occ[LANES], in[SKYBOX_MAX_CAMERAS * LANES];
+    if (count < 1) count = 1;
+    if (count > LANES) count = LANES;
+    if (ncam > SKYBOX_MAX_CAMERAS) ncam = SKYBOX_MAX_CAMERAS;
+    for (int lane = 0; lane < LANES; lane++) occ[lane] = first_occ[lane < count ? lane : count - 1];
+    for (int32_t c = 0; c < ncam; c++)
+        for (int lane = 0; lane < LANES; lane++) in[c * LANES + lane] = inner[c * LANES + (lane < count ? lane : count - 1)];
+    sky_resolve(occ, ncam, in, mask);
+    for (int lane = 0; lane < LANES; lane++) fraction[lane] = 1.0f - minps(maxps(occ[lane], 0.0f), 1.0f);
+}
+
+/* TestLineDoesHitSky for one group (EmitSkyTest + ResolveSkyTest): every real
+ * lane's first ray p -> stop (LightRayLog.Occlusion: a hit short of the end
+ * that is not sky), then with a skybox the recursion: the lanes whose start
+ * recurses, unless every lane was blocked, cast per camera a ray from
+ * origin + start * world_to_sky along their first ray's direction normalised
+ * (Vec3.Normalise: three divides by the length; zero for a zero length),
+ * MaxTraceLength long (EmitRecursion, :587-607).  Only those lanes' skybox
+ * rays are traced and counted: the others' answers are never read.  occ gets
+ * the 4 lanes' occlusions, unclamped, padding copying the last real lane. */
+static void sky_test(gjob *j, float p[LANES][3], float stop[LANES][3], int count, float occ[LANES],
+                     int64_t *rays)
+{
+    for (int lane = 0; lane < count; lane++) {
+        float dir[3] = {stop[lane][0] - p[lane][0], stop[lane][1] - p[lane][1], stop[lane][2] - p[lane][2]};
+        occ[lane] = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
+        ++*rays;
+    }
+    for (int lane = count; lane < LANES; lane++) occ[lane] = occ[count - 1];
+    const vc_skybox *s = j->d->skybox;
+    if (!s || s->ncameras <= 0) return;
+    const int32_t ncam = s->ncameras < SKYBOX_MAX_CAMERAS ? s->ncameras : SKYBOX_MAX_CAMERAS;
+    int mask = 0;
+    for (int lane = 0; lane < LANES; lane++)
+        if (lane_recurses(s, g_mut == MUT_SKYBOX_LANE_ZERO ? p[0] : p[lane])) mask |= 1 << lane;
+    int fully = 1;
+    for (int lane = 0; lane < LANES; lane++) fully &= occ[lane] >= 1.0f;
+    if (mask == 0 || (fully && g_mut != MUT_SKYBOX_NO_FULLY)) return;
+    float u[LANES][3];
+    for (int lane = 0; lane < count; lane++) {
+        float v[3] = {stop[lane][0] - p[lane][0], stop[lane][1] - p[lane][1], stop[lane][2] - p[lane][2]};
+        float len = sqrtf((v[0] * v[0]) + (v[1] * v[1]) + (v[2] * v[2]));
+        for (int k = 0; k < 3; k++) u[lane][k] = len == 0.0f ? 0.0f : v[k] / len;
+    }
+    float inner[SKYBOX_MAX_CAMERAS * LANES];
+    for (int32_t c = 0; c < ncam; c++) {
+        const float *o = s->camera_origin + 3 * (size_t)c;
+        const float w = g_mut == MUT_SKYBOX_UNSCALED ? 1.0f : s->camera_world_to_sky[c];
+        for (int lane = 0; lane < count; lane++) {
+            inner[c * LANES + lane] = 0.0f;
+            if (!((mask >> lane) & 1)) continue;
+            float ss[3], dir[3];
+            for (int k = 0; k < 3; k++) {
+                ss[k] = o[k] + (p[lane][k] * w);
+                dir[k] = (ss[k] + (u[lane][k] * MAX_TRACE_LENGTH)) - ss[k];
+            }
+            inner[c * LANES + lane] = vc_bvh_sky_occluded(j->bvh, ss, dir, 1.0f) ? 1.0f : 0.0f;
+            ++*rays;
+        }
+        for (int lane = count; lane < LANES; lane++) inner[c * LANES + lane] = inner[c * LANES + count - 1];
+    }
+    sky_resolve(occ, ncam, inner, mask);
+}
+
 /* One SampleGroup: EmitGroup + ResolveGroup (FaceLightJob.cs:329-463) for
  * the four samples of face f from `first`, adding into the face's style
  * slots `styles` as lights first touch it. */
@@ -333,14 +463,11 @@ static void gather_group(gjob *j, int32_t f, int32_t first, uint8_t *styles, int
                         delta[1] = delta[1] + (jt[1] * jitter_scale);
                         delta[2] = delta[2] + (jt[2] * jitter_scale);
                     }
-                    float occ[LANES];
-                    for (int lane = 0; lane < count; lane++) {
-                        float stop[3] = {delta[0] + p[lane][0], delta[1] + p[lane][1], delta[2] + p[lane][2]};
-                        float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
-                        occ[lane] = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
-                        ++*rays;
-                    }
-                    for (int lane = count; lane < LANES; lane++) occ[lane] = occ[count - 1];
+                    /* the sun traces every real lane (:780-790) */
+                    float occ[LANES], stop[LANES][3];
+                    for (int lane = 0; lane < LANES; lane++)
+                        for (int k = 0; k < 3; k++) stop[lane][k] = delta[k] + p[lane][k];
+                    sky_test(j, p, stop, count, occ, rays);
                     for (int lane = 0; lane < LANES; lane++) {
                         float o = minps(maxps(occ[lane], 0.0f), 1.0f);
                         total[lane] += 1.0f - o;
@@ -383,23 +510,34 @@ static void gather_group(gjob *j, int32_t f, int32_t first, uint8_t *styles, int
                     }
                 }
                 float frac[LANES];
-                for (int lane = 0; lane < LANES; lane++) {
-                    if (lane >= count) {
-                        frac[lane] = frac[count - 1];
-                        continue;
-                    }
-                    int need = tr[lane] && ((mask >> lane) & 1);
-                    if (need) {
-                        float stop[3] = {(an[0] * -MAX_TRACE_LENGTH) + p[lane][0],
-                                         (an[1] * -MAX_TRACE_LENGTH) + p[lane][1],
-                                         (an[2] * -MAX_TRACE_LENGTH) + p[lane][2]};
-                        float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
-                        float occ = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
-                        ++*rays;
-                        if (g_mut == MUT_SKY_AMBIENT_OPEN) occ = 0.0f;
-                        frac[lane] = 1.0f - occ;
-                    } else {
-                        frac[lane] = 1.0f;
+                const vc_skybox *sb = d->skybox;
+                if (sb && sb->ncameras > 0) {
+                    /* EmitAmbientRays / ResolveAmbientRays with traceAll
+                     * (:1014-1058): every real lane through the sky test */
+                    float occ[LANES], stop[LANES][3];
+                    for (int lane = 0; lane < LANES; lane++)
+                        for (int k = 0; k < 3; k++) stop[lane][k] = (an[k] * -MAX_TRACE_LENGTH) + p[lane][k];
+                    sky_test(j, p, stop, count, occ, rays);
+                    for (int lane = 0; lane < LANES; lane++) frac[lane] = 1.0f - minps(maxps(occ[lane], 0.0f), 1.0f);
+                } else {
+                    for (int lane = 0; lane < LANES; lane++) {
+                        if (lane >= count) {
+                            frac[lane] = frac[count - 1];
+                            continue;
+                        }
+                        int need = tr[lane] && ((mask >> lane) & 1);
+                        if (need) {
+                            float stop[3] = {(an[0] * -MAX_TRACE_LENGTH) + p[lane][0],
+                                             (an[1] * -MAX_TRACE_LENGTH) + p[lane][1],
+                                             (an[2] * -MAX_TRACE_LENGTH) + p[lane][2]};
+                            float dir[3] = {stop[0] - p[lane][0], stop[1] - p[lane][1], stop[2] - p[lane][2]};
+                            float occ = vc_bvh_sky_occluded(j->bvh, p[lane], dir, 1.0f) ? 1.0f : 0.0f;
+                            ++*rays;
+                            if (g_mut == MUT_SKY_AMBIENT_OPEN) occ = 0.0f;
+                            frac[lane] = 1.0f - occ;
+                        } else {
+                            frac[lane] = 1.0f;
+                        }
                     }
                 }
                 for (int n = 0; n < normals; n++)

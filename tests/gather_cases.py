@@ -262,6 +262,96 @@ def fact_bump_dot_zeroed_with_the_flat_dot(run):  # :297-312
     return not np.any(o.ldr[0, :, 0, :3])
 
 
+# --- the 3D skybox (TestLineDoesHitSky's recursion, DirectLightGatherer.cs:510-714;
+# SkyRecursionTests, GatherTests.cs:361-555), on rooms.skybox_grid ---
+
+def _skybox_case(closed=False, open_skybox=False):
+    from hl2sdk_tools_sharp_amd import rooms
+
+    m = rooms.skybox_grid(closed=closed, open_skybox=open_skybox)
+    return m, rooms.room_faces(m.grid, luxel=32.0)
+
+
+def _sun_only(m, normal):
+    from hl2sdk_tools_sharp_amd import rooms
+
+    L, pvs = rooms.room_lights(m.grid, sun=True)
+    k = len(m.grid.lights)
+    L, pvs = L[k:k + 1].copy(), pvs[k:k + 1].copy()
+    L[0]["normal"] = normal
+    return L, pvs
+
+
+def fact_skybox_roof_shades_its_rooms(run):
+    """A vertical sun over sky ceilings: every floor sample's first ray sees
+    sky, so a sample whose leaf recurses is shaded exactly where its point,
+    scaled into a camera's skybox, lies under that skybox's roof -- room (0, 0)
+    in skybox 0, the last room in skybox 1 -- and keeps its sunlight
+    elsewhere, as the samples of the corner leaf (the cameras' own area)
+    do under the roof."""
+    m, f = _skybox_case()
+    L, pvs = _sun_only(m, (0.0, 0.0, -1.0))
+    base = run(f, L, pvs=pvs, casters=m.casters).ldr[0, 0, :, 3]
+    got = run(f, L, pvs=pvs, casters=m.casters, skybox=m.skybox).ldr[0, 0, :, 3]
+    g = m.grid
+    p = f.sample_pos + f.face_normals[np.repeat(np.arange(f.nfaces), np.diff(f.face_sample_ptr)), 0]
+    rec = np.array([m.skybox.leaf_recurses[m.tree.leaf_from_point(q)] for q in p], bool)
+    last = len(g.room_lo) - 1
+    under = (np.all((p[:, :2] >= g.room_lo[0, :2]) & (p[:, :2] <= g.room_hi[0, :2]), axis=1)
+             | np.all((p[:, :2] >= g.room_lo[last, :2]) & (p[:, :2] <= g.room_hi[last, :2]), axis=1))
+    want = np.where(rec & under, np.float32(0.0), base)
+    lit_under = (base > 0) & under
+    return (np.array_equal(got, want) and np.any(lit_under & rec) and np.any(lit_under & ~rec)
+            and np.any(base[~under] > 0))
+
+
+def fact_skybox_only_adds_occlusion(run):
+    """The skybox rays can only add to a lane's occlusion: with a slanted,
+    jittered sun, sky ambient and point lights, no output of any sample
+    grows, and some fall."""
+    from hl2sdk_tools_sharp_amd import rooms
+
+    m, f = _skybox_case()
+    L, pvs = rooms.room_lights(m.grid, sun=True, sky_ambient=True)
+    sun = np.array([0.15, 0.1, 1.0], np.float32)
+    L[len(m.grid.lights)]["normal"] = -sun / np.float32(np.linalg.norm(sun))
+    kw = dict(pvs=pvs, casters=m.casters, sun_samples=light.SUN_AREA_LIGHT_SAMPLES, sun_extent=0.05,
+              sky_dirs=light.sky_directions())
+    a = run(f, L, **kw)
+    b = run(f, L, skybox=m.skybox, **kw)
+    return bool(np.all(b.ldr <= a.ldr) and np.all(b.hdr <= a.hdr) and np.any(b.ldr < a.ldr))
+
+
+def fact_skybox_closed_world_casts_no_skybox_rays(run):
+    """Opaque ceilings: every group's first rays are all blocked, so no
+    skybox ray is cast (AFullyOccludedDeferredTestTracesNoSkyboxRays) -- the
+    rays equal those of a skybox no leaf recurses into, and so does the light."""
+    from hl2sdk_tools_sharp_amd import rooms
+    from hl2sdk_tools_sharp_amd.skybox import Skybox
+
+    m, f = _skybox_case(closed=True)
+    L, pvs = rooms.room_lights(m.grid, sun=True, sky_ambient=True)
+    kw = dict(pvs=pvs, casters=m.casters, sky_dirs=light.sky_directions())
+    none = Skybox(m.skybox.camera_origin, m.skybox.camera_world_to_sky, m.tree,
+                  np.zeros_like(m.skybox.leaf_recurses))
+    a = run(f, L, skybox=none, **kw)
+    b = run(f, L, skybox=m.skybox, **kw)
+    return a.rays == b.rays and np.array_equal(a.ldr, b.ldr) and a.rays > 0
+
+
+def fact_skybox_open_changes_nothing(run):
+    """Skyboxes with nothing in them but sky: every skybox ray is answered
+    sky, and the light is the light without skyboxes, bit for bit."""
+    from hl2sdk_tools_sharp_amd import rooms
+
+    m, f = _skybox_case(open_skybox=True)
+    L, pvs = rooms.room_lights(m.grid, sun=True, sky_ambient=True)
+    kw = dict(pvs=pvs, casters=m.casters, sky_dirs=light.sky_directions())
+    a = run(f, L, **kw)
+    b = run(f, L, skybox=m.skybox, **kw)
+    return np.array_equal(a.ldr.view(np.uint32), b.ldr.view(np.uint32)) and b.rays > a.rays
+
+
 FACTS = [
     (fact_point_overhead_unit_dot, None),
     (fact_light_behind_asks_for_no_ray, 5),
@@ -282,6 +372,11 @@ FACTS = [
     (fact_closed_sky_ambient_nothing, 9),
     (fact_padding_lanes_not_traced, 6),
     (fact_bump_dot_zeroed_with_the_flat_dot, 4),
+    (fact_skybox_roof_shades_its_rooms, 14),
+    (fact_skybox_roof_shades_its_rooms, 15),
+    (fact_skybox_only_adds_occlusion, 17),
+    (fact_skybox_closed_world_casts_no_skybox_rays, 16),
+    (fact_skybox_open_changes_nothing, None),
 ]
 
 

@@ -63,7 +63,8 @@ def gpu_run(faces, L, **kw):
     with tr:
         fl.load(faces, L, light_pvs=kw.get("pvs"), sun_extent=kw.get("sun_extent", 0.0),
                 sun_samples=kw.get("sun_samples", 1), sky_dirs=kw.get("sky_dirs"),
-                ao_count=kw.get("ao_count", 0), ao_radius=kw.get("ao_radius", 40.0))
+                ao_count=kw.get("ao_count", 0), ao_radius=kw.get("ao_radius", 40.0),
+                skybox=kw.get("skybox"))
         before = tr.stats()["rays_gather"]
         got = fl.light_faces()
         rays = tr.stats()["rays_gather"] - before

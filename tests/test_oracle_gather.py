@@ -110,7 +110,8 @@ def run(faces, L, **kw):
 def run_full(faces, L, **kw):
     """The gather with style slots and the ray count."""
     d = light.pack_desc(faces, L, kw.get("pvs"), kw.get("sun_extent", 0.0), kw.get("sun_samples", 1),
-                        kw.get("sky_dirs"), kw.get("ao_count", 0), kw.get("ao_radius", 40.0))
+                        kw.get("sky_dirs"), kw.get("ao_count", 0), kw.get("ao_radius", 40.0),
+                        kw.get("skybox"))
     return og.gather_styles(kw.get("casters", far_scene()), d, faces.nsamples)
 
 
