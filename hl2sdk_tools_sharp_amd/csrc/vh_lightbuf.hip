@@ -296,7 +296,9 @@ __device__ __forceinline__ bool tri_test_record(const float4 pl, const float4 *_
     odotn = (r.oy * pl.y) + odotn;
     odotn = (r.oz * pl.z) + odotn;
     const float num = pl.w - odotn;
-    const bool may = (fabsf(ddotn) > kParallelEps) & (num * ddotn > 0.0f) & (fabsf(num) <= fabsf(ddotn));
+    // the three screens evaluated together (no short circuit)
+    const bool may = ((int)(fabsf(ddotn) > kParallelEps) & (int)(num * ddotn > 0.0f) &
+                      (int)(fabsf(num) <= fabsf(ddotn))) != 0;
     bool hit = false;
     if (may) {
         const float4 e0 = edge[0], e1 = edge[1];
