@@ -44,7 +44,11 @@ FLOP_PER_TRI = 27
 # entry read the depth compare (1), besides the triangle tests.
 FLOP_PER_LOOKUP = 15
 FLOP_PER_ENTRY = 1
-PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (packed FMA) peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 peak (the same figure for the vector and the matrix pipes)
+# the pipe the traversal kernels actually issue on: their FP32 work is vector
+# FMA, min/max and compares, with no matrix work; the roofline's "bound" names
+# the compute roof ("mfma") and this field says which unit meets it
+FP32_PIPE = "valu"
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
 
@@ -260,7 +264,8 @@ def batching_leg(a, log):
     flops = nv.value * FLOP_PER_NODE + nt.value * FLOP_PER_TRI
     ach = flops / (q / 1e3) / 1e12
     lanes = (nv.value + nt.value) / max(64 * (ns.value + ts.value), 1)
-    roofline = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "mfma", "pipe": FP32_PIPE, "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_facelight<true>",
                 "kernel_ms": round(q, 3), "flop_per_launch": flops,
                 "node_visits_per_ray": round(nv.value / max(rays, 1), 3),
@@ -907,7 +912,8 @@ def main(argv=None) -> int:
         flops = nv * FLOP_PER_NODE + nt * FLOP_PER_TRI
     achieved = flops / (kernel_ms / 1e3) / 1e12
     roofline = {
-        "bound": "valu",
+        "bound": "mfma",
+        "pipe": FP32_PIPE,
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
