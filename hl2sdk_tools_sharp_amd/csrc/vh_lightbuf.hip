@@ -369,24 +369,44 @@ __global__ __launch_bounds__(kBlock) void k_point_lights_lb(
             const uint32_t end = off[(cell + 1) * kImageParts];
             const uint32_t dbits = __float_as_uint(depth);
             // each step loads the next entry's depth and plane before testing
-            // this one, so the loads of consecutive steps overlap
+            // this one, so the loads of consecutive steps overlap; the loop is
+            // unrolled twice, so the two register sets take turns (entry j in
+            // A while j + 1 loads into B, then the other way) and no step
+            // moves one into the other
             uint32_t j = off[cell * kImageParts];
-            uint32_t dj = j < end ? rdepth[j] : 0xffffffffu;
-            float4 pl = j < end ? rplane[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            while (j < end) {
-                const uint32_t jn = j + 1 < end ? j + 1 : j;
-                const uint32_t dn = rdepth[jn];
-                const float4 pn = rplane[jn];
+            uint32_t dA = j < end ? rdepth[j] : 0xffffffffu;
+            float4 pA = j < end ? rplane[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t dB;
+            float4 pB;
+            for (;;) {
+                if (j >= end) break;
+                {
+                    const uint32_t jn = j + 1 < end ? j + 1 : j;
+                    dB = rdepth[jn];
+                    pB = rplane[jn];
+                }
                 if (COUNT) ++steps;
-                if (dj >= dbits) break;  // this and every later triangle starts at or past p
+                if (dA >= dbits) break;  // this and every later triangle starts at or past p
                 if (COUNT) ++ntest;
-                if (tri_test_record(pl, redge + (size_t)j * 2, r, tmin)) {
+                if (tri_test_record(pA, redge + (size_t)j * 2, r, tmin)) {
                     blocked = true;
                     break;
                 }
                 j++;
-                dj = dn;
-                pl = pn;
+                if (j >= end) break;
+                {
+                    const uint32_t jn = j + 1 < end ? j + 1 : j;
+                    dA = rdepth[jn];
+                    pA = rplane[jn];
+                }
+                if (COUNT) ++steps;
+                if (dB >= dbits) break;
+                if (COUNT) ++ntest;
+                if (tri_test_record(pB, redge + (size_t)j * 2, r, tmin)) {
+                    blocked = true;
+                    break;
+                }
+                j++;
             }
         }
         if (COUNT) {
