@@ -372,7 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_point_lights_lb(
             // this one, so the loads of consecutive steps overlap; the loop is
             // unrolled twice, so the two register sets take turns (entry j in
             // A while j + 1 loads into B, then the other way) and no step
-            // moves one into the other
+            // moves one into the other.  Entry j + 1 is loaded only once
+            // j < end <= total, so it is at most entry `total`: the plane
+            // array is followed by the edges and the depth array by the
+            // buffer's tail, and a value past the list is never used.
             uint32_t j = off[cell * kImageParts];
             uint32_t dA = j < end ? rdepth[j] : 0xffffffffu;
             float4 pA = j < end ? rplane[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -380,11 +383,8 @@ __global__ __launch_bounds__(kBlock) void k_point_lights_lb(
             float4 pB;
             for (;;) {
                 if (j >= end) break;
-                {
-                    const uint32_t jn = j + 1 < end ? j + 1 : j;
-                    dB = rdepth[jn];
-                    pB = rplane[jn];
-                }
+                dB = rdepth[j + 1];
+                pB = rplane[j + 1];
                 if (COUNT) ++steps;
                 if (dA >= dbits) break;  // this and every later triangle starts at or past p
                 if (COUNT) ++ntest;
@@ -394,11 +394,8 @@ __global__ __launch_bounds__(kBlock) void k_point_lights_lb(
                 }
                 j++;
                 if (j >= end) break;
-                {
-                    const uint32_t jn = j + 1 < end ? j + 1 : j;
-                    dA = rdepth[jn];
-                    pA = rplane[jn];
-                }
+                dA = rdepth[j + 1];
+                pA = rplane[j + 1];
                 if (COUNT) ++steps;
                 if (dB >= dbits) break;
                 if (COUNT) ++ntest;
