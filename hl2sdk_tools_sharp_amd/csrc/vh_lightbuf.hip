@@ -113,12 +113,12 @@ struct Lists {
 };
 
 constexpr int kImageThreads = 256;
-constexpr int kImageParts = 4;    // workgroups per (light, face) image, each over a quarter of the triangles
+constexpr int kImageParts = 4;    // workgroups per (light, face) image, each over every fourth triangle
 constexpr int kInlineCells = 32;  // rectangles with more cells go to the workgroup's shared list
 constexpr int kBigCap = 256;
 
 // kImageParts workgroups per (light, cube face) image, each over its share of
-// the triangles with the image's N x N cell counters in LDS: FILL false counts
+// the triangles (every kImageParts-th) with the image's N x N cell counters in LDS: FILL false counts
 // each cell's triangles and stores the counts as count[(image cell) * parts +
 // part]; FILL true starts each cell's cursor at that slot's offset (after an
 // exclusive scan) and writes the entries, so a cell's list is its parts' runs
@@ -184,9 +184,10 @@ __global__ __launch_bounds__(kImageThreads) void k_lb_image(
             }
         }
     };
-    const int32_t k0 = (int32_t)(((int64_t)ntri * part) / kImageParts);
-    const int32_t k1 = (int32_t)(((int64_t)ntri * (part + 1)) / kImageParts);
-    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += blockDim.x) {
+    // the parts take the triangles in turn (k = part mod kImageParts): the
+    // triangles are in leaf order, so a light's near ones lie together, and
+    // dealt out this way they load the image's workgroups evenly
+    for (int32_t k = part + kImageParts * (int32_t)threadIdx.x; k < ntri; k += kImageParts * (int32_t)blockDim.x) {
         LbCells pc;
         if (!pair_cells(boxes, tv, lights, l, k, a, side, N, pmax, scene_mag, pad, pc)) continue;
         if ((pc.iu1 - pc.iu0 + 1) * (pc.iw1 - pc.iw0 + 1) > kInlineCells) {
