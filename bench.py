@@ -707,6 +707,31 @@ def bake_leg(a, log, model):
     out["speedup_vs_plain_batched"] = batched.wall_s / fast.wall_s
     out["lump_rel_l1_optimised_vs_plain"] = lump_rel_l1(fast.lump, plain.lump)
     out["plain_batched_lump_equals_plain"] = bool(np.array_equal(batched.lump, plain.lump))
+    # the bounce alone on this map from each start, FP16 weights as in the
+    # optimised form: zero, the direct-light prior (gain x direct light), the
+    # network's prediction
+    from hl2sdk_tools_sharp_amd.mlp import WarmStartMLP
+
+    offer = HipTracerFactory(device=0).try_create(g.casters)
+    with offer.tracer as tr:
+        st = HipBounceStage(tr, g)
+        st.build_transfers()
+        net = WarmStartMLP(tr, params=model["params"], gain=model["gain"], scale=model["scale"])
+        e0 = np.ascontiguousarray(plain.direct, np.float32)
+        leaf = (g.leaf & ~g.sky.astype(bool))[:, None]
+        starts = {}
+        for start in ("zero", "direct", "network"):
+            t0 = time.perf_counter()
+            if start == "network":
+                net.predict(e0, read=False)
+                r = st.bounce(e0, tol=1e-4, max_iter=300, weights="f16", from_prediction=True)
+            else:
+                x0 = np.zeros_like(e0) if start == "zero" else (model["gain"] * e0 * leaf).astype(np.float32)
+                r = st.bounce(e0, start=x0, tol=1e-4, max_iter=300, weights="f16")
+            starts[start] = {"iterations": r.iterations, "wall_s": time.perf_counter() - t0}
+        out["bounce_starts_f16"] = starts
+        log("[bench] bake map bounce (FP16) from zero / direct prior / network: "
+            + " / ".join(f"{v['iterations']} it {v['wall_s'] * 1e3:.1f} ms" for v in starts.values()))
     if not a.no_cpu:
         offer = HipTracerFactory(device=0).try_create(g.casters)
         with offer.tracer as tr:
